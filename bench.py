@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark: Bellard terms/s at hex position N = 10^12 on one MI355X.
+
+One step = the complete normal-mode computation at N = 10^12 (d = 399,999,999,999): the seven
+formulas' head terms k in [0, d) in one persistent launch of the precision-split (fp64 exact)
+kernel, the 7 x 20 tail terms, and the combine.  value = d / (seconds per step), one "term" being
+one k over all seven formulas (seven modular exponentiations).
+
+Also reported, outside the timed steps:
+  * the plain all-integer kernel and the split kernel on the same seeded windows (64 windows of
+    2^22 consecutive k per formula, starts from SplitMix64 seed 2), bit-compared;
+  * the same windows summed by the CPU oracle on a bounded sample, bit-compared, and its rate
+    (cpu_baseline, kind "port");
+  * roofline of the dominant kernel (the head-term kernel) from its HIP-event launch time.
+
+    python bench.py [--gpus 1] [--steps 2] [--warmup 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Bellard terms/sec (7 modexps each) at hex pos 10^12; frac err vs exact oracle"
+POSITION = 10**12
+PEAK_FP64_TFLOPS = 78.6  # MI355X published FP64 peak (one FMA = 2 flops per lane per cycle)
+NUMER_SUM = 32 + 1 + 256 + 64 + 4 + 4 + 1
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def splitmix64(seed):
+    x = seed & 0xFFFFFFFFFFFFFFFF
+    while True:
+        x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        yield z ^ (z >> 31)
+
+
+def sum_bitlen_10x(d):
+    """sum over x = 1..d of bitlen(10 x): the exponents 10 (d - k) of the head terms."""
+    total, b = 0, 1
+    while True:
+        lo = max(1, -(-(1 << (b - 1)) // 10))
+        hi = min(d, ((1 << b) - 1) // 10)
+        if lo > d:
+            break
+        if hi >= lo:
+            total += b * (hi - lo + 1)
+        b += 1
+    return total
+
+
+def algorithmic_fp64_slots(d):
+    """FP64 issue slots one normal-mode launch needs at minimum: per head term and formula,
+    6 per modular square (exact product hi/lo, quotient estimate, rounding, residue, sum) for
+    bitlen(e) - 1 squares, plus 16 for the four 48-bit quotient digits."""
+    squares = sum_bitlen_10x(d) - d
+    return 7 * (6 * squares + 16 * d)
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t.get("head_kernel_hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--position", type=int, default=POSITION)
+    ap.add_argument("--windows", type=int, default=64)
+    ap.add_argument("--window-log2", type=int, default=22)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the digit "
+              "extraction shards by term ranges, see DESIGN.md", file=sys.stderr)
+        return 2
+
+    import clbellard_amd as clb
+
+    n_pos, d, _ = clb.normalize(args.position)
+    ctx = clb.Context(0, clb.PREC_U192, kernel=clb.KERNEL_SPLIT)
+    log(f"device {clb.device_name(0)}; N={n_pos} d={d}")
+
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        ctx.normal(n_pos)
+        log(f"warmup {i}: {time.perf_counter() - t0:.2f}s")
+
+    head_ms, step_s, result = [], [], None
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        frac, hexs, _ = ctx.normal(n_pos)
+        dt = time.perf_counter() - t0
+        h, tot = ctx.last_timing()
+        head_ms.append(h)
+        step_s.append(dt)
+        if result is not None and result != frac:
+            raise SystemExit("nondeterministic result across steps")
+        result = frac
+        log(f"step {i}: {dt:.3f}s (head kernel {h / 1e3:.3f}s) digits {hexs}")
+    total_s = sum(step_s)
+    ms_per_step = 1e3 * total_s / len(step_s)
+    value = d * len(step_s) / total_s
+    head_avg_s = sum(head_ms) / len(head_ms) / 1e3
+    slots = algorithmic_fp64_slots(d)
+    achieved = 2.0 * slots / head_avg_s / 1e12  # FMA-equivalent TFLOP/s
+
+    # seeded windows: plain vs split, GPU vs oracle
+    rng = splitmix64(2)
+    span = 1 << args.window_log2
+    windows = [next(rng) % (d - span) for _ in range(args.windows)]
+    jobs = [(d, s, s + span, j) for j in range(7) for s in windows]
+    win_terms = span * len(windows)
+    ctx.set_kernel(clb.KERNEL_SPLIT)
+    split_parts = ctx.run_jobs(jobs)
+    t0 = time.perf_counter()
+    split_parts = ctx.run_jobs(jobs)
+    split_win_s = ctx.last_timing()[1] / 1e3
+    ctx.set_kernel(clb.KERNEL_PLAIN)
+    plain_parts = ctx.run_jobs(jobs)
+    plain_win_s = ctx.last_timing()[1] / 1e3
+    if plain_parts != split_parts:
+        raise SystemExit("plain and split kernels disagree on the seeded windows")
+    log(f"windows: split {win_terms / split_win_s:.3e} terms/s, plain "
+        f"{win_terms / plain_win_s:.3e} terms/s, bit-identical")
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from clb_testlib import load_oracle, w2i, SHIFT  # the checker, timed as the baseline
+        orc = load_oracle()
+        threads = min(16, os.cpu_count() or 1)
+        # sample: the first window of each formula, shortened to fit the CPU time budget
+        probe = 1 << 14
+        t0 = time.perf_counter()
+        orc.range_sum(d, windows[0], windows[0] + probe, 0, threads=threads)
+        per_term = (time.perf_counter() - t0) / probe
+        sample = max(probe, min(span, int(args.cpu_seconds / 7 / max(per_term, 1e-9))))
+        sample_jobs = [(d, windows[0], windows[0] + sample, j) for j in range(7)]
+        ctx.set_kernel(clb.KERNEL_SPLIT)
+        gpu_sample = ctx.run_jobs(sample_jobs)
+        t0 = time.perf_counter()
+        ok = True
+        for j, g in enumerate(gpu_sample):
+            s = orc.range_sum(d, windows[0], windows[0] + sample, j, threads=threads)
+            ok = ok and ((s << SHIFT[j]) & ((1 << 192) - 1)) == w2i(g)
+        cpu_s = time.perf_counter() - t0
+        if not ok:
+            raise SystemExit("GPU and CPU oracle disagree on the sample window")
+        cpu = {"value": sample / cpu_s, "unit": "terms/s", "cores": threads, "kind": "port",
+               "sample": f"k in [{windows[0]}, {windows[0] + sample}) of N=10^12, all 7 formulas, "
+                         f"bit-identical to the GPU sum"}
+        log(f"cpu oracle: {sample / cpu_s:.3e} terms/s on {threads} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "terms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"normal-mode hex digit extraction at N={n_pos}: 7 formulas x {d} head "
+                        "terms + 140 tail terms + combine, one persistent launch",
+            "position": n_pos, "d": d, "kernel": "split (fp64 exact, 3-bit windows)",
+            "digits": clb.hex48(result),
+            "frac_err_bound_log2": round(-192 + (NUMER_SUM * (d + 20)).bit_length(), 1),
+        },
+        "roofline": {
+            "bound": "mfma", "pipe": "VALU FP64 (no MFMA; FP64 peak applies to both)",
+            "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS, "traffic": load_traffic(),
+            "head_kernel_ms": 1e3 * head_avg_s, "algorithmic_fp64_slots": slots,
+        },
+        "plain_kernel": {"value": win_terms / plain_win_s, "unit": "terms/s",
+                         "windows": f"{args.windows} x 2^{args.window_log2} k per formula, seed 2"},
+        "split_kernel_windows": {"value": win_terms / split_win_s, "unit": "terms/s"},
+        "split_vs_plain": plain_win_s / split_win_s,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

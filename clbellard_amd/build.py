@@ -1,0 +1,80 @@
+"""Build libclbellard.so (HIP kernels + C ABI host) and the clbellard CLI for gfx950.
+
+    python -m clbellard_amd.build            # library + CLI
+Outputs are written next to this file (in-tree) so they travel with the source tree.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libclbellard.so")
+CLI = os.path.join(HERE, "clbellard")
+
+ARCH = os.environ.get("CLB_OFFLOAD_ARCH", "gfx950")
+
+# -ffp-contract=off: every fused multiply-add in the kernels is written explicitly; the exact
+# FP64 engine relies on the other products and sums being rounded on their own.
+# -packed-fp32-ops off: keep the compiler from emitting packed FP32 instructions.
+HIP_FLAGS = [
+    "-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result",
+]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd) -> None:
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(line for line in proc.stdout.splitlines()
+                    if "packed-fp32-ops' is not a recognized feature" not in line)
+    if proc.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
+    if out.strip():
+        print(out)
+
+
+def build_library() -> str:
+    srcs = [os.path.join(CSRC, f) for f in ("clb_kernels.hip", "clb_host.cpp")]
+    tmp = LIB + ".tmp"
+    _run([_hipcc(), *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_cli() -> str:
+    src = os.path.join(CSRC, "clb_cli.cpp")
+    tmp = CLI + ".tmp"
+    _run([_hipcc(), "-O2", "-std=c++17", "-o", tmp, src, f"-L{HERE}", "-lclbellard",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, CLI)
+    return CLI
+
+
+def build_oracle() -> str:
+    """The CPU oracle used by tests and bench.py's cpu_baseline (not part of the product)."""
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    return os.path.join(ROOT, "oracle", "liboracle.so")
+
+
+def main(argv=None) -> int:
+    build_library()
+    if os.path.exists(os.path.join(CSRC, "clb_cli.cpp")):
+        build_cli()
+    print(f"built {LIB}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

@@ -1,0 +1,191 @@
+// clb_device.hpp -- gfx950 device arithmetic for the Bellard term kernels.
+//
+// Two exact engines compute, for a modulus m and exponent e, the 192-bit quotient
+//   Q = floor(r * 2^192 / m) mod 2^192,   r = 2^e mod m,
+// which is the per-term value of the reference's all-integer kernel (Bellard_div2by1.cl:329-341,
+// with the sign applied later, see clb_kernels.hip).
+//
+//  * fp64 engine ("split"): every intermediate is an exact integer held in a double.  A modular
+//    square uses the FMA error-free product (hi = x*y, lo = fma(x, y, -hi)), a quotient estimate
+//    from a double reciprocal of m, and an exact FMA residue.  Residues stay in balanced form
+//    |x| <= 0.75 m, so no correction step is needed inside the powering loop.  Valid while
+//    m * 2^c <= 2^50, c = the window's largest shift (checked by the host).
+//  * int engine ("plain"): 64-bit Montgomery (R = 2^64) built from 32x32->64 products, lazy
+//    residues in [0, 2m), exponent windows folded into the REDC input, and the quotient words
+//    obtained by exact division (Q word = s * (-m^-1) mod 2^64 for s = r 2^(64i) mod m).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace clb {
+
+// ------------------------------------------------------------------ 192-bit helpers (LSW first)
+
+struct U192 {
+  uint64_t w0, w1, w2;  // value = w2 * 2^128 + w1 * 2^64 + w0
+};
+
+__device__ __forceinline__ void add192(U192& a, uint64_t b0, uint64_t b1, uint64_t b2) {
+  uint64_t s0 = a.w0 + b0;
+  uint64_t c0 = s0 < b0;
+  uint64_t s1 = a.w1 + b1;
+  uint64_t c1 = s1 < b1;
+  s1 += c0;
+  c1 += s1 < c0;
+  a.w2 = a.w2 + b2 + c1;
+  a.w1 = s1;
+  a.w0 = s0;
+}
+
+// a = -a - n  (mod 2^192): the sum of n negative terms -Q - 1 given a = sum of their Q.
+__device__ __forceinline__ void neg192_minus(U192& a, uint64_t n) {
+  // ~a = -a - 1, so -a - n = ~a - (n - 1) = ~a + (1 - n)
+  U192 t = {~a.w0, ~a.w1, ~a.w2};
+  uint64_t add = 1 - n;                   // two's complement of n - 1
+  uint64_t ext = (n == 0) ? 0 : ~0ull;    // sign extension of (1 - n) for n >= 1
+  add192(t, add, ext, ext);
+  a = t;
+}
+
+// ------------------------------------------------------------------ fp64 engine
+
+// Exact integer in [0, 2^53] -> uint64 (v_floor + v_fma + two v_cvt_u32).
+__device__ __forceinline__ uint64_t f2u64(double f) {
+  double h = __builtin_floor(f * 0x1p-32);
+  double l = __builtin_fma(h, -0x1p32, f);
+  return ((uint64_t)(uint32_t)h << 32) | (uint64_t)(uint32_t)l;
+}
+
+// Reciprocal of m (an exact integer >= 1) to within about one ulp: hardware estimate plus two
+// Newton steps.
+__device__ __forceinline__ double recip(double m) {
+  double r = __builtin_amdgcn_rcp(m);
+  double e = __builtin_fma(-m, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-m, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+// x * (x * 2^c) mod m in balanced form; x, the result: |.| <= 0.75 m.  Six FP64 operations plus
+// the scaling: hi + lo is the exact product, q its rounded quotient, t the exact residue of hi.
+__device__ __forceinline__ double fsq(double x, double m, double minv) {
+  double hi = x * x;
+  double lo = __builtin_fma(x, x, -hi);
+  double q = __builtin_rint(hi * minv);
+  double t = __builtin_fma(-q, m, hi);
+  return t + lo;
+}
+
+__device__ __forceinline__ double fsq_shift(double x, int c, double m, double minv) {
+  double y = __builtin_amdgcn_ldexp(x, c);
+  double hi = x * y;
+  double lo = __builtin_fma(x, y, -hi);
+  double q = __builtin_rint(hi * minv);
+  double t = __builtin_fma(-q, m, hi);
+  return t + lo;
+}
+
+// One 48-bit quotient digit of the long division rem * 2^192 / m.  In: rem in [0, 2m) (or the
+// balanced residue for the first digit).  Out: digit q with N - q m in [0, 2m), N = rem * 2^48.
+// The estimate floor(N/m - 1/4 + err), |err| < 0.16, is floor(N/m) or one less.
+__device__ __forceinline__ double fdigit(double& rem, double m, double minv) {
+  double n = __builtin_amdgcn_ldexp(rem, 48);
+  double q = __builtin_floor(__builtin_fma(n, minv, -0.25));
+  rem = __builtin_fma(-q, m, n);
+  return q;
+}
+
+// 2^e mod m (balanced) for an exponent whose top (bl - s) bits form t < 32, followed by s bits
+// consumed in W-bit windows, most significant first.  s is a multiple of W, uniform per wave.
+template <int W>
+__device__ __forceinline__ double fpow2(uint64_t e, int s, double m, double minv) {
+  const uint32_t t = (uint32_t)(e >> s);
+  const double p = __builtin_amdgcn_ldexp(1.0, (int)t);
+  double x = __builtin_fma(-__builtin_rint(p * minv), m, p);
+  for (int pos = s - W; pos >= 0; pos -= W) {
+#pragma unroll
+    for (int i = 0; i < W - 1; ++i) x = fsq(x, m, minv);
+    const int c = (int)((e >> pos) & ((1u << W) - 1));
+    x = fsq_shift(x, c, m, minv);
+  }
+  return x;
+}
+
+// ------------------------------------------------------------------ int engine
+
+__device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) { return __umul64hi(a, b); }
+
+// -m^-1 mod 2^64 for odd m: 5-bit seed (3m xor 2), three 32-bit Newton steps, one 64-bit step.
+__device__ __forceinline__ uint64_t neg_inv64(uint64_t m) {
+  const uint32_t m32 = (uint32_t)m;
+  uint32_t x = (3u * m32) ^ 2u;
+  x *= 2u - m32 * x;
+  x *= 2u - m32 * x;
+  x *= 2u - m32 * x;
+  uint64_t X = x;
+  X *= 2ull - m * X;
+  return 0ull - X;
+}
+
+// REDC((hi:lo)): (hi:lo + q m) / 2^64 with q = lo * mp mod 2^64.  Input < m * 2^64.
+__device__ __forceinline__ uint64_t redc(uint64_t hi, uint64_t lo, uint64_t m, uint64_t mp) {
+  const uint64_t q = lo * mp;
+  return hi + mulhi64(q, m) + (lo != 0);
+}
+
+// Montgomery square with the window shift folded in: REDC(h^2 * 2^c).  h < 2m (lazy), c <= 7,
+// m < 2^55 keeps h^2 2^c < m 2^64, so the result is again < 2m.
+__device__ __forceinline__ uint64_t isq_shift(uint64_t h, int c, uint64_t m, uint64_t mp) {
+  uint64_t lo = h * h;
+  uint64_t hi = mulhi64(h, h);
+  hi = (hi << c) | ((lo >> 1) >> (63 - c));  // branch-free for c = 0
+  lo <<= c;
+  return redc(hi, lo, m, mp);
+}
+
+__device__ __forceinline__ uint64_t isq(uint64_t h, uint64_t m, uint64_t mp) {
+  return redc(mulhi64(h, h), h * h, m, mp);
+}
+
+// Montgomery form of 2^E (= 2^E * 2^64 mod m) in [0, 2m) for m < 2^55: W-bit windows over the
+// low bl bits of E (bl uniform per wave, E < 2^bl), starting from 2^64 mod m, the Montgomery form
+// of 1; leading zero windows leave it unchanged.
+template <int W>
+__device__ __forceinline__ uint64_t ipow2_mont(uint64_t E, int bl, uint64_t m, uint64_t mp,
+                                               uint64_t one) {
+  int pos = ((bl + W - 1) / W) * W - W;  // start of the top (possibly partial) window
+  uint64_t h = isq_shift(one, (int)(E >> pos), m, mp);
+  for (pos -= W; pos >= 0; pos -= W) {
+#pragma unroll
+    for (int i = 0; i < W - 1; ++i) h = isq(h, m, mp);
+    h = isq_shift(h, (int)((E >> pos) & ((1u << W) - 1)), m, mp);
+  }
+  return h;
+}
+
+// Same for m < 2^63 with fully reduced residues and one bit per square.
+__device__ __forceinline__ uint64_t ipow2_mont_wide(uint64_t E, uint64_t m, uint64_t mp,
+                                                    uint64_t one) {
+  uint64_t h = one;
+  for (int pos = 63 - __builtin_clzll(E | 1); pos >= 0; --pos) {
+    h = isq_shift(h, (int)((E >> pos) & 1), m, mp);
+    if (h >= m) h -= m;
+  }
+  return h;
+}
+
+// Quotient words of floor(r 2^192 / m) from h = 2^(e+192) * 2^64 mod m in [0, 2m):
+// s3 = REDC(h) = r 2^192 mod m, then REDC(s_i) = s_{i-1} with quotient word s_i * mp.
+__device__ __forceinline__ void iquot192(uint64_t h, uint64_t m, uint64_t mp, uint64_t& q0,
+                                         uint64_t& q1, uint64_t& q2) {
+  if (h >= m) h -= m;
+  const uint64_t s3 = mulhi64(h * mp, m) + (h != 0);
+  q0 = s3 * mp;
+  const uint64_t s2 = mulhi64(q0, m) + (s3 != 0);
+  q1 = s2 * mp;
+  const uint64_t s1 = mulhi64(q1, m) + (s2 != 0);
+  q2 = s1 * mp;
+}
+
+}  // namespace clb

@@ -1,0 +1,890 @@
+// clb_host.cpp -- host side of libclbellard.so: the C ABI declared in include/clbellard.h.
+//
+// This replaces the reference's HSP host (Bellard_192.hsp, KernelFuncs.as, hexsademical.as,
+// workmodeSum0_6.hsp) and its OpenCL plugin calls with one C++ library over the HIP runtime.
+// Every computation of terms happens in the kernels of clb_kernels.hip; this file only builds
+// job tables, moves a few hundred bytes, and formats results.
+#include <hip/hip_runtime.h>
+#include <sys/stat.h>
+#include <sys/types.h>
+
+#include <cerrno>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/clbellard.h"
+#include "clb_internal.hpp"
+
+using namespace clb;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(CLB_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),    \
+                  __FILE__, __LINE__);                                                   \
+  } while (0)
+
+// KernelFuncs.as:9-12: numerator, sign flag (1 = minus), denominator den0*k + den1.
+constexpr uint64_t kNumer[7] = {32, 1, 256, 64, 4, 4, 1};
+constexpr int64_t kNumeSign[7] = {1, 1, 0, 1, 1, 1, 0};
+constexpr uint64_t kDen0[7] = {4, 4, 10, 10, 10, 10, 10};
+constexpr uint64_t kDen1[7] = {1, 3, 1, 3, 5, 7, 9};
+// Sum7_192 (Bellard.cl:1364-1368): multiplication by c_j is a left shift by log2(c_j).
+constexpr uint8_t kShift[7] = {5, 0, 8, 6, 2, 2, 0};
+
+constexpr uint64_t kMaxDigits = 999999999999999999ull;  // Bellard_192.hsp:450
+constexpr uint64_t kBatchSize = 137438953472ull;        // Bellard_192.hsp:19, 2^37 terms
+
+// Head-term job before tiling.
+struct HostJob {
+  uint64_t d, k0, k1;  // k0 < k1 <= d, or k0 == k1 for no head terms
+  uint64_t den0, den1;
+  uint32_t numesign;
+  uint32_t slot;
+  bool tail;
+};
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  int reserve(size_t n) {
+    if (n <= cap) return CLB_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = n < 64 ? 64 : n;
+    hipError_t e = hipMalloc((void**)&p, want * sizeof(T));
+    if (e != hipSuccess) return fail(CLB_ENOMEM, "hipMalloc(%zu bytes): %s", want * sizeof(T),
+                                     hipGetErrorString(e));
+    cap = want;
+    return CLB_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+}  // namespace
+
+struct clb_ctx {
+  int device = 0;
+  int precision = CLB_PREC_U192;
+  int kernel = CLB_KERNEL_AUTO;
+  int blocks_per_cu = 0;
+  int num_cus = 0;
+  char device_name[256] = {0};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  uint64_t* partial = nullptr;  // bigSumMem[0..2] (most significant word first)
+  uint64_t* ans = nullptr;      // dev_ans[0..2]
+  DevBuf<JobDev> jobs;
+  DevBuf<TailDev> tails;
+  DevBuf<GroupDev> groups;
+  DevBuf<uint8_t> shifts;
+  DevBuf<unsigned long long> acc;
+  DevBuf<uint64_t> out;
+  DevBuf<uint64_t> slot_out;
+  float head_ms = 0.f, total_ms = 0.f;
+};
+
+namespace {
+
+int set_device(clb_ctx* ctx) {
+  HIP_TRY(hipSetDevice(ctx->device));
+  return CLB_OK;
+}
+
+// Term kernel for the largest modulus of a launch (bounds in clb_device.hpp / clb_internal.hpp).
+int choose_kind(int pref, uint64_t max_m, int* kind) {
+  const bool split_ok = max_m < (1ull << 49);
+  if (pref == CLB_KERNEL_SPLIT && !split_ok)
+    return fail(CLB_ERANGE, "split kernel needs moduli < 2^49 (largest is %llu)",
+                (unsigned long long)max_m);
+  if (pref == CLB_KERNEL_SPLIT || (pref == CLB_KERNEL_AUTO && split_ok)) {
+    *kind = max_m < (1ull << 43) ? kSplitW3 : max_m < (1ull << 47) ? kSplitW2 : kSplitW1;
+    return CLB_OK;
+  }
+  if (max_m >= (1ull << 63))
+    return fail(CLB_ERANGE, "modulus %llu exceeds 2^63", (unsigned long long)max_m);
+  *kind = max_m < (1ull << 55) ? kPlainW3 : kPlainWide;
+  return CLB_OK;
+}
+
+// Runs head jobs + tails into slots [0, nslots), then combines the slots by groups.
+// out_host receives 3 words per group unless accumulate_dev is given, in which case the single
+// group's value is added into that device buffer.  slot_host (optional) receives each slot's
+// shifted value.
+int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8_t>& shift,
+        const std::vector<GroupDev>& groups, uint64_t* out_host, uint64_t* accumulate_dev,
+        uint64_t* slot_host) {
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  const size_t nslots = shift.size();
+  std::vector<JobDev> jobs;
+  std::vector<TailDev> tails;
+  uint64_t flat = 0, max_m = 1;
+  for (const HostJob& h : hjobs) {
+    if (h.tail) tails.push_back({h.d, h.den0, h.den1, h.numesign, h.slot});
+    if (h.k0 >= h.k1) continue;
+    JobDev j{};
+    j.d = h.d;
+    j.k0 = h.k0;
+    j.k1 = h.k1;
+    j.tile_first = h.k0 / kTileK;
+    j.flat_begin = flat;
+    j.den0 = h.den0;
+    j.den1 = h.den1;
+    j.numesign = h.numesign;
+    j.slot = h.slot;
+    flat += (h.k1 + kTileK - 1) / kTileK - j.tile_first;
+    const uint64_t m = h.den0 * (h.k1 - 1) + h.den1;
+    if (m > max_m) max_m = m;
+    jobs.push_back(j);
+  }
+  int kind = kSplitW3;
+  if (!jobs.empty() && (rc = choose_kind(ctx->kernel, max_m, &kind))) return rc;
+  if (jobs.size() > 0x7fffffff || tails.size() > 0x7fffffff || groups.size() > 0x7fffffff)
+    return fail(CLB_EINVAL, "too many jobs");
+  if ((rc = ctx->jobs.reserve(jobs.size() + 1)) || (rc = ctx->tails.reserve(tails.size() + 1)) ||
+      (rc = ctx->groups.reserve(groups.size())) || (rc = ctx->shifts.reserve(nslots)) ||
+      (rc = ctx->acc.reserve(nslots * kAccWords)) || (rc = ctx->out.reserve(3 * groups.size())) ||
+      (rc = ctx->slot_out.reserve(3 * nslots)))
+    return rc;
+  hipStream_t s = ctx->stream;
+  HIP_TRY(hipEventRecord(ctx->ev[0], s));
+  if (!jobs.empty())
+    HIP_TRY(hipMemcpyAsync(ctx->jobs.p, jobs.data(), jobs.size() * sizeof(JobDev),
+                           hipMemcpyHostToDevice, s));
+  if (!tails.empty())
+    HIP_TRY(hipMemcpyAsync(ctx->tails.p, tails.data(), tails.size() * sizeof(TailDev),
+                           hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(ctx->groups.p, groups.data(), groups.size() * sizeof(GroupDev),
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(ctx->shifts.p, shift.data(), nslots, hipMemcpyHostToDevice, s));
+  HIP_TRY(launch_zero(ctx->acc.p, nslots * kAccWords, s));
+  const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : 8;
+  HIP_TRY(hipEventRecord(ctx->ev[1], s));
+  HIP_TRY(launch_head(kind, ctx->jobs.p, (int)jobs.size(), flat, ctx->acc.p, ctx->num_cus * bpc, s));
+  HIP_TRY(hipEventRecord(ctx->ev[2], s));
+  HIP_TRY(launch_tail(ctx->tails.p, (int)tails.size(), ctx->acc.p, s));
+  uint64_t* dst = accumulate_dev ? accumulate_dev : ctx->out.p;
+  HIP_TRY(launch_finalize(ctx->acc.p, ctx->shifts.p, (int)nslots, ctx->groups.p,
+                          (int)groups.size(), dst, accumulate_dev != nullptr,
+                          slot_host ? ctx->slot_out.p : nullptr, s));
+  if (out_host && !accumulate_dev)
+    HIP_TRY(hipMemcpyAsync(out_host, ctx->out.p, 3 * groups.size() * sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, s));
+  if (slot_host)
+    HIP_TRY(hipMemcpyAsync(slot_host, ctx->slot_out.p, 3 * nslots * sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, s));
+  hipEvent_t end;
+  HIP_TRY(hipEventCreate(&end));
+  hipError_t e1 = hipEventRecord(end, s);
+  hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(s) : e1;
+  float head = 0.f, total = 0.f;
+  if (e2 == hipSuccess) e2 = hipEventElapsedTime(&head, ctx->ev[1], ctx->ev[2]);
+  if (e2 == hipSuccess) e2 = hipEventElapsedTime(&total, ctx->ev[0], end);
+  (void)hipEventDestroy(end);
+  if (e2 != hipSuccess)
+    return fail(CLB_EDEVICE, "term launch failed: %s", hipGetErrorString(e2));
+  ctx->head_ms = head;
+  ctx->total_ms = total;
+  return CLB_OK;
+}
+
+// KernelFuncs.as:25-66 semantics: head = [k_start, min(k_end, d)), tail iff the final offset
+// (the end of the head if it ran, else k_start) equals d.
+HostJob batch_job(uint64_t d, uint64_t k_start, uint64_t k_end, uint64_t den0, uint64_t den1,
+                  uint32_t numesign, uint32_t slot) {
+  const uint64_t hi = k_end < d ? k_end : d;
+  HostJob h{};
+  h.d = d;
+  h.den0 = den0;
+  h.den1 = den1;
+  h.numesign = numesign;
+  h.slot = slot;
+  uint64_t offset = k_start;
+  if (k_start < hi) {
+    h.k0 = k_start;
+    h.k1 = hi;
+    offset = hi;
+  } else {
+    h.k0 = h.k1 = 0;
+  }
+  h.tail = (offset == d);
+  return h;
+}
+
+int check_ctx(clb_ctx* ctx) {
+  if (!ctx) return fail(CLB_EINVAL, "null context");
+  return CLB_OK;
+}
+
+int normalize(uint64_t n_in, uint64_t* n_out, uint64_t* d_out, int* overdouble) {
+  if (n_in <= 4 || n_in > kMaxDigits)
+    return fail(CLB_EINVAL, "invalid digits %llu (need 5 <= N <= %llu)",
+                (unsigned long long)n_in, (unsigned long long)kMaxDigits);
+  uint64_t d = ((n_in - 1) * 2 - 3) / 5;  // Bellard_192.hsp:468
+  if (d % 2 == 0) d--;                    // :469
+  if (n_out) *n_out = (d * 5 + 3) / 2 + 1;  // :470, a multiple of 5
+  if (d_out) *d_out = d;
+  if (overdouble) *overdouble = d * 10 + 9 >= 9007199254740992ull;  // :472
+  return CLB_OK;
+}
+
+void hex48(const uint64_t w[3], char out[49]) {
+  static const char hx[] = "0123456789abcdef";
+  for (int i = 0; i < 3; ++i)
+    for (int b = 0; b < 16; ++b) out[16 * i + b] = hx[(w[i] >> (60 - 4 * b)) & 15];
+  out[48] = 0;
+}
+
+void twosum(double a, double b, double* x, double* y) {
+  double s = a + b;
+  double t = s - a;
+  *y = (a - (s - t)) + (b - t);
+  *x = s;
+}
+
+// addcab of hexsademical.as:12-19.
+void dd_add(double* h, double* l, double bh, double bl) {
+  double x, y;
+  twosum(*h, bh, &x, &y);
+  y = y + *l + bl;
+  twosum(x, y, h, l);
+}
+
+// hexadecimal_dd, hexsademical.as:21-101: 14 base-256 digits of a double-double in [-1, 1),
+// with carries and borrows pushed into the already written (more significant) bytes of a
+// 24-byte little-endian buffer, printed as 48 hex digits (the last 20 are zero).
+void hex_dd(const double dd[2], char out[49]) {
+  double hh = dd[0], ll = dd[1];
+  uint32_t words[6] = {0, 0, 0, 0, 0, 0};
+  uint8_t* bytes = reinterpret_cast<uint8_t*>(words);
+  if (hh < 0.0) dd_add(&hh, &ll, 1.0, 0.0);
+  auto carry = [&](int idx, int dir) {
+    if (idx >= 24) return;
+    int b = idx / 4;
+    uint32_t add = 1u << ((idx % 4) * 8);
+    words[b] += dir > 0 ? add : (uint32_t)0 - add;
+    for (int rep = 5 - b; rep > 0; --rep) {
+      const int32_t v = (int32_t)words[b];
+      const bool more = dir > 0 ? (v >= 0 && v < (int32_t)add) : (v < 0 && v >= -(int32_t)add);
+      if (!more) break;
+      ++b;
+      add = 1;
+      words[b] += dir > 0 ? 1u : 0xffffffffu;
+    }
+  };
+  for (int cnt = 0; cnt < 14; ++cnt) {
+    hh *= 256.0;
+    ll *= 256.0;
+    twosum(hh, ll, &hh, &ll);
+    int digit = (int)hh;
+    if (digit < 256 && hh >= 0.0) {
+      bytes[23 - cnt] = (uint8_t)digit;
+      hh -= digit;
+      continue;
+    }
+    if (digit == 256) {
+      carry(23 - cnt + 1, +1);
+      bytes[23 - cnt] = 0;
+      hh -= 256.0;
+    }
+    if (hh < 0.0) {
+      digit = (int)(hh + 256.0);
+      carry(23 - cnt + 1, -1);
+      bytes[23 - cnt] = (uint8_t)digit;
+      dd_add(&hh, &ll, 256.0, 0.0);
+    }
+  }
+  static const char hx[] = "0123456789abcdef";
+  for (int i = 0; i < 24; ++i) {
+    out[2 * i] = hx[bytes[23 - i] >> 4];
+    out[2 * i + 1] = hx[bytes[23 - i] & 15];
+  }
+  out[48] = 0;
+}
+
+// Round a non-negative integer given as up to 3 words (most significant first) times 2^-192 to
+// the nearest double (ties to even).
+double round_u192(const uint64_t w[3], bool* exact) {
+  int lead = -1;
+  for (int i = 0; i < 3 && lead < 0; ++i)
+    if (w[i]) lead = (2 - i) * 64 + (63 - __builtin_clzll(w[i]));
+  *exact = true;
+  if (lead < 0) return 0.0;
+  auto bit = [&](int p) -> int { return p < 0 ? 0 : (int)((w[2 - p / 64] >> (p % 64)) & 1); };
+  uint64_t mant = 0;
+  const int low = lead - 52;  // position of the last kept bit
+  for (int p = lead; p >= low; --p) mant = (mant << 1) | (uint64_t)bit(p);
+  bool sticky = false;
+  for (int p = low - 2; p >= 0; --p)
+    if (bit(p)) {
+      sticky = true;
+      break;
+    }
+  const int rbit = low - 1 >= 0 ? bit(low - 1) : 0;
+  if (low - 1 >= 0 && (rbit || sticky)) *exact = false;
+  if (rbit && (sticky || (mant & 1))) mant++;
+  return std::ldexp((double)mant, low - 192);
+}
+
+int u192_to_dd(const uint64_t w[3], double dd[2]) {
+  bool exact;
+  const double hi = round_u192(w, &exact);
+  // remainder w - hi * 2^192 as a signed 192-bit integer
+  uint64_t h[3] = {0, 0, 0};
+  int ex;
+  const double fr = std::frexp(hi, &ex);  // hi = fr * 2^ex, fr in [0.5, 1)
+  if (hi != 0.0) {
+    const uint64_t mant = (uint64_t)std::ldexp(fr, 53);
+    const int pos = ex - 53 + 192;  // bit position of mant's lsb
+    for (int b = 0; b < 54; ++b)
+      if ((mant >> b) & 1) {
+        const int p = pos + b;
+        if (p >= 0 && p < 192) h[2 - p / 64] |= 1ull << (p % 64);
+      }
+  }
+  uint64_t r[3];
+  uint64_t borrow = 0;
+  for (int i = 2; i >= 0; --i) {
+    const uint64_t a = w[i], b = h[i];
+    const uint64_t d1 = a - b;
+    const uint64_t b1 = a < b;
+    const uint64_t d2 = d1 - borrow;
+    const uint64_t b2 = d1 < borrow;
+    r[i] = d2;
+    borrow = b1 | b2;
+  }
+  const bool neg = (r[0] >> 63) != 0;
+  if (neg) {
+    uint64_t c = 1;
+    for (int i = 2; i >= 0; --i) {
+      r[i] = ~r[i] + c;
+      c = (c && r[i] == 0) ? 1 : 0;
+    }
+  }
+  bool ex2;
+  const double lo = round_u192(r, &ex2);
+  dd[0] = hi;
+  dd[1] = neg ? -lo : lo;
+  return CLB_OK;
+}
+
+// ------------------------------------------------------------------ work mode helpers
+
+bool file_exists(const std::string& p) {
+  struct stat st;
+  return stat(p.c_str(), &st) == 0;
+}
+
+int read_text(const std::string& path, std::string* out) {
+  FILE* f = fopen(path.c_str(), "rb");
+  if (!f) return fail(CLB_EIO, "cannot open %s: %s", path.c_str(), strerror(errno));
+  out->clear();
+  char buf[4096];
+  size_t n;
+  while ((n = fread(buf, 1, sizeof buf, f)) > 0) out->append(buf, n);
+  fclose(f);
+  return CLB_OK;
+}
+
+std::vector<std::string> split_lines(const std::string& text) {
+  std::vector<std::string> lines;
+  std::string cur;
+  for (char c : text) {
+    if (c == '\n') {
+      if (!cur.empty() && cur.back() == '\r') cur.pop_back();
+      lines.push_back(cur);
+      cur.clear();
+    } else {
+      cur.push_back(c);
+    }
+  }
+  if (!cur.empty() && cur.back() == '\r') cur.pop_back();
+  if (!cur.empty()) lines.push_back(cur);
+  return lines;
+}
+
+void strrep(std::string* s, const std::string& from) {
+  size_t p;
+  while (!from.empty() && (p = s->find(from)) != std::string::npos) s->erase(p, from.size());
+}
+
+// HSP int() / int64() of a string: optional sign, then leading decimal digits (0 if none).
+int64_t hsp_int(const std::string& s) {
+  size_t i = 0;
+  bool neg = false;
+  if (i < s.size() && (s[i] == '-' || s[i] == '+')) neg = s[i++] == '-';
+  int64_t v = 0;
+  for (; i < s.size() && s[i] >= '0' && s[i] <= '9'; ++i) {
+    if (v > (INT64_MAX - 9) / 10) break;
+    v = v * 10 + (s[i] - '0');
+  }
+  return neg ? -v : v;
+}
+
+int make_dir(const std::string& p) {
+  if (mkdir(p.c_str(), 0755) == 0 || errno == EEXIST) return CLB_OK;
+  return fail(CLB_EIO, "cannot create %s: %s", p.c_str(), strerror(errno));
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+const char* clb_last_error(void) { return g_last_error.c_str(); }
+
+const char* clb_version(void) { return "clbellard-mi355x 0.1 (gfx950)"; }
+
+int clb_get_dev_count(int* count) {
+  if (!count) return fail(CLB_EINVAL, "null count");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) {
+    *count = 0;
+    return fail(CLB_EDEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
+  }
+  *count = n;
+  return CLB_OK;
+}
+
+int clb_get_device_name(int device, char* name, size_t cap) {
+  if (!name || cap == 0) return fail(CLB_EINVAL, "null name buffer");
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  snprintf(name, cap, "%s", prop.name);
+  return CLB_OK;
+}
+
+int clb_ctx_create(int device, int precision, clb_ctx** out) {
+  if (!out) return fail(CLB_EINVAL, "null output");
+  *out = nullptr;
+  if (precision < CLB_PREC_DD106 || precision > CLB_PREC_U192_BETA)
+    return fail(CLB_EINVAL, "precision must be 0, 1 or 2");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(CLB_EDEVICE, "no HIP device available");  // Bellard_192.hsp:17
+  if (device < 0 || device >= n) return fail(CLB_EINVAL, "device %d of %d", device, n);
+  clb_ctx* ctx = new clb_ctx();
+  ctx->device = device;
+  ctx->precision = precision;
+  auto cleanup = [&](int rc) {
+    clb_ctx_destroy(ctx);
+    return rc;
+  };
+  hipError_t e = hipSetDevice(device);
+  hipDeviceProp_t prop;
+  if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess)
+    return cleanup(fail(CLB_EDEVICE, "device %d: %s", device, hipGetErrorString(e)));
+  ctx->num_cus = prop.multiProcessorCount;
+  snprintf(ctx->device_name, sizeof ctx->device_name, "%s", prop.name);
+  if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess)
+    return cleanup(fail(CLB_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e)));
+  for (auto& ev : ctx->ev)
+    if ((e = hipEventCreate(&ev)) != hipSuccess)
+      return cleanup(fail(CLB_EDEVICE, "hipEventCreate: %s", hipGetErrorString(e)));
+  if ((e = hipMalloc((void**)&ctx->partial, 3 * sizeof(uint64_t))) != hipSuccess ||
+      (e = hipMalloc((void**)&ctx->ans, 3 * sizeof(uint64_t))) != hipSuccess)
+    return cleanup(fail(CLB_ENOMEM, "hipMalloc: %s", hipGetErrorString(e)));
+  if ((e = hipMemsetAsync(ctx->partial, 0, 3 * sizeof(uint64_t), ctx->stream)) != hipSuccess ||
+      (e = hipMemsetAsync(ctx->ans, 0, 3 * sizeof(uint64_t), ctx->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+    return cleanup(fail(CLB_EDEVICE, "clear: %s", hipGetErrorString(e)));
+  *out = ctx;
+  return CLB_OK;
+}
+
+int clb_ctx_destroy(clb_ctx* ctx) {
+  if (!ctx) return CLB_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->jobs.release();
+  ctx->tails.release();
+  ctx->groups.release();
+  ctx->shifts.release();
+  ctx->acc.release();
+  ctx->out.release();
+  ctx->slot_out.release();
+  if (ctx->partial) (void)hipFree(ctx->partial);
+  if (ctx->ans) (void)hipFree(ctx->ans);
+  for (auto& ev : ctx->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CLB_OK;
+}
+
+int clb_set_kernel(clb_ctx* ctx, int kernel) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (kernel < CLB_KERNEL_AUTO || kernel > CLB_KERNEL_SPLIT)
+    return fail(CLB_EINVAL, "kernel must be 0 (auto), 1 (plain) or 2 (split)");
+  ctx->kernel = kernel;
+  return CLB_OK;
+}
+
+int clb_set_launch(clb_ctx* ctx, int blocks_per_cu) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(CLB_EINVAL, "blocks_per_cu 0..64");
+  ctx->blocks_per_cu = blocks_per_cu;
+  return CLB_OK;
+}
+
+int clb_normalize(uint64_t n_in, uint64_t* n_out, uint64_t* d_out, int* overdouble) {
+  return normalize(n_in, n_out, d_out, overdouble);
+}
+
+int clb_formula(int j, uint64_t* numer, int64_t* numesign, uint64_t* den0, uint64_t* den1) {
+  if (j < 0 || j > 6) return fail(CLB_EINVAL, "formula %d (need 0..6)", j);
+  if (numer) *numer = kNumer[j];
+  if (numesign) *numesign = kNumeSign[j];
+  if (den0) *den0 = kDen0[j];
+  if (den1) *den1 = kDen1[j];
+  return CLB_OK;
+}
+
+int clb_set0mem(clb_ctx* ctx) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (int rc = set_device(ctx)) return rc;
+  HIP_TRY(launch_zero(reinterpret_cast<unsigned long long*>(ctx->partial), 3, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return CLB_OK;
+}
+
+int clb_clear_ans(clb_ctx* ctx) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (int rc = set_device(ctx)) return rc;
+  HIP_TRY(launch_zero(reinterpret_cast<unsigned long long*>(ctx->ans), 3, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return CLB_OK;
+}
+
+static int run_into_partial(clb_ctx* ctx, const HostJob& h) {
+  std::vector<HostJob> jobs{h};
+  std::vector<uint8_t> shift{0};
+  std::vector<GroupDev> groups{{0, 1}};
+  return run(ctx, jobs, shift, groups, nullptr, ctx->partial, nullptr);
+}
+
+int clb_batch_calc(clb_ctx* ctx, uint64_t d, uint64_t k_start, uint64_t k_end, int j) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (j < 0 || j > 6) return fail(CLB_EINVAL, "formula %d (need 0..6)", j);
+  if (d == 0) return fail(CLB_EINVAL, "d must be positive");
+  return run_into_partial(ctx, batch_job(d, k_start, k_end, kDen0[j], kDen1[j],
+                                         (uint32_t)kNumeSign[j], 0));
+}
+
+static int check_formula_args(uint64_t d, int64_t numesign, uint64_t den0, uint64_t den1) {
+  if (d == 0) return fail(CLB_EINVAL, "d must be positive");
+  if (numesign != 0 && numesign != 1) return fail(CLB_EINVAL, "numesign must be 0 or 1");
+  if (den0 == 0 || (den0 & 1) || !(den1 & 1))
+    return fail(CLB_EINVAL, "denominator den0*k+den1 must be odd with den0 > 0 even");
+  if (d > (UINT64_MAX - den1) / den0 / 2) return fail(CLB_ERANGE, "modulus overflows");
+  return CLB_OK;
+}
+
+int clb_sglobal(clb_ctx* ctx, uint64_t offset, uint64_t d, uint64_t k_max, int64_t numesign,
+                uint64_t den0, uint64_t den1) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (int rc = check_formula_args(d, numesign, den0, den1)) return rc;
+  if (k_max > d) return fail(CLB_EINVAL, "k_max %llu beyond d %llu", (unsigned long long)k_max,
+                             (unsigned long long)d);
+  HostJob h{};
+  h.d = d;
+  h.den0 = den0;
+  h.den1 = den1;
+  h.numesign = (uint32_t)numesign;
+  h.k0 = offset < k_max ? offset : 0;
+  h.k1 = offset < k_max ? k_max : 0;
+  h.tail = false;
+  return run_into_partial(ctx, h);
+}
+
+int clb_sglobal_after(clb_ctx* ctx, uint64_t d, int64_t numesign, uint64_t den0, uint64_t den1) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (int rc = check_formula_args(d, numesign, den0, den1)) return rc;
+  HostJob h{};
+  h.d = d;
+  h.den0 = den0;
+  h.den1 = den1;
+  h.numesign = (uint32_t)numesign;
+  h.tail = true;
+  return run_into_partial(ctx, h);
+}
+
+int clb_add_ans(clb_ctx* ctx, int j) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (j < 0 || j > 6) return fail(CLB_EINVAL, "formula %d (need 0..6)", j);
+  if (int rc = set_device(ctx)) return rc;
+  HIP_TRY(launch_sum7(ctx->ans, ctx->partial, kShift[j], ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return CLB_OK;
+}
+
+static int read3(clb_ctx* ctx, const uint64_t* dev, uint64_t w[3]) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!w) return fail(CLB_EINVAL, "null output");
+  if (int rc = set_device(ctx)) return rc;
+  HIP_TRY(hipMemcpyAsync(w, dev, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return CLB_OK;
+}
+
+int clb_read_partial(clb_ctx* ctx, uint64_t w[3]) { return read3(ctx, ctx ? ctx->partial : nullptr, w); }
+int clb_read_ans(clb_ctx* ctx, uint64_t w[3]) { return read3(ctx, ctx ? ctx->ans : nullptr, w); }
+
+static void format_result(int precision, const uint64_t w[3], char hex[49]) {
+  if (precision == CLB_PREC_DD106) {
+    double dd[2];
+    u192_to_dd(w, dd);
+    hex_dd(dd, hex);
+  } else {
+    hex48(w, hex);
+  }
+}
+
+int clb_normal(clb_ctx* ctx, uint64_t n, uint64_t frac[3], char hex[49], uint64_t* n_out) {
+  if (int rc = check_ctx(ctx)) return rc;
+  uint64_t d, nn;
+  if (int rc = normalize(n, &nn, &d, nullptr)) return rc;
+  std::vector<HostJob> jobs;
+  std::vector<uint8_t> shift(7);
+  for (int j = 0; j < 7; ++j) {
+    jobs.push_back(batch_job(d, 0, d, kDen0[j], kDen1[j], (uint32_t)kNumeSign[j], (uint32_t)j));
+    shift[j] = kShift[j];
+  }
+  std::vector<GroupDev> groups{{0, 7}};
+  uint64_t res[3], slots[21];
+  if (int rc = run(ctx, jobs, shift, groups, res, nullptr, slots)) return rc;
+  // Leave the device state as Allloop does: dev_ans = the answer, bigSum = formula 6's partial.
+  HIP_TRY(hipMemcpyAsync(ctx->ans, res, sizeof res, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(ctx->partial, slots + 18, 3 * sizeof(uint64_t), hipMemcpyHostToDevice,
+                         ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (frac) memcpy(frac, res, sizeof res);
+  if (hex) format_result(ctx->precision, res, hex);
+  if (n_out) *n_out = nn;
+  return CLB_OK;
+}
+
+int clb_batch_positions(clb_ctx* ctx, const uint64_t* n, size_t count, uint64_t* frac) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (count == 0) return CLB_OK;
+  if (!n || !frac) return fail(CLB_EINVAL, "null array");
+  if (count > (1u << 26)) return fail(CLB_EINVAL, "too many positions");
+  std::vector<HostJob> jobs;
+  std::vector<uint8_t> shift(7 * count);
+  std::vector<GroupDev> groups(count);
+  jobs.reserve(7 * count);
+  for (size_t i = 0; i < count; ++i) {
+    uint64_t d;
+    if (int rc = normalize(n[i], nullptr, &d, nullptr)) return rc;
+    for (int j = 0; j < 7; ++j) {
+      const uint32_t slot = (uint32_t)(7 * i + j);
+      jobs.push_back(batch_job(d, 0, d, kDen0[j], kDen1[j], (uint32_t)kNumeSign[j], slot));
+      shift[slot] = kShift[j];
+    }
+    groups[i] = {(uint32_t)(7 * i), 7};
+  }
+  return run(ctx, jobs, shift, groups, frac, nullptr, nullptr);
+}
+
+int clb_run_jobs(clb_ctx* ctx, const clb_job* jobs_in, size_t count, uint64_t* partial) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (count == 0) return CLB_OK;
+  if (!jobs_in || !partial) return fail(CLB_EINVAL, "null array");
+  if (count > (1u << 28)) return fail(CLB_EINVAL, "too many jobs");
+  std::vector<HostJob> jobs;
+  std::vector<uint8_t> shift(count);
+  std::vector<GroupDev> groups(count);
+  for (size_t i = 0; i < count; ++i) {
+    const clb_job& jb = jobs_in[i];
+    if (jb.formula < 0 || jb.formula > 6)
+      return fail(CLB_EINVAL, "job %zu: formula %d", i, jb.formula);
+    if (jb.d == 0 || jb.d > kMaxDigits) return fail(CLB_EINVAL, "job %zu: bad d", i);
+    const int j = jb.formula;
+    jobs.push_back(batch_job(jb.d, jb.k_begin, jb.k_end, kDen0[j], kDen1[j],
+                             (uint32_t)kNumeSign[j], (uint32_t)i));
+    shift[i] = kShift[j];
+    groups[i] = {(uint32_t)i, 1};
+  }
+  return run(ctx, jobs, shift, groups, partial, nullptr, nullptr);
+}
+
+int clb_hex48(const uint64_t w[3], char hex[49]) {
+  if (!w || !hex) return fail(CLB_EINVAL, "null pointer");
+  hex48(w, hex);
+  return CLB_OK;
+}
+
+int clb_hex_dd(const double dd[2], char hex[49]) {
+  if (!dd || !hex) return fail(CLB_EINVAL, "null pointer");
+  hex_dd(dd, hex);
+  return CLB_OK;
+}
+
+int clb_u192_to_dd(const uint64_t w[3], double dd[2]) {
+  if (!w || !dd) return fail(CLB_EINVAL, "null pointer");
+  return u192_to_dd(w, dd);
+}
+
+int clb_last_timing(clb_ctx* ctx, double* head_kernel_ms, double* total_ms) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (head_kernel_ms) *head_kernel_ms = ctx->head_ms;
+  if (total_ms) *total_ms = ctx->total_ms;
+  return CLB_OK;
+}
+
+int clb_run_worklist(clb_ctx* ctx, const char* worklist_path, const char* work_dir,
+                     uint64_t batch_size, const char* device_label, int* jobs_run) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!worklist_path || !work_dir) return fail(CLB_EINVAL, "null path");
+  if (jobs_run) *jobs_run = 0;
+  std::string text;
+  if (int rc = read_text(worklist_path, &text)) return rc;
+  const std::vector<std::string> lines = split_lines(text);
+  if (lines.size() < 2) return fail(CLB_EFORMAT, "worklist needs digits= and precision= lines");
+  // *chktxtstr, Bellard_192.hsp:176-207
+  std::string digits = lines[0];
+  strrep(&digits, "digits");
+  strrep(&digits, "=");
+  strrep(&digits, " ");
+  int64_t nd = hsp_int(digits);
+  if (nd < 0) nd = 0;
+  if ((uint64_t)nd > kMaxDigits) nd = (int64_t)kMaxDigits;
+  std::string prec = lines[1];
+  strrep(&prec, "precision");
+  strrep(&prec, "=");
+  strrep(&prec, " ");
+  // *workStart, :221-227
+  if (nd < 5) return fail(CLB_EFORMAT, "worklist digits must be >= 5");
+  if (prec.empty() || (prec[0] != '0' && prec[0] != '1' && prec[0] != '2'))
+    return fail(CLB_EFORMAT, "worklist precision must be 0, 1 or 2");
+  const int precision = prec[0] - '0';
+  uint64_t d;
+  if (int rc = normalize((uint64_t)nd, nullptr, &d, nullptr)) return rc;
+  const uint64_t batch = batch_size ? batch_size : kBatchSize;
+  // *workStart2 (:229-247) and *Batchloop (:298-340)
+  std::vector<clb_job> todo;
+  std::vector<std::string> paths;
+  std::vector<int> formulas;
+  for (size_t li = 2; li < lines.size(); ++li) {
+    const std::string& t = lines[li];
+    if (t.empty()) continue;  // a blank line is not a job
+    const int j = (int)hsp_int(t.substr(0, 1));
+    const int64_t b = hsp_int(t.size() > 2 ? t.substr(2) : std::string());
+    if (j < 0 || j > 6 || b < 0) return fail(CLB_EFORMAT, "bad job line '%s'", t.c_str());
+    if ((uint64_t)b > UINT64_MAX / batch - 1) return fail(CLB_EFORMAT, "batch %lld too large", (long long)b);
+    const uint64_t k_start = batch * (uint64_t)b, k_end = batch * ((uint64_t)b + 1);
+    if (k_start > d) continue;  // :306
+    char name[64];
+    snprintf(name, sizeof name, "/%d/%d_%lld.txt", j, j, (long long)b);
+    const std::string path = std::string(work_dir) + name;
+    if (file_exists(path)) continue;  // :307-308, the resume rule
+    bool dup = false;
+    for (const auto& p : paths) dup = dup || p == path;
+    if (dup) continue;
+    todo.push_back({d, k_start, k_end, j, 0});
+    paths.push_back(path);
+    formulas.push_back(j);
+  }
+  if (todo.empty()) return CLB_OK;
+  std::vector<uint64_t> part(3 * todo.size());
+  const int saved_precision = ctx->precision;
+  ctx->precision = precision;
+  int rc = clb_run_jobs(ctx, todo.data(), todo.size(), part.data());
+  ctx->precision = saved_precision;
+  if (rc) return rc;
+  const long long ms = (long long)std::llround(ctx->total_ms);
+  if ((rc = make_dir(work_dir))) return rc;
+  for (size_t i = 0; i < todo.size(); ++i) {
+    char dir[32];
+    snprintf(dir, sizeof dir, "/%d", formulas[i]);
+    if ((rc = make_dir(std::string(work_dir) + dir))) return rc;
+    char hex[49];
+    format_result(precision, &part[3 * i], hex);
+    // fsave, Bellard_192.hsp:514-533 (HSP's "\n" is CR LF)
+    std::string body = std::string(hex) + "\r\n";
+    if (device_label)
+      body += device_label;
+    else
+      body += std::to_string(ctx->device) + "\t" + ctx->device_name;
+    body += "\r\n" + std::to_string(ms) + "ms";
+    FILE* f = fopen(paths[i].c_str(), "wb");
+    if (!f) return fail(CLB_EIO, "cannot write %s: %s", paths[i].c_str(), strerror(errno));
+    const bool ok = fwrite(body.data(), 1, body.size(), f) == body.size();
+    if (fclose(f) != 0 || !ok) return fail(CLB_EIO, "write failed: %s", paths[i].c_str());
+    if (jobs_run) ++*jobs_run;
+  }
+  return CLB_OK;
+}
+
+int clb_work_sum(const char* work_dir, char hex[49], int counts[7]) {
+  if (!work_dir || !hex) return fail(CLB_EINVAL, "null pointer");
+  int sum[48] = {0};
+  for (int j = 0; j < 7; ++j) {
+    int used = 0;
+    for (long long b = 0;; ++b) {
+      char name[64];
+      snprintf(name, sizeof name, "/%d/%d_%lld.txt", j, j, b);
+      const std::string path = std::string(work_dir) + name;
+      if (!file_exists(path)) break;  // workmodeSum0_6.hsp:14-16: stop at the first gap
+      std::string text;
+      if (int rc = read_text(path, &text)) return rc;
+      const std::vector<std::string> lines = split_lines(text);
+      if (lines.empty() || lines[0].size() < 48)
+        return fail(CLB_EFORMAT, "%s: first line is not 48 hex digits", path.c_str());
+      const std::string& line = lines[0];
+      for (int c = 0; c < 48; ++c) {  // :21-28, least significant digit first
+        const char ch = line[47 - c];
+        int v;
+        if (ch >= '0' && ch <= '9')
+          v = ch - '0';
+        else if (ch >= 'a' && ch <= 'f')
+          v = ch - 'a' + 10;
+        else
+          return fail(CLB_EFORMAT, "%s: '%c' is not a lowercase hex digit", path.c_str(), ch);
+        sum[47 - c] += v;
+        if (sum[47 - c] >= 16) {
+          sum[47 - c] -= 16;
+          if (c != 47) sum[46 - c]++;
+        }
+      }
+      ++used;
+    }
+    if (counts) counts[j] = used;
+  }
+  static const char hx[] = "0123456789abcdef";
+  for (int i = 0; i < 48; ++i) hex[i] = hx[sum[i]];
+  hex[48] = 0;
+  return CLB_OK;
+}
+
+}  // extern "C"

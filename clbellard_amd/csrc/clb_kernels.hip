@@ -1,0 +1,331 @@
+// clb_kernels.hip -- HIP kernels for gfx950: Bellard head terms, tail terms and the combine.
+//
+// Work decomposition.  A launch holds jobs (formula j, loop count d, head range [k0, k1)).  Each
+// job's range is cut into tiles of kTileK consecutive k aligned to multiples of kTileK; the tiles
+// of all jobs form one flat index space that a persistent grid walks round-robin, one tile per
+// wave at a time.  Lane L of a tile evaluates k = base + L + 64 i, i < kTermsPerLane, so every
+// lane sees one parity of k and hence one sign (KernelFuncs.as:10 and the (numesign + k) % 2 rule
+// of Bellard_div2by1.cl:335).  Each lane keeps the 192-bit sum Q of its terms; when the wave moves
+// to another job (and at the end) the lane applies its sign (-Q - n for n negative terms,
+// Bellard_div2by1.cl:335-341), the wave sums its 64 lanes with cross-lane shuffles, and lane 0
+// adds the six 32-bit limbs into the job's accumulator slot with 64-bit atomics.  Integer sums
+// are exact, so the result does not depend on the schedule.
+#include "clb_device.hpp"
+#include "clb_internal.hpp"
+
+namespace clb {
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int mask) {
+  const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, mask, kWave);
+  const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), mask, kWave);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Sum of a 192-bit value over the 64 lanes of the wave (every lane receives the sum).
+__device__ __forceinline__ void wave_sum192(U192& v) {
+#pragma unroll
+  for (int mask = 32; mask >= 1; mask >>= 1) {
+    const uint64_t o0 = shfl_xor64(v.w0, mask);
+    const uint64_t o1 = shfl_xor64(v.w1, mask);
+    const uint64_t o2 = shfl_xor64(v.w2, mask);
+    add192(v, o0, o1, o2);
+  }
+}
+
+__device__ __forceinline__ void atomic_add192(unsigned long long* slot, const U192& v) {
+  atomicAdd(slot + 0, (unsigned long long)(v.w0 & 0xffffffffull));
+  atomicAdd(slot + 1, (unsigned long long)(v.w0 >> 32));
+  atomicAdd(slot + 2, (unsigned long long)(v.w1 & 0xffffffffull));
+  atomicAdd(slot + 3, (unsigned long long)(v.w1 >> 32));
+  atomicAdd(slot + 4, (unsigned long long)(v.w2 & 0xffffffffull));
+  atomicAdd(slot + 5, (unsigned long long)(v.w2 >> 32));
+}
+
+// Sign, wave sum and atomic add of the lanes' job sums.
+__device__ __forceinline__ void flush_job(U192 v, uint64_t nterms, uint32_t numesign, int lane,
+                                          unsigned long long* slot) {
+  if ((numesign + (uint32_t)lane) & 1u) neg192_minus(v, nterms);
+  wave_sum192(v);
+  if (lane == 0) atomic_add192(slot, v);
+}
+
+// ------------------------------------------------------------------ per-tile term evaluation
+
+// fp64 engine.  Per lane per tile at most kTermsPerLane = 16 terms, so the four digit sums stay
+// exact integers below 2^53 and are converted to the lane's 192-bit sum once per tile.
+template <int W>
+__device__ __forceinline__ void tile_split(const JobDev& jb, uint64_t kbase, int lane, U192& acc,
+                                           uint64_t& nterms) {
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0;
+  uint64_t k = kbase + (uint64_t)lane;
+  double m = (double)(jb.den0 * k + jb.den1);
+  uint64_t e = 10ull * (jb.d - k);
+  const double mstep = (double)(jb.den0 * (uint64_t)kWave);
+  for (int i = 0; i < kTermsPerLane; ++i) {
+    const uint64_t kw = kbase + (uint64_t)kWave * (uint64_t)i;  // k of lane 0: largest exponent
+    if (kw >= jb.k1) break;
+    if (k >= jb.k0 && k < jb.k1) {
+      const uint64_t emax = 10ull * (jb.d - kw);
+      const int bl = 64 - __builtin_clzll(emax);
+      const int s = bl > 3 ? W * ((bl - 3) / W) : 0;
+      const double minv = recip(m);
+      double rem = fpow2<W>(e, s, m, minv);
+      f0 += fdigit(rem, m, minv);
+      f1 += fdigit(rem, m, minv);
+      f2 += fdigit(rem, m, minv);
+      const double q3 = fdigit(rem, m, minv);
+      f3 += q3 + (rem >= m ? 1.0 : 0.0);  // lazy remainder in [m, 2m): one more
+      nterms += (m > 1.0);                 // m = 1 gives r = 0 and a zero term of either sign
+    }
+    k += kWave;
+    m += mstep;
+    e -= 10ull * kWave;
+  }
+  // Digit i has weight 2^(144 - 48 i).  f0 may be negative (balanced residue); adding 2^52 keeps
+  // it non-negative and changes the sum by 2^196 = 0 mod 2^192.
+  const uint64_t u3 = f2u64(f3);
+  const uint64_t u2 = f2u64(f2);
+  const uint64_t u1 = f2u64(f1);
+  const uint64_t u0 = f2u64(f0 + 0x1p52);
+  add192(acc, u3, 0, 0);
+  add192(acc, u2 << 48, u2 >> 16, 0);
+  add192(acc, 0, u1 << 32, u1 >> 32);
+  add192(acc, 0, 0, u0 << 16);
+}
+
+// int engine: Montgomery form of 2^(e+192), then three exact-division quotient words.
+template <bool kWide>
+__device__ __forceinline__ void tile_plain(const JobDev& jb, uint64_t kbase, int lane, U192& acc,
+                                           uint64_t& nterms) {
+  uint64_t k = kbase + (uint64_t)lane;
+  uint64_t m = jb.den0 * k + jb.den1;
+  uint64_t e = 10ull * (jb.d - k);
+  const uint64_t mstep = jb.den0 * (uint64_t)kWave;
+  for (int i = 0; i < kTermsPerLane; ++i) {
+    const uint64_t kw = kbase + (uint64_t)kWave * (uint64_t)i;
+    if (kw >= jb.k1) break;
+    if (k >= jb.k0 && k < jb.k1) {
+      const uint64_t mp = neg_inv64(m);
+      const uint64_t one = (0ull - m) % m;  // 2^64 mod m
+      const uint64_t E = e + 192;
+      uint64_t h;
+      if (kWide) {
+        h = ipow2_mont_wide(E, m, mp, one);
+      } else {
+        const int bl = 64 - __builtin_clzll(10ull * (jb.d - kw) + 192);
+        h = ipow2_mont<3>(E, bl, m, mp, one);
+      }
+      uint64_t q0, q1, q2;
+      iquot192(h, m, mp, q0, q1, q2);
+      add192(acc, q0, q1, q2);
+      nterms += (m > 1);
+    }
+    k += kWave;
+    m += mstep;
+    e -= 10ull * kWave;
+  }
+}
+
+template <int KIND>
+__device__ __forceinline__ void tile_terms(const JobDev& jb, uint64_t kbase, int lane, U192& acc,
+                                           uint64_t& nterms) {
+  if constexpr (KIND == kSplitW3) tile_split<3>(jb, kbase, lane, acc, nterms);
+  if constexpr (KIND == kSplitW2) tile_split<2>(jb, kbase, lane, acc, nterms);
+  if constexpr (KIND == kSplitW1) tile_split<1>(jb, kbase, lane, acc, nterms);
+  if constexpr (KIND == kPlainW3) tile_plain<false>(jb, kbase, lane, acc, nterms);
+  if constexpr (KIND == kPlainWide) tile_plain<true>(jb, kbase, lane, acc, nterms);
+}
+
+// Persistent head-term kernel.  Flat tile f belongs to the last job with flat_begin <= f.
+template <int KIND>
+__global__ void __launch_bounds__(kBlock) k_head(const JobDev* __restrict__ jobs, int njobs,
+                                                 uint64_t total_tiles,
+                                                 unsigned long long* __restrict__ acc) {
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  const uint32_t wave =
+      __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  const uint32_t nwaves = (gridDim.x * blockDim.x) / kWave;
+  int cur = -1;
+  JobDev jb{};
+  U192 sum = {0, 0, 0};
+  uint64_t nterms = 0;
+  for (uint64_t f = wave; f < total_tiles; f += nwaves) {
+    int lo = (cur < 0) ? 0 : cur, hi = njobs - 1;  // jobs are visited in ascending order
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].flat_begin <= f)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    if (lo != cur) {
+      if (cur >= 0) flush_job(sum, nterms, jb.numesign, lane, acc + (size_t)jb.slot * kAccWords);
+      cur = lo;
+      jb = jobs[lo];
+      sum = {0, 0, 0};
+      nterms = 0;
+    }
+    const uint64_t kbase = (jb.tile_first + (f - jb.flat_begin)) * kTileK;
+    tile_terms<KIND>(jb, kbase, lane, sum, nterms);
+  }
+  if (cur >= 0) flush_job(sum, nterms, jb.numesign, lane, acc + (size_t)jb.slot * kAccWords);
+}
+
+// ------------------------------------------------------------------ tail terms
+
+// Tail k = d + i, i = lane < 20: value 2^(-10i)/m (Bellard.cl:1164-1239).  Term =
+// floor(2^(192-10i) / m) by bit-serial long division (exact for m < 2^63), complemented when
+// negative.  One wave per tail job.
+__global__ void __launch_bounds__(kWave) k_tail(const TailDev* __restrict__ tails,
+                                                unsigned long long* __restrict__ acc) {
+  const TailDev tj = tails[blockIdx.x];
+  const int lane = (int)threadIdx.x;
+  U192 q = {0, 0, 0};
+  if (lane < kTailTerms) {
+    const uint64_t k = tj.d + (uint64_t)lane;
+    const uint64_t m = tj.den0 * k + tj.den1;
+    const int top = 192 - 10 * lane;  // numerator 2^top; its leading step leaves remainder 1
+    uint64_t rem = 1;
+    for (int b = top - 1; b >= 0; --b) {
+      rem <<= 1;
+      if (rem >= m) {
+        rem -= m;
+        if (b >= 128)
+          q.w2 |= 1ull << (b - 128);
+        else if (b >= 64)
+          q.w1 |= 1ull << (b - 64);
+        else
+          q.w0 |= 1ull << b;
+      }
+    }
+    if ((tj.numesign + (uint32_t)(k & 1)) & 1u) q = {~q.w0, ~q.w1, ~q.w2};
+  }
+  wave_sum192(q);
+  if (lane == 0) atomic_add192(acc + (size_t)tj.slot * kAccWords, q);
+}
+
+// ------------------------------------------------------------------ combine
+
+__global__ void k_zero(unsigned long long* p, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+// Carry-propagate the six 32-bit limb sums of a slot into a 192-bit value.
+__device__ __forceinline__ U192 slot_value(const unsigned long long* s) {
+  uint64_t limb[6];
+  uint64_t c = 0;
+  for (int i = 0; i < 6; ++i) {
+    const uint64_t t = s[i] + c;  // s[i] < 2^63 while a slot takes fewer than 2^31 additions
+    limb[i] = t & 0xffffffffull;
+    c = t >> 32;
+  }
+  return {limb[0] | (limb[1] << 32), limb[2] | (limb[3] << 32), limb[4] | (limb[5] << 32)};
+}
+
+__device__ __forceinline__ U192 shl192(U192 v, int sh) {
+  if (sh == 0) return v;
+  return {v.w0 << sh, (v.w1 << sh) | (v.w0 >> (64 - sh)), (v.w2 << sh) | (v.w1 >> (64 - sh))};
+}
+
+__global__ void k_finalize(const unsigned long long* __restrict__ acc,
+                           const uint8_t* __restrict__ shift, const GroupDev* __restrict__ groups,
+                           int ngroups, uint64_t* __restrict__ out, int accumulate,
+                           uint64_t* __restrict__ slot_out) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngroups) return;
+  const GroupDev gr = groups[g];
+  U192 total = {0, 0, 0};
+  if (accumulate) total = {out[3 * g + 2], out[3 * g + 1], out[3 * g + 0]};
+  for (uint32_t i = 0; i < gr.count; ++i) {
+    const uint32_t s = gr.first + i;
+    const U192 v = shl192(slot_value(acc + (size_t)s * kAccWords), shift[s]);
+    if (slot_out) {
+      slot_out[3 * s + 0] = v.w2;
+      slot_out[3 * s + 1] = v.w1;
+      slot_out[3 * s + 2] = v.w0;
+    }
+    add192(total, v.w0, v.w1, v.w2);
+  }
+  out[3 * g + 0] = total.w2;
+  out[3 * g + 1] = total.w1;
+  out[3 * g + 2] = total.w0;
+}
+
+__global__ void k_sum7(uint64_t* ans, uint64_t* partial, int shift) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const U192 p = shl192({partial[2], partial[1], partial[0]}, shift);
+  partial[0] = p.w2;
+  partial[1] = p.w1;
+  partial[2] = p.w0;
+  U192 a = {ans[2], ans[1], ans[0]};
+  add192(a, p.w0, p.w1, p.w2);
+  ans[0] = a.w2;
+  ans[1] = a.w1;
+  ans[2] = a.w0;
+}
+
+// ------------------------------------------------------------------ launch wrappers
+
+hipError_t launch_head(int kind, const JobDev* jobs, int njobs, uint64_t total_tiles,
+                       unsigned long long* acc, int grid, hipStream_t stream) {
+  if (njobs <= 0 || total_tiles == 0) return hipSuccess;
+  switch (kind) {
+    case kSplitW3:
+      hipLaunchKernelGGL(k_head<kSplitW3>, dim3(grid), dim3(kBlock), 0, stream, jobs, njobs,
+                         total_tiles, acc);
+      break;
+    case kSplitW2:
+      hipLaunchKernelGGL(k_head<kSplitW2>, dim3(grid), dim3(kBlock), 0, stream, jobs, njobs,
+                         total_tiles, acc);
+      break;
+    case kSplitW1:
+      hipLaunchKernelGGL(k_head<kSplitW1>, dim3(grid), dim3(kBlock), 0, stream, jobs, njobs,
+                         total_tiles, acc);
+      break;
+    case kPlainW3:
+      hipLaunchKernelGGL(k_head<kPlainW3>, dim3(grid), dim3(kBlock), 0, stream, jobs, njobs,
+                         total_tiles, acc);
+      break;
+    case kPlainWide:
+      hipLaunchKernelGGL(k_head<kPlainWide>, dim3(grid), dim3(kBlock), 0, stream, jobs, njobs,
+                         total_tiles, acc);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_tail(const TailDev* tails, int ntails, unsigned long long* acc,
+                       hipStream_t stream) {
+  if (ntails <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tail, dim3(ntails), dim3(kWave), 0, stream, tails, acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(unsigned long long* p, uint64_t nwords, hipStream_t stream) {
+  if (nwords == 0) return hipSuccess;
+  const uint64_t blocks = (nwords + 255) / 256;
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0, stream,
+                     p, nwords);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(const unsigned long long* acc, const uint8_t* shift, int nslots,
+                           const GroupDev* groups, int ngroups, uint64_t* out, int accumulate,
+                           uint64_t* slot_out, hipStream_t stream) {
+  (void)nslots;
+  if (ngroups <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_finalize, dim3((ngroups + 63) / 64), dim3(64), 0, stream, acc, shift,
+                     groups, ngroups, out, accumulate, slot_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum7(uint64_t* ans, uint64_t* partial, int shift, hipStream_t stream) {
+  hipLaunchKernelGGL(k_sum7, dim3(1), dim3(64), 0, stream, ans, partial, shift);
+  return hipGetLastError();
+}
+
+}  // namespace clb

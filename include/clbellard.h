@@ -1,0 +1,151 @@
+/*
+ * clbellard.h -- C ABI of the MI355X-native Bellard-formula hex digit extraction library
+ * (libclbellard.so).  Plain C types only; no exceptions cross this boundary.
+ *
+ * Every entry point returns CLB_OK (0) or a negative CLB_E* code; clb_last_error() gives the
+ * message of the calling thread's last failure.  Outputs are caller-owned.  A clb_ctx owns the
+ * device buffers and the HIP stream of one GPU; calls on one context must not overlap, calls on
+ * different contexts may run from different threads.
+ *
+ * 192-bit values are uint64_t[3] in the reference's bigSum order: w[0] is the most significant
+ * word (Bellard.cl:1082-1084 "ulsum0"), the value is read as the binary fraction w / 2^192.
+ *
+ * Reference surface replaced by this header (toropippi/CLBellard):
+ *   Bellard_192.hsp      the host app: normalisation, normal mode, work mode, fsave
+ *   KernelFuncs.as       BatchCalc / AddAns / Set0mem* launch orchestration
+ *   HSPCL64.as + .dll    the HSP->OpenCL plugin calls on this path (HCLinit, HCLSetDev,
+ *                        HCLCreateBuffer, HCLSetKrns, HCLDoKrn1, HCLReadBuffer, ...)
+ *   Bellard.cl,
+ *   Bellard_div2by1.cl   the device kernels (Sglobal*_192*, Sglobal_after_192, Ssum_192, Sum7_192)
+ *   hexsademical.as      hexadecimal_i32 / hexadecimal_dd output formatting
+ *   workmodeSum0_6.hsp   the work-file summer
+ */
+#ifndef CLBELLARD_H
+#define CLBELLARD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLB_OK 0
+#define CLB_EINVAL -1  /* bad argument (digits out of range, bad formula, null pointer ...) */
+#define CLB_EDEVICE -2 /* HIP runtime / device failure */
+#define CLB_ERANGE -3  /* a modulus is beyond what the selected kernel supports */
+#define CLB_EIO -4     /* file could not be read or written */
+#define CLB_ENOMEM -5  /* host or device allocation failed */
+#define CLB_EFORMAT -6 /* malformed worklist or work file */
+
+/* Precision modes of the reference GUI, Bellard_192.hsp:64 ("106bit", "192bit", "192bit(beta)").
+ * All three evaluate every term exactly as floor(2^192 frac(+-2^(10(d-k))/m)), the per-term rule
+ * of the reference's all-integer kernel (Bellard_div2by1.cl:327-347); they differ only in how the
+ * result is written: CLB_PREC_DD106 emits the 28-hex-digit double-double layout of
+ * hexadecimal_dd, the two 192-bit modes emit all 48 digits. */
+#define CLB_PREC_DD106 0
+#define CLB_PREC_U192 1
+#define CLB_PREC_U192_BETA 2
+
+/* Term kernels.  Both produce bit-identical sums. */
+#define CLB_KERNEL_AUTO 0  /* SPLIT where every modulus allows it, else PLAIN */
+#define CLB_KERNEL_PLAIN 1 /* all-integer: 64-bit Montgomery on 32-bit limb products */
+#define CLB_KERNEL_SPLIT 2 /* FP64 error-free modexp + FP64-reciprocal quotient digits */
+
+typedef struct clb_ctx clb_ctx;
+
+/* One term-range job: head terms k in [k_begin, min(k_end, d)) of formula j at loop count d,
+ * plus the tail terms k = d..d+19 iff the range reaches d (KernelFuncs.as:56-59). */
+typedef struct {
+  uint64_t d;
+  uint64_t k_begin;
+  uint64_t k_end;
+  int32_t formula;
+  int32_t reserved;
+} clb_job;
+
+/* Message of this thread's last failed call ("" if none). */
+const char *clb_last_error(void);
+/* Library version string. */
+const char *clb_version(void);
+
+/* HCLinit + HCLGetDevCount (HSPCL64.as:11, :29; Bellard_192.hsp:16-17). */
+int clb_get_dev_count(int *count);
+/* HCLGetDeviceInfo(CL_DEVICE_NAME) (HSPCL64.as:21; Bellard_192.hsp:36-44, :528). */
+int clb_get_device_name(int device, char *name, size_t cap);
+
+/* HCLSetDev + LoadOpenCLFile + GlobalMemSet (Bellard_192.hsp:250-255, :376-420): selects the GPU,
+ * allocates the partial-sum and answer buffers, zeroes the answer. */
+int clb_ctx_create(int device, int precision, clb_ctx **out);
+int clb_ctx_destroy(clb_ctx *ctx);
+/* Kernel choice (CLB_KERNEL_*) for the following calls; default CLB_KERNEL_AUTO. */
+int clb_set_kernel(clb_ctx *ctx, int kernel);
+/* Launch shape: persistent workgroups per compute unit (0 = library default). */
+int clb_set_launch(clb_ctx *ctx, int blocks_per_cu);
+
+/* BuildOptions core (Bellard_192.hsp:449-450, :468-474): N -> odd loop count d and N' (a multiple
+ * of 5, the 1-based hex position of the first output digit); overdouble = (10d + 9 >= 2^53).
+ * CLB_EINVAL for N <= 4 or N > 999999999999999999. */
+int clb_normalize(uint64_t n_in, uint64_t *n_out, uint64_t *d_out, int *overdouble);
+/* numer_denom_init (KernelFuncs.as:3-13): c_j, sign flag (1 = minus), den0, den1 for j = 0..6. */
+int clb_formula(int j, uint64_t *numer, int64_t *numesign, uint64_t *den0, uint64_t *den1);
+
+/* Set0memUlong3 / Set0memdd (KernelFuncs.as:15-23): clear the formula partial (bigSumMem). */
+int clb_set0mem(clb_ctx *ctx);
+/* BatchCalc k_start,k_end,formulano (KernelFuncs.as:25-66): adds formula j's terms of
+ * [k_start, min(k_end, d)), plus the tail iff the range reaches d, into the partial. */
+int clb_batch_calc(clb_ctx *ctx, uint64_t d, uint64_t k_start, uint64_t k_end, int j);
+/* AddAns formulano (KernelFuncs.as:68-74 -> Sum7_192, Bellard.cl:1359-1398): partial <<= log2 c_j
+ * (mod 2^192, written back to the partial) and answer += partial. */
+int clb_add_ans(clb_ctx *ctx, int j);
+/* HCLReadBuffer of bigSumMem / dev_ans (Bellard_192.hsp:287-288, :292). */
+int clb_read_partial(clb_ctx *ctx, uint64_t w[3]);
+int clb_read_ans(clb_ctx *ctx, uint64_t w[3]);
+/* GlobalMemSet's answer clear (Bellard_192.hsp:413-419). */
+int clb_clear_ans(clb_ctx *ctx);
+
+/* Sglobal*mtg_192* kernel operator (Bellard.cl:771, :992; Bellard_div2by1.cl:313): adds the head
+ * terms k in [offset, k_max) of the formula (numesign, den0, den1) into the partial. */
+int clb_sglobal(clb_ctx *ctx, uint64_t offset, uint64_t d, uint64_t k_max, int64_t numesign,
+                uint64_t den0, uint64_t den1);
+/* Sglobal_after_192 kernel operator (Bellard.cl:1153-1253): adds the tail terms k = d..d+19. */
+int clb_sglobal_after(clb_ctx *ctx, uint64_t d, int64_t numesign, uint64_t den0, uint64_t den1);
+
+/* Allloop (Bellard_192.hsp:264-296): the full normal-mode computation at hex position N.
+ * frac = frac(16^(N'-1) pi) (192 bits), hex = the display string (precision dependent),
+ * n_out = N' (may be NULL). */
+int clb_normal(clb_ctx *ctx, uint64_t n, uint64_t frac[3], char hex[49], uint64_t *n_out);
+
+/* Many digit positions in one persistent launch (no reference counterpart; the reference
+ * computes one position per run).  frac[3*i..] = frac(16^(N'_i - 1) pi). */
+int clb_batch_positions(clb_ctx *ctx, const uint64_t *n, size_t count, uint64_t *frac);
+/* Many term-range jobs in one persistent launch (work mode's jobs batched, Bellard_192.hsp:
+ * 298-340).  partial[3*i..] = c_j * (job i's sum) mod 1, the value fsave writes. */
+int clb_run_jobs(clb_ctx *ctx, const clb_job *jobs, size_t count, uint64_t *partial);
+
+/* hexadecimal_i32 with the host word swap (Bellard_192.hsp:354-359, hexsademical.as:104-127). */
+int clb_hex48(const uint64_t w[3], char hex[49]);
+/* hexadecimal_dd (hexsademical.as:21-101): double-double -> 28 hex digits + 20 zeros. */
+int clb_hex_dd(const double dd[2], char hex[49]);
+/* The double-double (hi, lo) nearest to the 192-bit fraction w (round to nearest, 106 bits). */
+int clb_u192_to_dd(const uint64_t w[3], double dd[2]);
+
+/* Work mode (Bellard_192.hsp:88-340, :514-533).  Reads the worklist (digits=, precision=, j_b
+ * lines), runs every job whose range starts at or below d and whose file
+ * <work_dir>/<j>/<j>_<b>.txt does not exist yet, and writes each file.  batch_size 0 selects the
+ * reference's 2^37 terms per batch.  device_label is written on the file's second line.
+ * *jobs_run receives the number of files written (may be NULL). */
+int clb_run_worklist(clb_ctx *ctx, const char *worklist_path, const char *work_dir,
+                     uint64_t batch_size, const char *device_label, int *jobs_run);
+/* workmodeSum0_6 (workmodeSum0_6.hsp:1-49): for j = 0..6 and b = 0, 1, ... until the first missing
+ * file, adds line 0 of <work_dir>/<j>/<j>_<b>.txt mod 16^48.  counts[j] = files used. */
+int clb_work_sum(const char *work_dir, char hex[49], int counts[7]);
+
+/* Timing of the last clb_normal / clb_batch_positions / clb_run_jobs / clb_batch_calc call on
+ * the context's stream (HIP events): the head-term kernel and the whole call, in ms. */
+int clb_last_timing(clb_ctx *ctx, double *head_kernel_ms, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

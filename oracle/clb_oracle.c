@@ -1,0 +1,447 @@
+/*
+ * clb_oracle.c -- CPU restatement of CLBellard's exact-integer digit-extraction path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Only tests/, the smoke check and bench.py's
+ * cpu_baseline leg load this library; the product never does.  See clb_oracle.h for the
+ * word order, the exact-integer semantics and the pinning status of each function.
+ */
+#include "clb_oracle.h"
+
+#include <pthread.h>
+#include <stdio.h>
+#include <string.h>
+
+typedef unsigned __int128 u128;
+
+/* ---------------------------------------------------------------- host-side constants */
+
+/* Bellard_192.hsp:449-450, :468-474. */
+int orc_normalize(uint64_t n_in, uint64_t *n_out, uint64_t *d_out, int *overdouble) {
+  if (n_in <= 4 || n_in > 999999999999999999ull) return -1;
+  uint64_t d = ((n_in - 1) * 2 - 3) / 5;
+  if (d % 2 == 0) d--;
+  if (n_out) *n_out = (d * 5 + 3) / 2 + 1;
+  if (d_out) *d_out = d;
+  if (overdouble) *overdouble = (d * 10 + 9 >= 9007199254740992ull);
+  return 0;
+}
+
+/* KernelFuncs.as:9-12 -- the seven terms of Bellard's formula. */
+static const uint64_t kNumer[7] = {32, 1, 256, 64, 4, 4, 1};
+static const int64_t kNumeSign[7] = {1, 1, 0, 1, 1, 1, 0};
+static const uint64_t kDen0[7] = {4, 4, 10, 10, 10, 10, 10};
+static const uint64_t kDen1[7] = {1, 3, 1, 3, 5, 7, 9};
+
+int orc_formula(int j, uint64_t *numer, int64_t *numesign, uint64_t *den0, uint64_t *den1) {
+  if (j < 0 || j > 6) return -1;
+  if (numer) *numer = kNumer[j];
+  if (numesign) *numesign = kNumeSign[j];
+  if (den0) *den0 = kDen0[j];
+  if (den1) *den1 = kDen1[j];
+  return 0;
+}
+
+/* ---------------------------------------------------------------- Montgomery arithmetic */
+
+/* Bellard.cl:211-215: Newton iteration x <- x(2 - m x) from x = m (3 correct bits), 4 steps. */
+uint32_t orc_modinv32(uint32_t m) {
+  uint32_t inv = m;
+  for (int i = 0; i < 4; ++i) inv *= 2u - inv * m;
+  return (uint32_t)0 - inv;
+}
+
+/* Bellard_div2by1.cl:255-262: same iteration on 64 bits, 6 steps. */
+uint64_t orc_modinv64(uint64_t m) {
+  uint64_t inv = m;
+  for (int i = 0; i < 6; ++i) inv *= (uint64_t)2 - inv * m;
+  return (uint64_t)0 - inv;
+}
+
+/* Bellard.cl:380-391.  x = xhi*2^32 + xlo < m*2^32; returns x * 2^-32 mod m in [0, m). */
+uint32_t orc_mr32(uint32_t xlo, uint32_t xhi, uint32_t inv, uint32_t m) {
+  uint32_t q = xlo * inv;
+  uint32_t t = (uint32_t)(((uint64_t)q * m) >> 32);
+  if (xlo != 0) t++; /* xlo + (q*m mod 2^32) is 0 or exactly 2^32 */
+  uint32_t s = t + xhi;
+  if (s >= m || s < xhi) s -= m;
+  return s;
+}
+
+/* Bellard_div2by1.cl:264-274, the 64-bit analogue of orc_mr32. */
+uint64_t orc_mr64(uint64_t xlo, uint64_t xhi, uint64_t inv, uint64_t m) {
+  uint64_t q = xlo * inv;
+  uint64_t t = (uint64_t)(((u128)q * m) >> 64);
+  if (xlo != 0) t++;
+  uint64_t s = t + xhi;
+  if (s >= m || s < xhi) s -= m;
+  return s;
+}
+
+static inline uint32_t mr32_mul(uint32_t a, uint32_t b, uint32_t inv, uint32_t m) {
+  uint64_t p = (uint64_t)a * b;
+  return orc_mr32((uint32_t)p, (uint32_t)(p >> 32), inv, m);
+}
+
+static inline uint64_t mr64_mul(uint64_t a, uint64_t b, uint64_t inv, uint64_t m) {
+  u128 p = (u128)a * b;
+  return orc_mr64((uint64_t)p, (uint64_t)(p >> 64), inv, m);
+}
+
+/* Bellard.cl:395-408: right-to-left binary powering of 1024 in Montgomery form. */
+uint32_t orc_expmod32(uint64_t n, uint32_t m, uint32_t inv, uint32_t r2) {
+  uint32_t p = orc_mr32(r2 << 10, r2 >> 22, inv, m); /* mont(1024) */
+  uint32_t x = orc_mr32(r2, 0, inv, m);              /* mont(1) */
+  for (int i = 0; i < 64; ++i) {
+    if (n & 1) x = mr32_mul(x, p, inv, m);
+    p = mr32_mul(p, p, inv, m);
+    n >>= 1;
+    if (n == 0) break;
+  }
+  return orc_mr32(x, 0, inv, m);
+}
+
+/* Bellard_div2by1.cl:276-305: left-to-right, fixed 3-bit windows, table b[i] = mont(1024^i). */
+uint64_t orc_expmod64(uint64_t n, uint64_t m, uint64_t inv, uint64_t r2) {
+  if (n == 0) return 1 % m;
+  uint64_t b[8];
+  b[0] = orc_mr64(r2, 0, inv, m);
+  b[1] = orc_mr64(r2 << 10, r2 >> 54, inv, m);
+  for (int i = 2; i < 8; ++i) b[i] = mr64_mul(b[i - 1], b[1], inv, m);
+  int bits = 64 - __builtin_clzll(n);
+  int first = bits % 3;
+  if (first == 0) first = 3;
+  int pos = bits - first;
+  uint64_t x = b[(n >> pos) & ((1u << first) - 1)];
+  while (pos > 0) {
+    x = mr64_mul(x, x, inv, m);
+    x = mr64_mul(x, x, inv, m);
+    x = mr64_mul(x, x, inv, m);
+    pos -= 3;
+    unsigned digit = (unsigned)((n >> pos) & 7);
+    if (digit) x = mr64_mul(x, b[digit], inv, m);
+  }
+  return orc_mr64(x, 0, inv, m);
+}
+
+/* ---------------------------------------------------------------- division by invariant integers */
+
+/* Moller & Granlund, "Improved division by invariant integers" (2011), Algorithm 2, as used at
+ * Bellard_div2by1.cl:105-156 with the table entry computed by division (CREATETABLEFLAG,
+ * Bellard_div2by1.cl:1, :99-101).  d must have its top bit set; returns floor((2^128-1)/d) - 2^64. */
+uint64_t orc_reciprocal_word(uint64_t d) {
+  const uint64_t d0 = d & 1, d9 = d >> 55, d40 = (d >> 24) + 1, d63 = (d >> 1) + d0;
+  const uint64_t v0 = ((1u << 19) - 3u * (1u << 8)) / d9;
+  const uint64_t v1 = (v0 << 11) - ((v0 * v0 * d40) >> 40) - 1;
+  const uint64_t v2 = (v1 << 13) + ((v1 * ((1ull << 60) - v1 * d40)) >> 47);
+  const uint64_t e = (v2 >> 1) * d0 - v2 * d63;
+  const uint64_t v3 = (v2 << 31) + ((uint64_t)(((u128)v2 * e) >> 64) >> 1);
+  const u128 p = (u128)v3 * d;
+  const uint64_t lo = (uint64_t)p, hi = (uint64_t)(p >> 64);
+  return v3 - (hi + d + (lo + d < lo));
+}
+
+/* Algorithm 4 of the same paper (Bellard_div2by1.cl:162-177): (u1:u0) / d with u1 < d, d
+ * normalized, v = orc_reciprocal_word(d).  Returns the quotient, *r the remainder. */
+uint64_t orc_div2by1(uint64_t u1, uint64_t u0, uint64_t d, uint64_t *r, uint64_t v) {
+  u128 q = (u128)v * u1 + (((u128)u1 << 64) | u0);
+  uint64_t q1 = (uint64_t)(q >> 64) + 1, q0 = (uint64_t)q;
+  uint64_t rem = u0 - q1 * d;
+  if (rem > q0) {
+    q1--;
+    rem += d;
+  }
+  if (rem >= d) {
+    q1++;
+    rem -= d;
+  }
+  *r = rem;
+  return q1;
+}
+
+static inline uint64_t shr64(uint64_t x, int n) { return n < 64 ? x >> n : 0; }
+
+/* Bellard_div2by1.cl:195-211: schoolbook n-by-1 division with the normalized divisor. */
+void orc_divnby1(int n, const uint64_t *u, uint64_t d, uint64_t *q) {
+  const int l = __builtin_clzll(d);
+  d <<= l;
+  const uint64_t v = orc_reciprocal_word(d);
+  uint64_t k = shr64(u[n - 1], 64 - l);
+  for (int i = n - 1; i >= 1; --i) {
+    uint64_t ui = (u[i] << l) | shr64(u[i - 1], 64 - l);
+    q[i] = orc_div2by1(k, ui, d, &k, v);
+  }
+  q[0] = orc_div2by1(k, u[0] << l, d, &k, v);
+}
+
+/* Bellard_div2by1.cl:229-235: x*y mod n with n << s normalized and v its reciprocal. */
+uint64_t orc_mulmodn(uint64_t x, uint64_t y, uint64_t n, int s, uint64_t v) {
+  u128 p = (u128)x * y;
+  uint64_t hi = (uint64_t)(p >> 64), lo = (uint64_t)p, r;
+  orc_div2by1((hi << s) | shr64(lo, 64 - s), lo << s, n << s, &r, v);
+  return r >> s;
+}
+
+/* ---------------------------------------------------------------- per-term evaluation */
+
+uint64_t orc_pow1024_mod(uint64_t n, uint64_t m) {
+  if (m < (1ull << 32)) {
+    /* Bellard.cl:785-788: R = 2^32, r2 = 2^64 mod m, with the m == 1 guard of :787. */
+    uint32_t inv = orc_modinv32((uint32_t)m);
+    uint32_t r2 = (uint32_t)(UINT64_MAX % m) + 1;
+    if (r2 == (uint32_t)m) r2 = 0;
+    return orc_expmod32(n, (uint32_t)m, inv, r2);
+  }
+  /* Bellard_div2by1.cl:329-333: R = 2^64, r2 = R^2 mod m through the reciprocal. */
+  int s = __builtin_clzll(m);
+  uint64_t v = orc_reciprocal_word(m << s);
+  uint64_t r1 = UINT64_MAX % m + 1;
+  uint64_t r2 = orc_mulmodn(r1, r1, m, s, v);
+  return orc_expmod64(n, m, orc_modinv64(m), r2);
+}
+
+/* Bellard_div2by1.cl:329-341.  Output in bigSum order (q[0] = most significant word). */
+void orc_term192(uint64_t d, uint64_t k, int64_t numesign, uint64_t den0, uint64_t den1,
+                 uint64_t q[3]) {
+  const uint64_t m = den0 * k + den1;
+  uint64_t r = orc_pow1024_mod(d - k, m);
+  if ((numesign + (int64_t)(k % 2)) % 2 == 1) r = m - r; /* :335-337 */
+  const uint64_t u[4] = {0, 0, 0, r};
+  uint64_t w[4];
+  orc_divnby1(4, u, m, w); /* w[3] = floor(r/m) is 0, or 1 when r == m (dropped mod 2^192) */
+  q[0] = w[2];
+  q[1] = w[1];
+  q[2] = w[0];
+}
+
+void orc_add192(uint64_t acc[3], const uint64_t q[3]) {
+  uint64_t s2 = acc[2] + q[2];
+  uint64_t c = s2 < q[2];
+  uint64_t s1 = acc[1] + q[1];
+  uint64_t c1 = s1 < q[1];
+  s1 += c;
+  c1 |= (s1 < c);
+  acc[0] = acc[0] + q[0] + c1;
+  acc[1] = s1;
+  acc[2] = s2;
+}
+
+/* Bellard.cl:1153-1253 computes the terms k = d+i, i = 0..19, whose value is
+ * (-1)^(k + numesign) 2^(-10i) / m.  Each term here is floor(2^192 * frac(value)), the same
+ * per-term rule the head uses: floor(2^(192-10i)/m), or its 192-bit complement when negative. */
+void orc_tail192(uint64_t d, int64_t numesign, uint64_t den0, uint64_t den1, uint64_t out[3]) {
+  out[0] = out[1] = out[2] = 0;
+  for (int i = 0; i < 20; ++i) {
+    const uint64_t k = d + (uint64_t)i;
+    const uint64_t m = den0 * k + den1;
+    const int e = 192 - 10 * i;
+    uint64_t u[4] = {0, 0, 0, 0};
+    u[e / 64] = 1ull << (e % 64);
+    uint64_t w[4];
+    orc_divnby1(4, u, m, w);
+    uint64_t q[3] = {w[2], w[1], w[0]};
+    if ((numesign + (int64_t)(k % 2)) % 2 == 1) {
+      q[0] = ~q[0];
+      q[1] = ~q[1];
+      q[2] = ~q[2];
+    }
+    orc_add192(out, q);
+  }
+}
+
+/* Bellard.cl:1359-1398. */
+void orc_sum7_192(uint64_t ans[3], uint64_t part[3], uint64_t numer) {
+  int sht = 0;
+  if (numer == 4) sht = 2;
+  if (numer == 32) sht = 5;
+  if (numer == 64) sht = 6;
+  if (numer == 256) sht = 8;
+  if (sht) {
+    part[0] = (part[0] << sht) | (part[1] >> (64 - sht));
+    part[1] = (part[1] << sht) | (part[2] >> (64 - sht));
+    part[2] = part[2] << sht;
+  }
+  orc_add192(ans, part);
+}
+
+/* ---------------------------------------------------------------- range sums */
+
+struct range_job {
+  uint64_t d, k0, k1, step;
+  int64_t numesign;
+  uint64_t den0, den1;
+  uint64_t acc[3];
+};
+
+static void *range_worker(void *arg) {
+  struct range_job *jb = (struct range_job *)arg;
+  jb->acc[0] = jb->acc[1] = jb->acc[2] = 0;
+  for (uint64_t k = jb->k0; k < jb->k1; k += jb->step) {
+    uint64_t q[3];
+    orc_term192(jb->d, k, jb->numesign, jb->den0, jb->den1, q);
+    orc_add192(jb->acc, q);
+  }
+  return NULL;
+}
+
+void orc_range_sum(uint64_t d, uint64_t k0, uint64_t k1, int j, int nthreads, uint64_t out[3]) {
+  out[0] = out[1] = out[2] = 0;
+  if (k1 <= k0) return;
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  if ((uint64_t)nthreads > k1 - k0) nthreads = (int)(k1 - k0);
+  struct range_job jobs[256];
+  pthread_t th[256];
+  for (int t = 0; t < nthreads; ++t) {
+    jobs[t].d = d;
+    jobs[t].k0 = k0 + (uint64_t)t;
+    jobs[t].k1 = k1;
+    jobs[t].step = (uint64_t)nthreads;
+    jobs[t].numesign = kNumeSign[j];
+    jobs[t].den0 = kDen0[j];
+    jobs[t].den1 = kDen1[j];
+    if (nthreads == 1 || pthread_create(&th[t], NULL, range_worker, &jobs[t]) != 0) {
+      range_worker(&jobs[t]);
+      th[t] = 0;
+    }
+  }
+  for (int t = 0; t < nthreads; ++t) {
+    if (th[t]) pthread_join(th[t], NULL);
+    orc_add192(out, jobs[t].acc);
+  }
+}
+
+/* KernelFuncs.as:25-66.  The launch chunking (GWS*16 / GWS*32 terms) does not change an exact
+ * sum, so only the range boundaries and the tail condition (offset == d, :56-59) matter. */
+void orc_batch_calc(uint64_t d, uint64_t k_start, uint64_t k_end, int j, int nthreads,
+                    uint64_t out[3]) {
+  const uint64_t hi = k_end < d ? k_end : d;
+  uint64_t offset = k_start;
+  out[0] = out[1] = out[2] = 0;
+  if (k_start < hi) {
+    orc_range_sum(d, k_start, hi, j, nthreads, out);
+    offset = hi;
+  }
+  if (offset == d) {
+    uint64_t t[3];
+    orc_tail192(d, kNumeSign[j], kDen0[j], kDen1[j], t);
+    orc_add192(out, t);
+  }
+}
+
+/* Bellard_192.hsp:264-296. */
+int orc_normal(uint64_t n, int nthreads, uint64_t ans[3]) {
+  uint64_t d;
+  int rc = orc_normalize(n, NULL, &d, NULL);
+  ans[0] = ans[1] = ans[2] = 0;
+  if (rc) return rc;
+  for (int j = 0; j < 7; ++j) {
+    uint64_t part[3];
+    orc_batch_calc(d, 0, d, j, nthreads, part);
+    orc_sum7_192(ans, part, kNumer[j]);
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------- formats */
+
+void orc_hex48(const uint64_t w[3], char out[49]) {
+  static const char hx[] = "0123456789abcdef";
+  for (int i = 0; i < 3; ++i)
+    for (int b = 0; b < 16; ++b) out[16 * i + b] = hx[(w[i] >> (60 - 4 * b)) & 15];
+  out[48] = 0;
+}
+
+static inline void dd_twosum(double a, double b, double *x, double *y) {
+  double s = a + b;
+  double t = s - a;
+  *y = (a - (s - t)) + (b - t);
+  *x = s;
+}
+
+/* hexsademical.as:12-19 (addcab): (a.0, a.1) + (b.0, b.1). */
+static void dd_add(double *h, double *l, double bh, double bl) {
+  double x, y;
+  dd_twosum(*h, bh, &x, &y);
+  y = y + *l + bl;
+  dd_twosum(x, y, h, l);
+}
+
+/* hexsademical.as:21-101.  outdd is six 32-bit HSP integers viewed as 24 little-endian bytes;
+ * bytes 23..10 receive the 14 emitted base-256 digits, with carries and borrows propagated
+ * into the already emitted (more significant) bytes. */
+void orc_hex_dd(const double dd[2], char out[49]) {
+  double h = dd[0], l = dd[1];
+  uint32_t outdd[6] = {0, 0, 0, 0, 0, 0};
+  if (h < 0.0) dd_add(&h, &l, 1.0, 0.0);
+  for (int cnt = 0; cnt < 14; ++cnt) {
+    h *= 256.0;
+    l *= 256.0;
+    dd_twosum(h, l, &h, &l);
+    int moji = (int)h;
+    if (moji < 256 && h >= 0.0) {
+      ((uint8_t *)outdd)[23 - cnt] = (uint8_t)moji;
+      h -= moji;
+    } else {
+      if (moji == 256) {
+        int idx = 23 - cnt + 1;
+        if (idx < 24) {
+          int bidx = idx / 4;
+          uint32_t addv = 1u << ((idx % 4) * 8);
+          outdd[bidx] += addv;
+          for (int rep = 5 - bidx; rep > 0; --rep) {
+            if ((int32_t)outdd[bidx] >= 0 && (int32_t)outdd[bidx] < (int32_t)addv) {
+              bidx++;
+              addv = 1;
+              outdd[bidx] += addv;
+            } else {
+              break;
+            }
+          }
+        }
+        ((uint8_t *)outdd)[23 - cnt] = 0;
+        h -= 256.0;
+      }
+      if (h < 0.0) {
+        moji = (int)(h + 256.0);
+        int idx = 23 - cnt + 1;
+        if (idx < 24) {
+          int bidx = idx / 4;
+          uint32_t addv = 1u << ((idx % 4) * 8);
+          outdd[bidx] -= addv;
+          for (int rep = 5 - bidx; rep > 0; --rep) {
+            if ((int32_t)outdd[bidx] < 0 && (int32_t)outdd[bidx] >= -(int32_t)addv) {
+              bidx++;
+              addv = 1;
+              outdd[bidx] -= addv;
+            } else {
+              break;
+            }
+          }
+        }
+        ((uint8_t *)outdd)[23 - cnt] = (uint8_t)moji;
+        dd_add(&h, &l, 256.0, 0.0);
+      }
+    }
+  }
+  static const char hx[] = "0123456789abcdef";
+  const uint8_t *bytes = (const uint8_t *)outdd;
+  for (int i = 0; i < 24; ++i) {
+    out[2 * i] = hx[bytes[23 - i] >> 4];
+    out[2 * i + 1] = hx[bytes[23 - i] & 15];
+  }
+  out[48] = 0;
+}
+
+/* workmodeSum0_6.hsp:21-28: lowercase hex only ('a'..'f' map through "moji > 10: moji -= 39"),
+ * single carry per digit, the carry out of digit 0 dropped (mod 16^48). */
+void orc_work_sum_add(int acc[48], const char *line48) {
+  for (int cnt = 0; cnt < 48; ++cnt) {
+    int moji = (unsigned char)line48[47 - cnt] - 48;
+    if (moji > 10) moji -= 39;
+    acc[47 - cnt] += moji;
+    if (acc[47 - cnt] >= 16) {
+      acc[47 - cnt] -= 16;
+      if (cnt != 47) acc[46 - cnt]++;
+    }
+  }
+}

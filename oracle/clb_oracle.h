@@ -1,0 +1,105 @@
+/*
+ * clb_oracle.h -- CPU restatement of CLBellard's exact-integer digit-extraction path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/, the smoke check in
+ * __graft_entry__.py and the cpu_baseline leg of bench.py may load it.  The product
+ * (clbellard_amd, libclbellard.so) never links, loads or calls it.
+ *
+ * Every function follows one piece of the reference and cites it (file:line).  192-bit values
+ * are uint64_t[3] in the reference's bigSum order: w[0] = most significant word ("ulsum0"),
+ * w[2] = least significant word ("ulsum2"), value = w[0]*2^128 + w[1]*2^64 + w[2], read as the
+ * binary fraction value / 2^192.
+ *
+ * Exact-integer semantics.  The reference's "192bit(beta)" mode evaluates every head term with
+ * m >= 2^32 as floor(r' * 2^192 / m) (Bellard_div2by1.cl:313-359), where r = 1024^(d-k) mod m
+ * and r' = r for a positive term, m - r for a negative one.  This oracle applies that same rule
+ * to every term: the head terms with m < 2^32 (where the reference keeps double-double sums,
+ * Bellard.cl:771-823) and the 20 tail terms k = d..d+19 (Bellard.cl:1153-1253), whose value is
+ * 2^(-10i)/m.  The result is independent of how the term range is partitioned.
+ *
+ * Pinning status (checked in tests/test_oracle.py):
+ *   orc_normalize, orc_formula          closed form, reference presets
+ *   orc_modinv32/64, orc_mr32/64,
+ *   orc_expmod32/64, orc_pow1024_mod    brute force pow(2, 10n, m), random moduli < 2^62
+ *   orc_reciprocal_word, orc_div2by1,
+ *   orc_divnby1, orc_mulmodn            exact big-integer division
+ *   orc_term192, orc_tail192            big-integer floor(r' 2^192 / m)
+ *   orc_range_sum, orc_batch_calc       brute-force big-integer sums; partition invariance
+ *   orc_sum7_192                        (c * S) mod 2^192
+ *   orc_normal                          hex digits of pi from mpmath (tests/golden/)
+ *   orc_hex48                           printf("%016llx") layout
+ *   orc_work_sum_add                    big-integer sum mod 16^48
+ *   orc_hex_dd                          parity unpinned below 2^-100 (reference formatting of
+ *                                       double-double values is only pinned on exact inputs)
+ */
+#ifndef CLB_ORACLE_H
+#define CLB_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Bellard_192.hsp:449-450 and :468-474.  Returns 0, or -1 for N <= 4 or N > 999999999999999999. */
+int orc_normalize(uint64_t n_in, uint64_t *n_out, uint64_t *d_out, int *overdouble);
+
+/* KernelFuncs.as:3-13: numerator c_j, sign flag (1 = minus), denominator den0*k + den1. */
+int orc_formula(int j, uint64_t *numer, int64_t *numesign, uint64_t *den0, uint64_t *den1);
+
+/* Bellard.cl:192-217 (modinv32) and Bellard_div2by1.cl:255-262 (modinv64_fast): -m^-1 mod 2^w. */
+uint32_t orc_modinv32(uint32_t m);
+uint64_t orc_modinv64(uint64_t m);
+
+/* Bellard.cl:380-391 (MR32) and Bellard_div2by1.cl:264-274 (MR64_fast): Montgomery REDC. */
+uint32_t orc_mr32(uint32_t xlo, uint32_t xhi, uint32_t inv, uint32_t m);
+uint64_t orc_mr64(uint64_t xlo, uint64_t xhi, uint64_t inv, uint64_t m);
+
+/* Bellard.cl:395-408 (ExpMod32v4): 1024^n mod m, R = 2^32, r2 = R^2 mod m. */
+uint32_t orc_expmod32(uint64_t n, uint32_t m, uint32_t inv, uint32_t r2);
+/* Bellard_div2by1.cl:276-305 (ExpMod64v4w3): 1024^n mod m, R = 2^64, 3-bit window. */
+uint64_t orc_expmod64(uint64_t n, uint64_t m, uint64_t inv, uint64_t r2);
+
+/* Moller-Granlund 2011, Alg. 2 / Alg. 4, as used at Bellard_div2by1.cl:105-177. */
+uint64_t orc_reciprocal_word(uint64_t d);
+uint64_t orc_div2by1(uint64_t u1, uint64_t u0, uint64_t d, uint64_t *r, uint64_t v);
+/* Bellard_div2by1.cl:195-211: q[0..n-1] = floor(u / d), u and q least significant word first. */
+void orc_divnby1(int n, const uint64_t *u, uint64_t d, uint64_t *q);
+/* Bellard_div2by1.cl:229-235. */
+uint64_t orc_mulmodn(uint64_t x, uint64_t y, uint64_t n, int s, uint64_t v);
+
+/* 1024^n mod m with the reference's per-term setup: the 32-bit Montgomery path of
+ * Bellard.cl:785-788 when m < 2^32, else Bellard_div2by1.cl:329-333. */
+uint64_t orc_pow1024_mod(uint64_t n, uint64_t m);
+
+/* One head term, Bellard_div2by1.cl:329-341: q = floor(r' * 2^192 / m) mod 2^192. */
+void orc_term192(uint64_t d, uint64_t k, int64_t numesign, uint64_t den0, uint64_t den1,
+                 uint64_t q[3]);
+/* 192-bit add mod 2^192, Bellard_div2by1.cl:342-347 (with a plain carry chain). */
+void orc_add192(uint64_t acc[3], const uint64_t q[3]);
+/* Tail terms k = d..d+19, Bellard.cl:1153-1253, each as floor(2^192 * frac(+-2^(-10i)/m)). */
+void orc_tail192(uint64_t d, int64_t numesign, uint64_t den0, uint64_t den1, uint64_t out[3]);
+/* Bellard.cl:1359-1398 (Sum7_192): part <<= log2(numer) mod 2^192 (written back), ans += part. */
+void orc_sum7_192(uint64_t ans[3], uint64_t part[3], uint64_t numer);
+
+/* Sum of head terms k in [k0, k1) for formula j (k1 <= d), split over nthreads host threads. */
+void orc_range_sum(uint64_t d, uint64_t k0, uint64_t k1, int j, int nthreads, uint64_t out[3]);
+/* KernelFuncs.as:25-66 (BatchCalc): head terms of [k_start, min(k_end, d)), plus the tail iff
+ * the range reaches d; out = the formula's partial sum (before Sum7). */
+void orc_batch_calc(uint64_t d, uint64_t k_start, uint64_t k_end, int j, int nthreads,
+                    uint64_t out[3]);
+/* Bellard_192.hsp:264-296 (Allloop): frac(16^(N'-1) * pi) as 192 bits. Returns orc_normalize's code. */
+int orc_normal(uint64_t n, int nthreads, uint64_t ans[3]);
+
+/* Bellard_192.hsp:354-359 + hexsademical.as:104-127: 48 lowercase hex digits, w[0] first. */
+void orc_hex48(const uint64_t w[3], char out[49]);
+/* hexsademical.as:21-101 (hexadecimal_dd): double-double -> 28 hex digits + 20 zeros. */
+void orc_hex_dd(const double dd[2], char out[49]);
+/* workmodeSum0_6.hsp:21-28: acc (48 digit values 0..15, most significant first) += line. */
+void orc_work_sum_add(int acc[48], const char *line48);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

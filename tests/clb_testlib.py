@@ -1,0 +1,218 @@
+"""Shared test helpers: the CPU oracle (oracle/liboracle.so) and exact big-integer references.
+
+The oracle is test infrastructure (the checker); the product library never loads it.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+import subprocess
+from fractions import Fraction
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+M192 = (1 << 192) - 1
+NUMER = (32, 1, 256, 64, 4, 4, 1)
+NUMESIGN = (1, 1, 0, 1, 1, 1, 0)
+DEN0 = (4, 4, 10, 10, 10, 10, 10)
+DEN1 = (1, 3, 1, 3, 5, 7, 9)
+SHIFT = (5, 0, 8, 6, 2, 2, 0)
+
+U3 = ctypes.c_uint64 * 3
+U4 = ctypes.c_uint64 * 4
+
+
+class Oracle:
+    """ctypes view of oracle/clb_oracle.h."""
+
+    def __init__(self, path: str):
+        lib = ctypes.CDLL(path)
+        u64, i64, c_int = ctypes.c_uint64, ctypes.c_int64, ctypes.c_int
+        p64 = ctypes.POINTER(ctypes.c_uint64)
+        sig = {
+            "orc_normalize": ([u64, p64, p64, ctypes.POINTER(c_int)], c_int),
+            "orc_formula": ([c_int, p64, ctypes.POINTER(i64), p64, p64], c_int),
+            "orc_modinv32": ([ctypes.c_uint32], ctypes.c_uint32),
+            "orc_modinv64": ([u64], u64),
+            "orc_mr32": ([ctypes.c_uint32] * 4, ctypes.c_uint32),
+            "orc_mr64": ([u64] * 4, u64),
+            "orc_expmod32": ([u64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32],
+                             ctypes.c_uint32),
+            "orc_expmod64": ([u64] * 4, u64),
+            "orc_reciprocal_word": ([u64], u64),
+            "orc_div2by1": ([u64, u64, u64, p64, u64], u64),
+            "orc_divnby1": ([c_int, p64, u64, p64], None),
+            "orc_mulmodn": ([u64, u64, u64, c_int, u64], u64),
+            "orc_pow1024_mod": ([u64, u64], u64),
+            "orc_term192": ([u64, u64, i64, u64, u64, p64], None),
+            "orc_add192": ([p64, p64], None),
+            "orc_tail192": ([u64, i64, u64, u64, p64], None),
+            "orc_sum7_192": ([p64, p64, u64], None),
+            "orc_range_sum": ([u64, u64, u64, c_int, c_int, p64], None),
+            "orc_batch_calc": ([u64, u64, u64, c_int, c_int, p64], None),
+            "orc_normal": ([u64, c_int, p64], c_int),
+            "orc_hex48": ([p64, ctypes.c_char_p], None),
+            "orc_hex_dd": ([ctypes.POINTER(ctypes.c_double), ctypes.c_char_p], None),
+            "orc_work_sum_add": ([ctypes.POINTER(ctypes.c_int), ctypes.c_char_p], None),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        self.lib = lib
+        self.threads = max(1, min(16, os.cpu_count() or 1))
+
+    # -- wrappers returning Python ints (192-bit values as integers)
+    def normalize(self, n):
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int()
+        rc = self.lib.orc_normalize(n, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return None if rc else (a.value, b.value, bool(c.value))
+
+    def term192(self, d, k, numesign, den0, den1):
+        q = U3()
+        self.lib.orc_term192(d, k, numesign, den0, den1, q)
+        return w2i(q)
+
+    def tail192(self, d, numesign, den0, den1):
+        q = U3()
+        self.lib.orc_tail192(d, numesign, den0, den1, q)
+        return w2i(q)
+
+    def range_sum(self, d, k0, k1, j, threads=None):
+        q = U3()
+        self.lib.orc_range_sum(d, k0, k1, j, threads or self.threads, q)
+        return w2i(q)
+
+    def batch_calc(self, d, k0, k1, j, threads=None):
+        q = U3()
+        self.lib.orc_batch_calc(d, k0, k1, j, threads or self.threads, q)
+        return w2i(q)
+
+    def normal(self, n, threads=None):
+        q = U3()
+        rc = self.lib.orc_normal(n, threads or self.threads, q)
+        assert rc == 0
+        return w2i(q)
+
+    def sum7(self, ans, part, numer):
+        a, p = U3(*i2w(ans)), U3(*i2w(part))
+        self.lib.orc_sum7_192(a, p, numer)
+        return w2i(a), w2i(p)
+
+    def hex48(self, v):
+        buf = ctypes.create_string_buffer(49)
+        self.lib.orc_hex48(U3(*i2w(v)), buf)
+        return buf.value.decode()
+
+    def hex_dd(self, hi, lo):
+        buf = ctypes.create_string_buffer(49)
+        self.lib.orc_hex_dd((ctypes.c_double * 2)(hi, lo), buf)
+        return buf.value.decode()
+
+    def pow1024_mod(self, n, m):
+        return self.lib.orc_pow1024_mod(n, m)
+
+
+def w2i(w) -> int:
+    return (int(w[0]) << 128) | (int(w[1]) << 64) | int(w[2])
+
+
+def i2w(v: int):
+    v &= M192
+    return (v >> 128, (v >> 64) & (2**64 - 1), v & (2**64 - 1))
+
+
+def build_oracle() -> None:
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def load_oracle() -> Oracle:
+    src = os.path.join(ORACLE_DIR, "clb_oracle.c")
+    if not os.path.exists(ORACLE_LIB) or os.path.getmtime(ORACLE_LIB) < os.path.getmtime(src):
+        build_oracle()
+    return Oracle(ORACLE_LIB)
+
+
+def load_pi_fixtures():
+    with open(os.path.join(GOLDEN, "pi_hex.json")) as f:
+        return json.load(f)
+
+
+# ----------------------------------------------------------------- exact big-integer references
+
+def py_normalize(n: int):
+    d = ((n - 1) * 2 - 3) // 5
+    if d % 2 == 0:
+        d -= 1
+    return (d * 5 + 3) // 2 + 1, d
+
+
+def py_term(d: int, k: int, numesign: int, den0: int, den1: int) -> int:
+    """floor(2^192 frac(+-1024^(d-k)/m)) for a head term k < d."""
+    m = den0 * k + den1
+    r = pow(2, 10 * (d - k), m)
+    if (numesign + k) % 2 == 1:
+        r = m - r
+    return ((r << 192) // m) & M192
+
+
+def py_tail_term(d: int, i: int, numesign: int, den0: int, den1: int) -> int:
+    k = d + i
+    m = den0 * k + den1
+    q = (1 << (192 - 10 * i)) // m
+    if (numesign + k) % 2 == 1:
+        q = M192 - q  # floor((1 - x) 2^192) for the non-integer x 2^192
+    return q
+
+
+def py_range_sum(d: int, k0: int, k1: int, j: int) -> int:
+    s = 0
+    for k in range(k0, k1):
+        s += py_term(d, k, NUMESIGN[j], DEN0[j], DEN1[j])
+    return s & M192
+
+
+def py_tail(d: int, j: int) -> int:
+    return sum(py_tail_term(d, i, NUMESIGN[j], DEN0[j], DEN1[j]) for i in range(20)) & M192
+
+
+def py_batch_calc(d: int, k_start: int, k_end: int, j: int) -> int:
+    hi = min(k_end, d)
+    offset = k_start
+    s = 0
+    if k_start < hi:
+        s = py_range_sum(d, k_start, hi, j)
+        offset = hi
+    if offset == d:
+        s += py_tail(d, j)
+    return s & M192
+
+
+def py_normal(n: int) -> int:
+    _, d = py_normalize(n)
+    ans = 0
+    for j in range(7):
+        ans += (py_batch_calc(d, 0, d, j) << SHIFT[j])
+    return ans & M192
+
+
+def bound_ulps(d: int) -> int:
+    """Truncation loss of the exact-integer semantics: each term's floor loses < 1 unit, scaled
+    by c_j in the combine; the tail beyond 20 terms is far below one unit."""
+    return sum(NUMER) * (d + 20) + 2
+
+
+def pi_series_check(terms: int = 12) -> Fraction:
+    """Bellard's series with the formula table, as an exact rational (for pinning the table)."""
+    total = Fraction(0)
+    for n in range(terms):
+        s = Fraction(0)
+        for j in range(7):
+            sign = -1 if NUMESIGN[j] else 1
+            s += sign * Fraction(NUMER[j], DEN0[j] * n + DEN1[j])
+        total += Fraction((-1) ** n, 1024 ** n) * s
+    return total / 64

@@ -1,0 +1,234 @@
+"""GPU parity tests: the HIP kernels of libclbellard.so (through the C ABI) against the CPU
+oracle, the golden pi digits, and the size-independent properties of the exact-integer sums
+(partition invariance, batched == single, plain == split)."""
+import os
+import random
+import sys
+
+import pytest
+
+import clbellard_amd as clb
+from clb_testlib import (DEN0, DEN1, M192, NUMESIGN, SHIFT, bound_ulps, i2w, py_normalize, w2i)
+
+pytestmark = pytest.mark.gpu
+R = random.Random(424242)
+
+
+def note(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = clb.Context(0, clb.PREC_U192)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def ctx_plain():
+    c = clb.Context(0, clb.PREC_U192, kernel=clb.KERNEL_PLAIN)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def ctx_split():
+    c = clb.Context(0, clb.PREC_U192, kernel=clb.KERNEL_SPLIT)
+    yield c
+    c.close()
+
+
+SMALL = [5, 10, 15, 20, 25, 30, 50, 100, 260, 500, 1000, 1005, 4100, 9995, 10000, 65540, 100000,
+         123455, 250000]
+
+
+@pytest.mark.parametrize("n", SMALL)
+def test_normal_bit_exact_vs_oracle(ctx_plain, ctx_split, oracle, pi_fixtures, n):
+    note(f"normal N={n}")
+    want = oracle.normal(n)
+    for c in (ctx_split, ctx_plain):
+        frac, hexs, nn = c.normal(n)
+        assert nn == n
+        assert w2i(frac) == want, (n, c)
+        assert hexs == f"{want:048x}"
+    _, d = py_normalize(n)
+    ref = int(pi_fixtures["digits"][str(n)], 16)
+    assert (ref - want) % (1 << 192) <= bound_ulps(d)
+
+
+@pytest.mark.parametrize("n", [500000, 777775, 1000000])
+def test_normal_vs_pi_fixture(ctx_plain, ctx_split, pi_fixtures, n):
+    note(f"normal N={n}")
+    a = w2i(ctx_split.normal(n)[0])
+    b = w2i(ctx_plain.normal(n)[0])
+    assert a == b
+    _, d = py_normalize(n)
+    ref = int(pi_fixtures["digits"][str(n)], 16)
+    assert (ref - a) % (1 << 192) <= bound_ulps(d)
+
+
+@pytest.mark.parametrize("n", [10**7, 10**8, 10**9, 10**10, 10**11])
+def test_published_digits(ctx, ctx_plain, pi_fixtures, n):
+    note(f"published N={n}")
+    frac, hexs, _ = ctx.normal(n)
+    assert hexs[:14] == pi_fixtures["published"][str(n)]
+    if n <= 10**9:
+        assert ctx_plain.normal(n)[0] == frac
+
+
+def _ranges():
+    cases = []
+    # (d, k0, k1, j) across every kernel variant: m < 2^43 (W3), < 2^47 (W2), < 2^49 (W1),
+    # < 2^55 (plain, lazy Montgomery), < 2^63 (plain, reduced residues)
+    for d in (1, 3, 7, 999, 1000001, 399999999999, 2 * 10**12 + 1, 3 * 10**12 + 1,
+              5 * 10**13 + 1, 10**15 + 1, 3 * 10**17 + 1):
+        for _ in range(3):
+            j = R.randrange(7)
+            k0 = R.randrange(0, d)
+            k1 = min(d, k0 + R.randrange(1, 3000))
+            cases.append((d, k0, k1, j))
+        cases.append((d, max(0, d - 2500), d, R.randrange(7)))       # reaches d: tail included
+        cases.append((d, d, d + 10, R.randrange(7)))                # tail only
+        cases.append((d, max(0, d - 100), d + 10**6, R.randrange(7)))  # k_end past d
+    cases.append((1001, 1002, 2000, 0))  # beyond d: nothing
+    cases.append((1001, 5, 5, 3))        # empty
+    return cases
+
+
+@pytest.mark.parametrize("kernel", [clb.KERNEL_AUTO, clb.KERNEL_PLAIN])
+def test_batch_calc_vs_oracle(oracle, kernel):
+    with clb.Context(0, clb.PREC_U192, kernel=kernel) as c:
+        for d, k0, k1, j in _ranges():
+            note(f"batch_calc kernel={kernel} d={d} k=[{k0},{k1}) j={j}")
+            c.set0mem()
+            c.batch_calc(d, k0, k1, j)
+            assert w2i(c.read_partial()) == oracle.batch_calc(d, k0, k1, j), (d, k0, k1, j)
+
+
+def test_split_variants_vs_oracle(ctx_split, oracle):
+    # force each fp64 window width at the top of its modulus range
+    for d, span in ((4 * 10**11 + 1, 4000), (2 * 10**12 + 1, 4000), (3 * 10**13 + 1, 4000)):
+        for j in range(7):
+            k0 = R.randrange(0, d - span)
+            note(f"split d={d} k0={k0} j={j}")
+            ctx_split.set0mem()
+            ctx_split.batch_calc(d, k0, k0 + span, j)
+            assert w2i(ctx_split.read_partial()) == oracle.batch_calc(d, k0, k0 + span, j)
+
+
+def test_split_rejects_large_moduli(ctx_split):
+    with pytest.raises(clb.ClbError) as e:
+        ctx_split.batch_calc(10**15, 0, 10, 0)
+    assert e.value.code == -3
+
+
+def test_accumulates_across_calls(ctx, oracle):
+    d, j = 10**9 + 1, 4
+    ctx.set0mem()
+    total = 0
+    for k0 in (0, 12345, 9999999):
+        ctx.batch_calc(d, k0, k0 + 5000, j)
+        total += oracle.batch_calc(d, k0, k0 + 5000, j)
+    assert w2i(ctx.read_partial()) == total & M192
+
+
+def test_kernel_operators_generic_formula(ctx, oracle):
+    for _ in range(6):
+        den0 = 2 * R.randrange(1, 20)
+        den1 = 2 * R.randrange(0, 20) + 1
+        numesign = R.randrange(2)
+        d = R.choice([10**6 + 1, 10**11 + 1])
+        k0 = R.randrange(0, d - 600)
+        k1 = k0 + R.randrange(1, 600)
+        ctx.set0mem()
+        ctx.sglobal(k0, d, k1, numesign, den0, den1)
+        want = sum(oracle.term192(d, k, numesign, den0, den1) for k in range(k0, k1)) & M192
+        assert w2i(ctx.read_partial()) == want
+        ctx.set0mem()
+        ctx.sglobal_after(d, numesign, den0, den1)
+        assert w2i(ctx.read_partial()) == oracle.tail192(d, numesign, den0, den1)
+    with pytest.raises(clb.ClbError):
+        ctx.sglobal(0, 100, 101, 0, 4, 1)  # k_max beyond d
+    with pytest.raises(clb.ClbError):
+        ctx.sglobal(0, 100, 10, 0, 3, 1)   # even moduli
+
+
+def test_allloop_through_operators_equals_normal(ctx, oracle):
+    n = 123455
+    _, d = py_normalize(n)
+    ctx.clear_ans()
+    for j in range(7):
+        ctx.set0mem()
+        ctx.batch_calc(d, 0, d, j)
+        ctx.add_ans(j)
+        part = w2i(ctx.read_partial())
+        assert part == (oracle.batch_calc(d, 0, d, j) << SHIFT[j]) & M192
+    assert w2i(ctx.read_ans()) == oracle.normal(n)
+    assert w2i(ctx.normal(n)[0]) == oracle.normal(n)
+
+
+def test_batch_positions_equal_single_runs(ctx):
+    rng = random.Random(0)
+    pos = [5 * round(10 ** (3 + 4 * rng.random()) / 5) for _ in range(48)] + [5, 10, 1000000]
+    note(f"batch of {len(pos)} positions")
+    batched = ctx.batch_positions(pos)
+    for p, b in zip(pos, batched):
+        assert ctx.normal(p)[0] == b, p
+
+
+def test_run_jobs_tiling_sums_to_normal(ctx):
+    n = 10**7
+    _, d = py_normalize(n)
+    cuts = sorted(R.sample(range(1, d), 9))
+    bounds = [0] + cuts + [d]
+    jobs = [(d, a, b, j) for j in range(7) for a, b in zip(bounds, bounds[1:])]
+    parts = ctx.run_jobs(jobs)
+    total = sum(w2i(p) for p in parts) & M192
+    assert total == w2i(ctx.normal(n)[0])
+    # one job per formula covering everything is the same as the jobs sum per formula
+    whole = ctx.run_jobs([(d, 0, d + 5, j) for j in range(7)])
+    assert sum(w2i(p) for p in whole) & M192 == total
+
+
+def test_launch_shape_does_not_change_bits(oracle):
+    n = 65540
+    want = oracle.normal(n)
+    with clb.Context(0) as c:
+        for bpc in (1, 2, 5, 8, 16):
+            c.set_launch(bpc)
+            assert w2i(c.normal(n)[0]) == want, bpc
+
+
+def _worklist(path, digits, precision, jobs):
+    with open(path, "w", newline="") as f:
+        f.write(f"digits={digits}\r\nprecision={precision} (0=106bit , 1=192bit)\r\n")
+        f.write("\r\n".join(jobs) + "\r\n")
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_worklist_end_to_end(tmp_path, precision):
+    n = 1000000
+    _, d = py_normalize(n)
+    batch = 100000
+    nb = d // batch + 1
+    jobs = [f"{j}_{b}" for j in range(7) for b in range(nb)] + ["3_99"]  # 3_99 starts past d
+    wl = os.path.join(str(tmp_path), "worklist.txt")
+    _worklist(wl, n, precision, jobs)
+    work = os.path.join(str(tmp_path), "work")
+    with clb.Context(0, precision) as c:
+        ran = c.run_worklist(wl, work, batch_size=batch, device_label="0\ttest-device")
+        assert ran == 7 * nb
+        assert c.run_worklist(wl, work, batch_size=batch) == 0  # resume: all files exist
+        os.remove(os.path.join(work, "2", "2_1.txt"))
+        assert c.run_worklist(wl, work, batch_size=batch) == 1
+        frac, hexs, _ = c.normal(n)
+    body = open(os.path.join(work, "0", "0_0.txt"), "rb").read().decode()
+    line0, line1, line2 = body.split("\r\n")
+    assert len(line0) == 48 and line1 == "0\ttest-device" and line2.endswith("ms")
+    if precision == 1:
+        total, counts = clb.work_sum(work)
+        assert counts == [nb] * 7
+        assert total == hexs
+    else:
+        assert line0.endswith("0" * 20)
