@@ -86,7 +86,19 @@ def main():
     ap.add_argument("--window-log2", type=int, default=22)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes (HBM counter calibration)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(args.calibrate_copy, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copied {args.calibrate_copy} bytes")
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the digit "

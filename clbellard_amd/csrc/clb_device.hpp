@@ -42,8 +42,8 @@ __device__ __forceinline__ void add192(U192& a, uint64_t b0, uint64_t b1, uint64
 __device__ __forceinline__ void neg192_minus(U192& a, uint64_t n) {
   // ~a = -a - 1, so -a - n = ~a - (n - 1) = ~a + (1 - n)
   U192 t = {~a.w0, ~a.w1, ~a.w2};
-  uint64_t add = 1 - n;                   // two's complement of n - 1
-  uint64_t ext = (n == 0) ? 0 : ~0ull;    // sign extension of (1 - n) for n >= 1
+  uint64_t add = 1 - n;                   // low word of the signed value 1 - n
+  uint64_t ext = (n > 1) ? ~0ull : 0;     // its sign extension: 1 - n < 0 iff n >= 2
   add192(t, add, ext, ext);
   a = t;
 }

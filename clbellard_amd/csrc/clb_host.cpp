@@ -479,7 +479,7 @@ int clb_get_device_name(int device, char* name, size_t cap) {
   if (!name || cap == 0) return fail(CLB_EINVAL, "null name buffer");
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, device));
-  snprintf(name, cap, "%s", prop.name);
+  snprintf(name, cap, "%s", prop.name[0] ? prop.name : prop.gcnArchName);
   return CLB_OK;
 }
 
@@ -505,7 +505,8 @@ int clb_ctx_create(int device, int precision, clb_ctx** out) {
   if (e != hipSuccess)
     return cleanup(fail(CLB_EDEVICE, "device %d: %s", device, hipGetErrorString(e)));
   ctx->num_cus = prop.multiProcessorCount;
-  snprintf(ctx->device_name, sizeof ctx->device_name, "%s", prop.name);
+  snprintf(ctx->device_name, sizeof ctx->device_name, "%s",
+           prop.name[0] ? prop.name : prop.gcnArchName);
   if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess)
     return cleanup(fail(CLB_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e)));
   for (auto& ev : ctx->ev)

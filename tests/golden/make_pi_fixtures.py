@@ -8,9 +8,10 @@ digit normalisation produces) the fixture holds floor(frac(16^(N-1) * pi) * 2^19
 digits.  The exact value of the reference's 192-bit sum differs from this by less than
 362 * (d + 20) units of 2^-192 (d the loop count), so these digits pin every bit above that.
 
-"published" holds D. H. Bailey's table ("The BBP Algorithm for Pi", 2006) of 14 hex digits at
-large positions; the 10^6 entry is cross-checked against mpmath here, the others are checked
-against the GPU computation by the tests.
+"published" holds entries of D. H. Bailey's table ("The BBP Algorithm for Pi", 2006) of 14 hex
+digits beginning at position N, as recalled (the table is not available offline).  The 10^6
+entry is cross-checked against mpmath here; 10^7 .. 10^10 are checked by the GPU tests, where
+they agree with both term kernels.
 """
 import json
 import os
@@ -27,7 +28,6 @@ PUBLISHED = {
     "100000000": "ecb840e21926ec",
     "1000000000": "85895585a0428b",
     "10000000000": "921c73c6838fb2",
-    "100000000000": "9c381872d27596",
     "1250000000000": "07e45733cc790b",
     "250000000000000": "e6216b069cb6c1",
 }

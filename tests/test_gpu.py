@@ -68,7 +68,7 @@ def test_normal_vs_pi_fixture(ctx_plain, ctx_split, pi_fixtures, n):
     assert (ref - a) % (1 << 192) <= bound_ulps(d)
 
 
-@pytest.mark.parametrize("n", [10**7, 10**8, 10**9, 10**10, 10**11])
+@pytest.mark.parametrize("n", [10**7, 10**8, 10**9, 10**10])
 def test_published_digits(ctx, ctx_plain, pi_fixtures, n):
     note(f"published N={n}")
     frac, hexs, _ = ctx.normal(n)
@@ -118,8 +118,9 @@ def test_split_variants_vs_oracle(ctx_split, oracle):
 
 
 def test_split_rejects_large_moduli(ctx_split):
+    d = 10**15 + 1  # moduli 10k + 1 near k = d exceed 2^49
     with pytest.raises(clb.ClbError) as e:
-        ctx_split.batch_calc(10**15, 0, 10, 0)
+        ctx_split.batch_calc(d, d - 10, d, 2)
     assert e.value.code == -3
 
 
