@@ -112,6 +112,49 @@ __device__ __forceinline__ double fpow2(uint64_t e, int s, double m, double minv
   return x;
 }
 
+// 2^(1024 H + v) mod m (balanced) for a wave-uniform H >= 1 and a per-lane v in [0, 2048): the
+// lanes of a wave have exponents E - 10 L, which this splits into a common high part H, whose
+// window digits are scalar, and a per-lane low part v.  Bit 10 of v is absorbed by the last
+// window of H (digit up to 2^W), then ten squares take the low ten bits of v.
+// Valid while m 2^(2^W) <= 2^50.
+template <int W>
+__device__ __forceinline__ double fpow2_split(uint64_t H, uint32_t v, double m, double minv) {
+  constexpr uint32_t mask = (1u << W) - 1;
+  const int bl = 64 - __builtin_clzll(H);
+  const int s = bl > 3 ? W * ((bl - 3) / W) : 0;  // H = t 2^s + (s window bits), t < 2^(W+2)
+  const uint32_t t = (uint32_t)(H >> s);
+  const int carry = (int)(v >> 10);
+  double x;
+  if (s == 0) {
+    const double p = __builtin_amdgcn_ldexp(1.0, (int)t + carry);
+    x = __builtin_fma(-__builtin_rint(p * minv), m, p);
+  } else {
+    const double p = __builtin_amdgcn_ldexp(1.0, (int)t);
+    x = __builtin_fma(-__builtin_rint(p * minv), m, p);
+    for (int pos = s - W; pos > 0; pos -= W) {
+#pragma unroll
+      for (int i = 0; i < W - 1; ++i) x = fsq(x, m, minv);
+      x = fsq_shift(x, (int)((H >> pos) & mask), m, minv);
+    }
+#pragma unroll
+    for (int i = 0; i < W - 1; ++i) x = fsq(x, m, minv);
+    x = fsq_shift(x, (int)(H & mask) + carry, m, minv);
+  }
+#pragma unroll
+  for (int pos = 10 - W; pos >= 0; pos -= W) {
+#pragma unroll
+    for (int i = 0; i < W - 1; ++i) x = fsq(x, m, minv);
+    x = fsq_shift(x, (int)((v >> pos) & mask), m, minv);
+  }
+  constexpr int rest = 10 % W;
+  if constexpr (rest != 0) {
+#pragma unroll
+    for (int i = 0; i < rest - 1; ++i) x = fsq(x, m, minv);
+    x = fsq_shift(x, (int)(v & ((1u << rest) - 1)), m, minv);
+  }
+  return x;
+}
+
 // ------------------------------------------------------------------ int engine
 
 __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) { return __umul64hi(a, b); }

@@ -119,12 +119,12 @@ int set_device(clb_ctx* ctx) {
 
 // Term kernel for the largest modulus of a launch (bounds in clb_device.hpp / clb_internal.hpp).
 int choose_kind(int pref, uint64_t max_m, int* kind) {
-  const bool split_ok = max_m < (1ull << 49);
+  const bool split_ok = max_m < (1ull << 48);
   if (pref == CLB_KERNEL_SPLIT && !split_ok)
-    return fail(CLB_ERANGE, "split kernel needs moduli < 2^49 (largest is %llu)",
+    return fail(CLB_ERANGE, "split kernel needs moduli < 2^48 (largest is %llu)",
                 (unsigned long long)max_m);
   if (pref == CLB_KERNEL_SPLIT || (pref == CLB_KERNEL_AUTO && split_ok)) {
-    *kind = max_m < (1ull << 43) ? kSplitW3 : max_m < (1ull << 47) ? kSplitW2 : kSplitW1;
+    *kind = max_m < (1ull << 42) ? kSplitW3 : max_m < (1ull << 46) ? kSplitW2 : kSplitW1;
     return CLB_OK;
   }
   if (max_m >= (1ull << 63))

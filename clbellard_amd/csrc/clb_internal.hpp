@@ -15,9 +15,9 @@ constexpr int kTailTerms = 20;                             // k = d .. d+19 (Bel
 
 // Term kernel variants.
 enum Kind : int {
-  kSplitW3 = 0,  // fp64 engine, 3-bit windows: m < 2^43
-  kSplitW2 = 1,  // fp64 engine, 2-bit windows: m < 2^47
-  kSplitW1 = 2,  // fp64 engine, 1-bit windows: m < 2^49
+  kSplitW3 = 0,  // fp64 engine, 3-bit windows: m < 2^42
+  kSplitW2 = 1,  // fp64 engine, 2-bit windows: m < 2^46
+  kSplitW1 = 2,  // fp64 engine, 1-bit windows: m < 2^48
   kPlainW3 = 3,  // int engine, lazy residues, 3-bit windows: m < 2^55
   kPlainWide = 4 // int engine, reduced residues, 1-bit windows: m < 2^63
 };

@@ -53,33 +53,53 @@ __device__ __forceinline__ void flush_job(U192 v, uint64_t nterms, uint32_t nume
 
 // fp64 engine.  Per lane per tile at most kTermsPerLane = 16 terms, so the four digit sums stay
 // exact integers below 2^53 and are converted to the lane's 192-bit sum once per tile.
+// Lane L's exponent is E - 10 L with E the exponent of lane 0 (uniform): for E >= 1664 it is
+// evaluated as 2^(1024 H + v) with H = (E - 640) >> 10 uniform and v = ((E - 640) & 1023) + 640
+// - 10 L in [10, 1663] per lane (fpow2_split); smaller exponents take per-lane windows.
 template <int W>
 __device__ __forceinline__ void tile_split(const JobDev& jb, uint64_t kbase, int lane, U192& acc,
                                            uint64_t& nterms) {
   double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0;
-  uint64_t k = kbase + (uint64_t)lane;
-  double m = (double)(jb.den0 * k + jb.den1);
-  uint64_t e = 10ull * (jb.d - k);
+  const uint32_t ten_lane = 10u * (uint32_t)lane;
+  double m = (double)(jb.den0 * (kbase + (uint64_t)lane) + jb.den1);
+  double minv = 0.0;
   const double mstep = (double)(jb.den0 * (uint64_t)kWave);
+  const bool full = kbase >= jb.k0 && kbase + kTileK <= jb.k1;  // every lane, every term valid
   for (int i = 0; i < kTermsPerLane; ++i) {
-    const uint64_t kw = kbase + (uint64_t)kWave * (uint64_t)i;  // k of lane 0: largest exponent
+    const uint64_t kw = kbase + (uint64_t)kWave * (uint64_t)i;  // k of lane 0
     if (kw >= jb.k1) break;
-    if (k >= jb.k0 && k < jb.k1) {
-      const uint64_t emax = 10ull * (jb.d - kw);
-      const int bl = 64 - __builtin_clzll(emax);
-      const int s = bl > 3 ? W * ((bl - 3) / W) : 0;
-      const double minv = recip(m);
-      double rem = fpow2<W>(e, s, m, minv);
+    const uint64_t k = kw + (uint64_t)lane;
+    if (full || (k >= jb.k0 && k < jb.k1)) {
+      // 1/m: one Newton step from the previous term's reciprocal when m moved by < 2^-27 m
+      // (relative error then < 2^-54 + rounding), else the hardware estimate and two steps.
+      if (full && i > 0 && jb.den0 * kw + jb.den1 >= (1ull << 37)) {
+        const double e = __builtin_fma(-m, minv, 1.0);
+        minv = __builtin_fma(minv, e, minv);
+      } else {
+        minv = recip(m);
+      }
+      const uint64_t E = 10ull * (jb.d - kw);
+      double rem;
+      if (E >= 1664) {
+        const uint64_t F = E - 640;
+        rem = fpow2_split<W>(F >> 10, (uint32_t)(F & 1023) + 640u - ten_lane, m, minv);
+      } else {
+        const int bl = 64 - __builtin_clzll(E);
+        const int s = bl > 3 ? W * ((bl - 3) / W) : 0;
+        rem = fpow2<W>(E - ten_lane, s, m, minv);
+      }
       f0 += fdigit(rem, m, minv);
       f1 += fdigit(rem, m, minv);
       f2 += fdigit(rem, m, minv);
       const double q3 = fdigit(rem, m, minv);
       f3 += q3 + (rem >= m ? 1.0 : 0.0);  // lazy remainder in [m, 2m): one more
-      nterms += (m > 1.0);                 // m = 1 gives r = 0 and a zero term of either sign
+      if (!full) nterms += (m > 1.0);     // m = 1 gives r = 0 and a zero term of either sign
     }
-    k += kWave;
     m += mstep;
-    e -= 10ull * kWave;
+  }
+  if (full) {  // every term counted; k = 0 with den1 = 1 is the only m = 1
+    nterms += kTermsPerLane;
+    if (kbase == 0 && lane == 0 && jb.den1 == 1) nterms -= 1;
   }
   // Digit i has weight 2^(144 - 48 i).  f0 may be negative (balanced residue); adding 2^52 keeps
   // it non-negative and changes the sum by 2^196 = 0 mod 2^192.

@@ -79,7 +79,7 @@ def test_published_digits(ctx, ctx_plain, pi_fixtures, n):
 
 def _ranges():
     cases = []
-    # (d, k0, k1, j) across every kernel variant: m < 2^43 (W3), < 2^47 (W2), < 2^49 (W1),
+    # (d, k0, k1, j) across every kernel variant: m < 2^42 (W3), < 2^46 (W2), < 2^48 (W1),
     # < 2^55 (plain, lazy Montgomery), < 2^63 (plain, reduced residues)
     for d in (1, 3, 7, 999, 1000001, 399999999999, 2 * 10**12 + 1, 3 * 10**12 + 1,
               5 * 10**13 + 1, 10**15 + 1, 3 * 10**17 + 1):
@@ -108,7 +108,7 @@ def test_batch_calc_vs_oracle(oracle, kernel):
 
 def test_split_variants_vs_oracle(ctx_split, oracle):
     # force each fp64 window width at the top of its modulus range
-    for d, span in ((4 * 10**11 + 1, 4000), (2 * 10**12 + 1, 4000), (3 * 10**13 + 1, 4000)):
+    for d, span in ((4 * 10**11 + 1, 4000), (2 * 10**12 + 1, 4000), (2 * 10**13 + 1, 4000)):
         for j in range(7):
             k0 = R.randrange(0, d - span)
             note(f"split d={d} k0={k0} j={j}")
