@@ -76,6 +76,84 @@ def load_traffic():
         return None
 
 
+def bench_batched(clb, args):
+    """Many digit positions (log-uniform in [10^6, 10^11], SplitMix64 seed 0, multiples of 5) in
+    one persistent launch; a sample is re-run one position at a time and compared bit for bit."""
+    rng = splitmix64(0)
+    pos = []
+    for _ in range(args.positions):
+        u = (next(rng) >> 11) * 2.0**-53
+        pos.append(5 * round(10 ** (6 + 5 * u) / 5))
+    terms = sum(clb.normalize(p)[1] for p in pos)
+    ctx = clb.Context(0, clb.PREC_U192)
+    log(f"batched: {len(pos)} positions, {terms:.4g} terms")
+    t0 = time.perf_counter()
+    fracs = ctx.batch_positions(pos)
+    dt = time.perf_counter() - t0
+    head_ms = ctx.last_timing()[0]
+    log(f"batched launch: {dt:.2f}s")
+    pick = sorted(range(len(pos)), key=lambda i: pos[i])
+    sample = [pick[int(i * (len(pick) - 1) / max(1, args.check - 1))] for i in range(args.check)]
+    for i in sample:
+        if ctx.normal(pos[i])[0] != fracs[i]:
+            raise SystemExit(f"position {pos[i]}: batched result differs from the single run")
+    log(f"{len(sample)} positions re-run alone: bit-identical")
+    print(json.dumps({
+        "metric": "Bellard terms/sec, batched digit positions", "value": terms / dt,
+        "unit": "terms/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": 1e3 * dt,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(pos)} positions log-uniform in [1e6, 1e11], seed 0, one "
+                               "persistent launch", "positions": len(pos), "terms": terms,
+                   "head_kernel_ms": head_ms, "checked_single": len(sample)}}), flush=True)
+    ctx.close()
+    return 0
+
+
+def bench_worklist(clb, args):
+    """64 uneven term-range jobs (lengths U[1e8, 1e11], SplitMix64 seed 1, rescaled to tile
+    [0, d) at N = 2.5e12) x 7 formulas in one launch, written as work files, summed like
+    workmodeSum0_6, and compared with the single normal-mode run."""
+    import tempfile
+    n_pos = args.position if args.position != POSITION else 2500000000000
+    n_pos, d, _ = clb.normalize(n_pos)
+    rng = splitmix64(1)
+    lens = [10**8 + next(rng) % (10**11 - 10**8 + 1) for _ in range(64)]
+    total, acc, cuts = sum(lens), 0, [0]
+    for ln in lens:
+        acc += ln
+        cuts.append((d * acc + total // 2) // total)
+    jobs = [(d, cuts[i], cuts[i + 1] if i < 63 else d + 1, j) for j in range(7) for i in range(64)]
+    ctx = clb.Context(0, clb.PREC_U192)
+    log(f"worklist: N={n_pos} d={d}, {len(jobs)} jobs")
+    t0 = time.perf_counter()
+    parts = ctx.run_jobs(jobs)
+    dt = time.perf_counter() - t0
+    head_ms = ctx.last_timing()[0]
+    log(f"jobs launch: {dt:.2f}s")
+    with tempfile.TemporaryDirectory() as work:
+        for idx, (jd, k0, k1, j) in enumerate(jobs):
+            clb.write_work_file(work, j, idx % 64, parts[idx], clb.PREC_U192, "0\tMI355X",
+                                int(1e3 * dt))
+        summed, counts = clb.work_sum(work)
+    t1 = time.perf_counter()
+    single, single_hex, _ = ctx.normal(n_pos)
+    dt_single = time.perf_counter() - t1
+    if summed != single_hex or counts != [64] * 7:
+        raise SystemExit(f"work-file sum {summed} != single run {single_hex}")
+    log(f"summed work files == single run ({dt_single:.2f}s): {summed}")
+    print(json.dumps({
+        "metric": "Bellard terms/sec, uneven worklist jobs", "value": d / dt, "unit": "terms/s",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": 1e3 * dt, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"64 uneven jobs x 7 formulas tiling [0, d) at N={n_pos}, one "
+                               "persistent launch, work files summed", "position": n_pos, "d": d,
+                   "head_kernel_ms": head_ms, "single_run_s": dt_single, "digits": summed}}),
+          flush=True)
+    ctx.close()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +164,12 @@ def main():
     ap.add_argument("--window-log2", type=int, default=22)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["normal", "batched", "worklist"], default="normal",
+                    help="normal: the metric (N = 10^12); batched: many positions in one launch; "
+                         "worklist: uneven term-range jobs written as work files and summed")
+    ap.add_argument("--positions", type=int, default=4096, help="batched: number of positions")
+    ap.add_argument("--check", type=int, default=8,
+                    help="batched: positions re-run alone and compared bit for bit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -106,6 +190,11 @@ def main():
         return 2
 
     import clbellard_amd as clb
+
+    if args.workload == "batched":
+        return bench_batched(clb, args)
+    if args.workload == "worklist":
+        return bench_worklist(clb, args)
 
     n_pos, d, _ = clb.normalize(args.position)
     ctx = clb.Context(0, clb.PREC_U192, kernel=clb.KERNEL_SPLIT)

@@ -25,7 +25,7 @@ from typing import Iterable, List, Optional, Sequence, Tuple
 
 __all__ = [
     "ClbError", "lib", "LIB_PATH", "normalize", "formula", "hex48", "hex_dd", "u192_to_dd",
-    "work_sum", "device_count", "device_name", "Context", "u192_to_int", "int_to_u192",
+    "work_sum", "parse_worklist", "write_work_file", "device_count", "device_name", "Context", "u192_to_int", "int_to_u192",
     "PREC_DD106", "PREC_U192", "PREC_U192_BETA", "KERNEL_AUTO", "KERNEL_PLAIN", "KERNEL_SPLIT",
     "NUMER", "NUMESIGN", "DEN0", "DEN1",
 ]
@@ -94,6 +94,11 @@ _SIGNATURES = {
                           ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "clb_work_sum": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)],
                      ctypes.c_int),
+    "clb_write_work_file": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int64, _P64, ctypes.c_int,
+                             ctypes.c_char_p, ctypes.c_int64], ctypes.c_int),
+    "clb_parse_worklist": ([ctypes.c_char_p, _P64, ctypes.POINTER(ctypes.c_int),
+                            ctypes.POINTER(ctypes.c_int64), ctypes.c_size_t,
+                            ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "clb_last_timing": ([_CTX, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
                         ctypes.c_int),
 }
@@ -112,7 +117,19 @@ def _load() -> ctypes.CDLL:
     return handle
 
 
-lib = _load()
+class _Lib:
+    """libclbellard.so, loaded on first use so that the package (and its build module) import
+    before the library exists; any call without the built library raises ImportError."""
+
+    _handle = None
+
+    def __getattr__(self, name):
+        if _Lib._handle is None:
+            _Lib._handle = _load()
+        return getattr(_Lib._handle, name)
+
+
+lib = _Lib()
 
 
 def _check(rc: int) -> None:
@@ -176,6 +193,24 @@ def work_sum(work_dir: str) -> Tuple[str, List[int]]:
     counts = (ctypes.c_int * 7)()
     _check(lib.clb_work_sum(os.fsencode(work_dir), buf, counts))
     return buf.value.decode(), list(counts)
+
+
+def write_work_file(work_dir: str, j: int, b: int, w: Sequence[int], precision: int = PREC_U192,
+                    device_label: str = "", ms: int = 0) -> None:
+    """fsave: <work_dir>/<j>/<j>_<b>.txt with the formatted value, a device line and the time."""
+    _check(lib.clb_write_work_file(os.fsencode(work_dir), j, b, _arr3(w), precision,
+                                   device_label.encode(), ms))
+
+
+def parse_worklist(path: str) -> Tuple[int, int, List[Tuple[int, int]]]:
+    """Worklist -> (digits, precision (-1 if invalid), [(formula, batch), ...])."""
+    digits, prec, n = _U64(), ctypes.c_int(), ctypes.c_size_t()
+    _check(lib.clb_parse_worklist(os.fsencode(path), ctypes.byref(digits), ctypes.byref(prec),
+                                  None, 0, ctypes.byref(n)))
+    arr = (ctypes.c_int64 * (2 * max(1, n.value)))()
+    _check(lib.clb_parse_worklist(os.fsencode(path), ctypes.byref(digits), ctypes.byref(prec),
+                                  arr, n.value, ctypes.byref(n)))
+    return digits.value, prec.value, [(arr[2 * i], arr[2 * i + 1]) for i in range(n.value)]
 
 
 def device_count() -> int:

@@ -129,7 +129,7 @@ __device__ __forceinline__ double fpow2_split(uint64_t H, uint32_t v, double m, 
     const double p = __builtin_amdgcn_ldexp(1.0, (int)t + carry);
     x = __builtin_fma(-__builtin_rint(p * minv), m, p);
   } else {
-    const double p = __builtin_amdgcn_ldexp(1.0, (int)t);
+    const double p = __builtin_bit_cast(double, (uint64_t)(1023 + t) << 52);  // 2^t, t uniform
     x = __builtin_fma(-__builtin_rint(p * minv), m, p);
     for (int pos = s - W; pos > 0; pos -= W) {
 #pragma unroll

@@ -453,6 +453,33 @@ int make_dir(const std::string& p) {
   return fail(CLB_EIO, "cannot create %s: %s", p.c_str(), strerror(errno));
 }
 
+void format_result(int precision, const uint64_t w[3], char hex[49]) {
+  if (precision == CLB_PREC_DD106) {
+    double dd[2];
+    u192_to_dd(w, dd);
+    hex_dd(dd, hex);
+  } else {
+    hex48(w, hex);
+  }
+}
+
+// fsave, Bellard_192.hsp:514-533: hex line, device line, "<ms>ms"; HSP's "\n" is CR LF.
+int write_work_file(const std::string& work_dir, int j, int64_t b, const uint64_t w[3],
+                    int precision, const std::string& label, long long ms) {
+  if (int rc = make_dir(work_dir)) return rc;
+  const std::string dir = work_dir + "/" + std::to_string(j);
+  if (int rc = make_dir(dir)) return rc;
+  const std::string path = dir + "/" + std::to_string(j) + "_" + std::to_string(b) + ".txt";
+  char hex[49];
+  format_result(precision, w, hex);
+  const std::string body = std::string(hex) + "\r\n" + label + "\r\n" + std::to_string(ms) + "ms";
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) return fail(CLB_EIO, "cannot write %s: %s", path.c_str(), strerror(errno));
+  const bool ok = fwrite(body.data(), 1, body.size(), f) == body.size();
+  if (fclose(f) != 0 || !ok) return fail(CLB_EIO, "write failed: %s", path.c_str());
+  return CLB_OK;
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -661,15 +688,6 @@ static int read3(clb_ctx* ctx, const uint64_t* dev, uint64_t w[3]) {
 int clb_read_partial(clb_ctx* ctx, uint64_t w[3]) { return read3(ctx, ctx ? ctx->partial : nullptr, w); }
 int clb_read_ans(clb_ctx* ctx, uint64_t w[3]) { return read3(ctx, ctx ? ctx->ans : nullptr, w); }
 
-static void format_result(int precision, const uint64_t w[3], char hex[49]) {
-  if (precision == CLB_PREC_DD106) {
-    double dd[2];
-    u192_to_dd(w, dd);
-    hex_dd(dd, hex);
-  } else {
-    hex48(w, hex);
-  }
-}
 
 int clb_normal(clb_ctx* ctx, uint64_t n, uint64_t frac[3], char hex[49], uint64_t* n_out) {
   if (int rc = check_ctx(ctx)) return rc;
@@ -763,16 +781,18 @@ int clb_last_timing(clb_ctx* ctx, double* head_kernel_ms, double* total_ms) {
   return CLB_OK;
 }
 
-int clb_run_worklist(clb_ctx* ctx, const char* worklist_path, const char* work_dir,
-                     uint64_t batch_size, const char* device_label, int* jobs_run) {
-  if (int rc = check_ctx(ctx)) return rc;
-  if (!worklist_path || !work_dir) return fail(CLB_EINVAL, "null path");
-  if (jobs_run) *jobs_run = 0;
+// *chktxtstr (Bellard_192.hsp:176-207), *workStart (:221-227), *workStart2 (:229-247).
+struct Worklist {
+  int64_t digits = 0;
+  int precision = -1;
+  std::vector<std::pair<int, int64_t>> jobs;
+};
+
+static int parse_worklist(const char* path, Worklist* wl) {
   std::string text;
-  if (int rc = read_text(worklist_path, &text)) return rc;
+  if (int rc = read_text(path, &text)) return rc;
   const std::vector<std::string> lines = split_lines(text);
   if (lines.size() < 2) return fail(CLB_EFORMAT, "worklist needs digits= and precision= lines");
-  // *chktxtstr, Bellard_192.hsp:176-207
   std::string digits = lines[0];
   strrep(&digits, "digits");
   strrep(&digits, "=");
@@ -784,25 +804,57 @@ int clb_run_worklist(clb_ctx* ctx, const char* worklist_path, const char* work_d
   strrep(&prec, "precision");
   strrep(&prec, "=");
   strrep(&prec, " ");
-  // *workStart, :221-227
-  if (nd < 5) return fail(CLB_EFORMAT, "worklist digits must be >= 5");
-  if (prec.empty() || (prec[0] != '0' && prec[0] != '1' && prec[0] != '2'))
-    return fail(CLB_EFORMAT, "worklist precision must be 0, 1 or 2");
-  const int precision = prec[0] - '0';
-  uint64_t d;
-  if (int rc = normalize((uint64_t)nd, nullptr, &d, nullptr)) return rc;
-  const uint64_t batch = batch_size ? batch_size : kBatchSize;
-  // *workStart2 (:229-247) and *Batchloop (:298-340)
-  std::vector<clb_job> todo;
-  std::vector<std::string> paths;
-  std::vector<int> formulas;
+  wl->digits = nd;
+  wl->precision = (!prec.empty() && prec[0] >= '0' && prec[0] <= '2') ? prec[0] - '0' : -1;
+  wl->jobs.clear();
   for (size_t li = 2; li < lines.size(); ++li) {
     const std::string& t = lines[li];
     if (t.empty()) continue;  // a blank line is not a job
     const int j = (int)hsp_int(t.substr(0, 1));
     const int64_t b = hsp_int(t.size() > 2 ? t.substr(2) : std::string());
     if (j < 0 || j > 6 || b < 0) return fail(CLB_EFORMAT, "bad job line '%s'", t.c_str());
-    if ((uint64_t)b > UINT64_MAX / batch - 1) return fail(CLB_EFORMAT, "batch %lld too large", (long long)b);
+    wl->jobs.push_back({j, b});
+  }
+  return CLB_OK;
+}
+
+int clb_parse_worklist(const char* worklist_path, uint64_t* digits, int* precision,
+                       int64_t* jobs, size_t cap, size_t* njobs) {
+  if (!worklist_path) return fail(CLB_EINVAL, "null path");
+  Worklist wl;
+  if (int rc = parse_worklist(worklist_path, &wl)) return rc;
+  if (digits) *digits = (uint64_t)wl.digits;
+  if (precision) *precision = wl.precision;
+  if (njobs) *njobs = wl.jobs.size();
+  for (size_t i = 0; jobs && i < wl.jobs.size() && i < cap; ++i) {
+    jobs[2 * i] = wl.jobs[i].first;
+    jobs[2 * i + 1] = wl.jobs[i].second;
+  }
+  return CLB_OK;
+}
+
+int clb_run_worklist(clb_ctx* ctx, const char* worklist_path, const char* work_dir,
+                     uint64_t batch_size, const char* device_label, int* jobs_run) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!worklist_path || !work_dir) return fail(CLB_EINVAL, "null path");
+  if (jobs_run) *jobs_run = 0;
+  Worklist wl;
+  if (int rc = parse_worklist(worklist_path, &wl)) return rc;
+  if (wl.digits < 5) return fail(CLB_EFORMAT, "worklist digits must be >= 5");
+  if (wl.precision < 0) return fail(CLB_EFORMAT, "worklist precision must be 0, 1 or 2");
+  const int precision = wl.precision;
+  uint64_t d;
+  if (int rc = normalize((uint64_t)wl.digits, nullptr, &d, nullptr)) return rc;
+  const uint64_t batch = batch_size ? batch_size : kBatchSize;
+  // *Batchloop (:298-340)
+  std::vector<clb_job> todo;
+  std::vector<std::string> paths;
+  std::vector<int> formulas;
+  for (const auto& jb : wl.jobs) {
+    const int j = jb.first;
+    const int64_t b = jb.second;
+    if ((uint64_t)b > UINT64_MAX / batch - 1)
+      return fail(CLB_EFORMAT, "batch %lld too large", (long long)b);
     const uint64_t k_start = batch * (uint64_t)b, k_end = batch * ((uint64_t)b + 1);
     if (k_start > d) continue;  // :306
     char name[64];
@@ -824,27 +876,24 @@ int clb_run_worklist(clb_ctx* ctx, const char* worklist_path, const char* work_d
   ctx->precision = saved_precision;
   if (rc) return rc;
   const long long ms = (long long)std::llround(ctx->total_ms);
-  if ((rc = make_dir(work_dir))) return rc;
+  const std::string label =
+      device_label ? std::string(device_label)
+                   : std::to_string(ctx->device) + "\t" + ctx->device_name;
   for (size_t i = 0; i < todo.size(); ++i) {
-    char dir[32];
-    snprintf(dir, sizeof dir, "/%d", formulas[i]);
-    if ((rc = make_dir(std::string(work_dir) + dir))) return rc;
-    char hex[49];
-    format_result(precision, &part[3 * i], hex);
-    // fsave, Bellard_192.hsp:514-533 (HSP's "\n" is CR LF)
-    std::string body = std::string(hex) + "\r\n";
-    if (device_label)
-      body += device_label;
-    else
-      body += std::to_string(ctx->device) + "\t" + ctx->device_name;
-    body += "\r\n" + std::to_string(ms) + "ms";
-    FILE* f = fopen(paths[i].c_str(), "wb");
-    if (!f) return fail(CLB_EIO, "cannot write %s: %s", paths[i].c_str(), strerror(errno));
-    const bool ok = fwrite(body.data(), 1, body.size(), f) == body.size();
-    if (fclose(f) != 0 || !ok) return fail(CLB_EIO, "write failed: %s", paths[i].c_str());
+    const int64_t b = (int64_t)(todo[i].k_begin / batch);
+    if ((rc = write_work_file(work_dir, formulas[i], b, &part[3 * i], precision, label, ms)))
+      return rc;
     if (jobs_run) ++*jobs_run;
   }
   return CLB_OK;
+}
+
+int clb_write_work_file(const char* work_dir, int j, int64_t b, const uint64_t w[3],
+                        int precision, const char* device_label, int64_t ms) {
+  if (!work_dir || !w) return fail(CLB_EINVAL, "null pointer");
+  if (j < 0 || j > 6 || b < 0) return fail(CLB_EINVAL, "bad job %d_%lld", j, (long long)b);
+  if (precision < 0 || precision > 2) return fail(CLB_EINVAL, "precision must be 0, 1 or 2");
+  return write_work_file(work_dir, j, b, w, precision, device_label ? device_label : "", ms);
 }
 
 int clb_work_sum(const char* work_dir, char hex[49], int counts[7]) {

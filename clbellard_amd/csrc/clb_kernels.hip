@@ -64,23 +64,27 @@ __device__ __forceinline__ void tile_split(const JobDev& jb, uint64_t kbase, int
   double m = (double)(jb.den0 * (kbase + (uint64_t)lane) + jb.den1);
   double minv = 0.0;
   const double mstep = (double)(jb.den0 * (uint64_t)kWave);
+  // Per-tile uniform facts, so that the term loop compares only 32-bit scalars:
+  const uint64_t left = (jb.k1 - kbase + kWave - 1) / kWave;  // terms of lane 0 before k1
+  const int nt = left < (uint64_t)kTermsPerLane ? (int)left : kTermsPerLane;
   const bool full = kbase >= jb.k0 && kbase + kTileK <= jb.k1;  // every lane, every term valid
-  for (int i = 0; i < kTermsPerLane; ++i) {
-    const uint64_t kw = kbase + (uint64_t)kWave * (uint64_t)i;  // k of lane 0
-    if (kw >= jb.k1) break;
-    const uint64_t k = kw + (uint64_t)lane;
+  // one Newton step refreshes 1/m from the previous term once m >= 2^37 (64 den0 < 2^-27 m)
+  const bool newton1 = full && jb.den0 * (kbase + kWave) + jb.den1 >= (1ull << 37);
+  const uint64_t E0 = 10ull * (jb.d - kbase);                    // exponent of lane 0, term 0
+  const bool big = E0 - 10ull * kWave * (uint64_t)(nt - 1) >= 1664;  // every term in 1024 H + v form
+  for (int i = 0; i < nt; ++i) {
+    const uint64_t k = kbase + (uint64_t)kWave * (uint64_t)i + (uint64_t)lane;
     if (full || (k >= jb.k0 && k < jb.k1)) {
-      // 1/m: one Newton step from the previous term's reciprocal when m moved by < 2^-27 m
-      // (relative error then < 2^-54 + rounding), else the hardware estimate and two steps.
-      if (full && i > 0 && jb.den0 * kw + jb.den1 >= (1ull << 37)) {
+      // 1/m: hardware estimate and two Newton steps, or one step from the previous term's.
+      if (newton1 && i > 0) {
         const double e = __builtin_fma(-m, minv, 1.0);
         minv = __builtin_fma(minv, e, minv);
       } else {
         minv = recip(m);
       }
-      const uint64_t E = 10ull * (jb.d - kw);
+      const uint64_t E = E0 - 10ull * kWave * (uint64_t)i;
       double rem;
-      if (E >= 1664) {
+      if (big || E >= 1664) {
         const uint64_t F = E - 640;
         rem = fpow2_split<W>(F >> 10, (uint32_t)(F & 1023) + 640u - ten_lane, m, minv);
       } else {

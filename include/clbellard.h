@@ -137,6 +137,16 @@ int clb_u192_to_dd(const uint64_t w[3], double dd[2]);
  * *jobs_run receives the number of files written (may be NULL). */
 int clb_run_worklist(clb_ctx *ctx, const char *worklist_path, const char *work_dir,
                      uint64_t batch_size, const char *device_label, int *jobs_run);
+/* *chktxtstr + *workStart2 (Bellard_192.hsp:176-247): parse a worklist without running it.
+ * digits, precision: the values of lines 0 and 1 (digits clamped to [0, 999999999999999999]);
+ * up to cap jobs (formula, batch index) from the following lines into jobs[2*i], jobs[2*i+1];
+ * *njobs = number of job lines (may exceed cap).  Blank lines are skipped. */
+int clb_parse_worklist(const char *worklist_path, uint64_t *digits, int *precision,
+                       int64_t *jobs, size_t cap, size_t *njobs);
+/* fsave (Bellard_192.hsp:514-533): writes <work_dir>/<j>/<j>_<b>.txt (creating the directories):
+ * the value w formatted for the precision, then device_label, then "<ms>ms", CR LF separated. */
+int clb_write_work_file(const char *work_dir, int j, int64_t b, const uint64_t w[3],
+                        int precision, const char *device_label, int64_t ms);
 /* workmodeSum0_6 (workmodeSum0_6.hsp:1-49): for j = 0..6 and b = 0, 1, ... until the first missing
  * file, adds line 0 of <work_dir>/<j>/<j>_<b>.txt mod 16^48.  counts[j] = files used. */
 int clb_work_sum(const char *work_dir, char hex[49], int counts[7]);

@@ -31,6 +31,15 @@ def test_exports_every_declared_symbol():
     assert set(names) <= set(clb._SIGNATURES)
 
 
+def test_missing_library_fails_loudly(tmp_path):
+    import subprocess, sys
+    code = ("import clbellard_amd as c; c.LIB_PATH = %r; c._Lib._handle = None\n"
+            "try:\n    c.normalize(10)\nexcept ImportError as e:\n    print('ImportError', e)\n")
+    out = subprocess.run([sys.executable, "-c", code % str(tmp_path / "none.so")], cwd=ROOT,
+                         capture_output=True, text=True)
+    assert "ImportError" in out.stdout and "no CPU fallback" in out.stdout
+
+
 def test_version():
     assert b"gfx950" in clb.lib.clb_version()
 
@@ -135,3 +144,68 @@ def test_null_and_range_errors():
     assert clb.lib.clb_hex48(None, None) == -1
     assert clb.lib.clb_batch_calc(None, 10, 0, 10, 0) == -1
     assert clb.lib.clb_last_error()
+
+
+def _wl(tmp_path, text, name="wl.txt"):
+    p = tmp_path / name
+    p.write_bytes(text.encode())
+    return str(p)
+
+
+def test_parse_worklist_reference_sample(tmp_path):
+    # the format of work/worklist_sample.txt: digits=, precision= with a comment, j_b lines, LF
+    p = _wl(tmp_path, "digits=2000000000\nprecision=1 (0=106bit , 1=192bit)\n0_0\n1_0\n2_0\n"
+                      "3_0\n4_0\n5_0\n6_0")
+    assert clb.parse_worklist(p) == (2000000000, 1, [(j, 0) for j in range(7)])
+
+
+def test_parse_worklist_variants(tmp_path):
+    # CR LF, spaces, a blank line, thousands of jobs, digits clamped to the reference maximum
+    jobs = "\r\n".join(f"{j}_{b}" for b in range(500) for j in range(7))
+    p = _wl(tmp_path, "digits = 1 000 000\r\nprecision = 2\r\n\r\n" + jobs + "\r\n")
+    d, prec, js = clb.parse_worklist(p)
+    assert (d, prec, len(js)) == (1000000, 2, 3500)
+    assert js[7] == (0, 1)
+    p = _wl(tmp_path, "digits=99999999999999999999\nprecision=0\n", "big.txt")
+    assert clb.parse_worklist(p)[:2] == (999999999999999999, 0)
+    p = _wl(tmp_path, "digits=100\nprecision=x\n0_0\n", "badprec.txt")
+    assert clb.parse_worklist(p)[1] == -1
+    p = _wl(tmp_path, "digits=100\nprecision=1\n9_0\n", "badjob.txt")
+    with pytest.raises(clb.ClbError) as e:
+        clb.parse_worklist(p)
+    assert e.value.code == -6
+    with pytest.raises(clb.ClbError) as e:
+        clb.parse_worklist(str(tmp_path / "missing.txt"))
+    assert e.value.code == -4
+
+
+def test_cli_sum_and_usage(tmp_path):
+    import subprocess
+    cli = os.path.join(ROOT, "clbellard_amd", "clbellard")
+    if not os.path.exists(cli):
+        pytest.skip("CLI not built")
+    v1, v2 = R.randrange(0, 2**192), R.randrange(0, 2**192)
+    _write_work(str(tmp_path), 0, 0, f"{v1:048x}")
+    _write_work(str(tmp_path), 5, 0, f"{v2:048x}")
+    out = subprocess.run([cli, "sum", "--workdir", str(tmp_path)], capture_output=True, text=True)
+    assert out.returncode == 0
+    assert f"{(v1 + v2) % 2**192:048x}" in out.stdout
+    assert "0:1" in out.stdout and "5:1" in out.stdout and "3:0" in out.stdout
+    assert subprocess.run([cli], capture_output=True).returncode == 2
+    assert subprocess.run([cli, "bogus", "1"], capture_output=True).returncode != 0
+
+
+def test_write_work_file_round_trip(tmp_path, oracle):
+    d = str(tmp_path / "work")
+    vals = [R.randrange(0, 2**192) for _ in range(5)]
+    for b, v in enumerate(vals):
+        clb.write_work_file(d, 4, b, i2w(v), clb.PREC_U192, "0\tdev", 1234)
+    body = open(os.path.join(d, "4", "4_2.txt"), "rb").read()
+    assert body == f"{vals[2]:048x}\r\n0\tdev\r\n1234ms".encode()
+    assert clb.work_sum(d) == (f"{sum(vals) % 2**192:048x}", [0, 0, 0, 0, 5, 0, 0])
+    clb.write_work_file(d, 6, 0, i2w(vals[0]), clb.PREC_DD106, "x", 1)
+    line0 = open(os.path.join(d, "6", "6_0.txt"), "rb").read().split(b"\r\n")[0].decode()
+    hi, lo = clb.u192_to_dd(i2w(vals[0]))
+    assert line0 == oracle.hex_dd(hi, lo) and line0.endswith("0" * 20)
+    with pytest.raises(clb.ClbError):
+        clb.write_work_file(d, 7, 0, i2w(1))
