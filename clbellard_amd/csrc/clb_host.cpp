@@ -108,6 +108,7 @@ struct clb_ctx {
   DevBuf<uint64_t> out;
   DevBuf<uint64_t> slot_out;
   float head_ms = 0.f, total_ms = 0.f;
+  int resident[5] = {0, 0, 0, 0, 0};  // resident k_head workgroups per CU, per kernel kind
 };
 
 namespace {
@@ -170,7 +171,7 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
     return fail(CLB_EINVAL, "too many jobs");
   if ((rc = ctx->jobs.reserve(jobs.size() + 1)) || (rc = ctx->tails.reserve(tails.size() + 1)) ||
       (rc = ctx->groups.reserve(groups.size())) || (rc = ctx->shifts.reserve(nslots)) ||
-      (rc = ctx->acc.reserve(nslots * kAccWords)) || (rc = ctx->out.reserve(3 * groups.size())) ||
+      (rc = ctx->acc.reserve((nslots + 1) * kAccWords)) || (rc = ctx->out.reserve(3 * groups.size())) ||
       (rc = ctx->slot_out.reserve(3 * nslots)))
     return rc;
   hipStream_t s = ctx->stream;
@@ -184,10 +185,25 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
   HIP_TRY(hipMemcpyAsync(ctx->groups.p, groups.data(), groups.size() * sizeof(GroupDev),
                          hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(ctx->shifts.p, shift.data(), nslots, hipMemcpyHostToDevice, s));
-  HIP_TRY(launch_zero(ctx->acc.p, nslots * kAccWords, s));
-  const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : 8;
+  HIP_TRY(launch_zero(ctx->acc.p, (nslots + 1) * kAccWords, s));  // slots + the tile counter
+  // persistent grid: every workgroup resident at once (one wave of blocks), unless overridden
+  int bpc = ctx->blocks_per_cu;
+  if (bpc <= 0) {
+    if (ctx->resident[kind] == 0) {
+      const int r = head_blocks_per_cu(kind);
+      ctx->resident[kind] = r > 0 ? r : 8;
+    }
+    bpc = ctx->resident[kind];
+  }
   HIP_TRY(hipEventRecord(ctx->ev[1], s));
-  HIP_TRY(launch_head(kind, ctx->jobs.p, (int)jobs.size(), flat, ctx->acc.p, ctx->num_cus * bpc, s));
+  const uint64_t grid = (uint64_t)ctx->num_cus * (uint64_t)bpc;
+  // Chunks of up to 256 tiles (~18 ms of one wave's work at N = 10^12), at least 8 chunks per
+  // wave.  Device-scope atomics go past the per-XCD L2, so fewer chunks keep the counter's
+  // memory transactions negligible.
+  const uint64_t per_wave = flat / (grid * (kBlock / kWave) * 8 + 1);
+  const uint32_t chunk = (uint32_t)(per_wave < 1 ? 1 : per_wave > 256 ? 256 : per_wave);
+  HIP_TRY(launch_head(kind, ctx->jobs.p, (int)jobs.size(), flat, chunk,
+                      ctx->acc.p + nslots * kAccWords, ctx->acc.p, (int)grid, s));
   HIP_TRY(hipEventRecord(ctx->ev[2], s));
   HIP_TRY(launch_tail(ctx->tails.p, (int)tails.size(), ctx->acc.p, s));
   uint64_t* dst = accumulate_dev ? accumulate_dev : ctx->out.p;

@@ -47,9 +47,14 @@ struct GroupDev {
   uint32_t count;
 };
 
+// Workgroups of k_head<kind> that fit on one compute unit at once (0 if unknown).
+int head_blocks_per_cu(int kind);
+
 // Kernel launch wrappers (clb_kernels.hip).  All asynchronous on `stream`.
+// counter: one zeroed u64 the waves take chunks of `chunk` tiles from.
 hipError_t launch_head(int kind, const JobDev* jobs, int njobs, uint64_t total_tiles,
-                       unsigned long long* acc, int grid, hipStream_t stream);
+                       uint32_t chunk, unsigned long long* counter, unsigned long long* acc,
+                       int grid, hipStream_t stream);
 hipError_t launch_tail(const TailDev* tails, int ntails, unsigned long long* acc,
                        hipStream_t stream);
 hipError_t launch_zero(unsigned long long* p, uint64_t nwords, hipStream_t stream);

@@ -160,37 +160,48 @@ __device__ __forceinline__ void tile_terms(const JobDev& jb, uint64_t kbase, int
   if constexpr (KIND == kPlainWide) tile_plain<true>(jb, kbase, lane, acc, nterms);
 }
 
-// Persistent head-term kernel.  Flat tile f belongs to the last job with flat_begin <= f.
+// Persistent head-term kernel.  Waves take chunks of `chunk` consecutive flat tiles from a
+// global counter (one atomic per chunk), so the launch balances itself whatever the cost of a
+// tile (bitlen of its exponents) and however many waves are resident.  A wave's chunks increase
+// monotonically, so its job index only moves forward.  Flat tile f belongs to the last job with
+// flat_begin <= f.
 template <int KIND>
 __global__ void __launch_bounds__(kBlock) k_head(const JobDev* __restrict__ jobs, int njobs,
-                                                 uint64_t total_tiles,
+                                                 uint64_t total_tiles, uint32_t chunk,
+                                                 unsigned long long* __restrict__ counter,
                                                  unsigned long long* __restrict__ acc) {
   const int lane = (int)(threadIdx.x & (kWave - 1));
-  const uint32_t wave =
-      __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
-  const uint32_t nwaves = (gridDim.x * blockDim.x) / kWave;
   int cur = -1;
   JobDev jb{};
   U192 sum = {0, 0, 0};
   uint64_t nterms = 0;
-  for (uint64_t f = wave; f < total_tiles; f += nwaves) {
-    int lo = (cur < 0) ? 0 : cur, hi = njobs - 1;  // jobs are visited in ascending order
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (jobs[mid].flat_begin <= f)
-        lo = mid;
-      else
-        hi = mid - 1;
+  for (;;) {
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(counter, (unsigned long long)chunk);
+    const uint64_t f0 =
+        ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32) |
+        (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)base);
+    if (f0 >= total_tiles) break;
+    const uint64_t f1 = f0 + chunk < total_tiles ? f0 + chunk : total_tiles;
+    for (uint64_t f = f0; f < f1; ++f) {
+      int lo = (cur < 0) ? 0 : cur, hi = njobs - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].flat_begin <= f)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      if (lo != cur) {
+        if (cur >= 0) flush_job(sum, nterms, jb.numesign, lane, acc + (size_t)jb.slot * kAccWords);
+        cur = lo;
+        jb = jobs[lo];
+        sum = {0, 0, 0};
+        nterms = 0;
+      }
+      const uint64_t kbase = (jb.tile_first + (f - jb.flat_begin)) * kTileK;
+      tile_terms<KIND>(jb, kbase, lane, sum, nterms);
     }
-    if (lo != cur) {
-      if (cur >= 0) flush_job(sum, nterms, jb.numesign, lane, acc + (size_t)jb.slot * kAccWords);
-      cur = lo;
-      jb = jobs[lo];
-      sum = {0, 0, 0};
-      nterms = 0;
-    }
-    const uint64_t kbase = (jb.tile_first + (f - jb.flat_begin)) * kTileK;
-    tile_terms<KIND>(jb, kbase, lane, sum, nterms);
   }
   if (cur >= 0) flush_job(sum, nterms, jb.numesign, lane, acc + (size_t)jb.slot * kAccWords);
 }
@@ -292,33 +303,41 @@ __global__ void k_sum7(uint64_t* ans, uint64_t* partial, int shift) {
 
 // ------------------------------------------------------------------ launch wrappers
 
-hipError_t launch_head(int kind, const JobDev* jobs, int njobs, uint64_t total_tiles,
-                       unsigned long long* acc, int grid, hipStream_t stream) {
-  if (njobs <= 0 || total_tiles == 0) return hipSuccess;
+template <int KIND>
+static int resident_blocks() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_head<KIND>, kBlock, 0) != hipSuccess)
+    n = 0;
+  return n;
+}
+
+int head_blocks_per_cu(int kind) {
   switch (kind) {
-    case kSplitW3:
-      hipLaunchKernelGGL(k_head<kSplitW3>, dim3(grid), dim3(kBlock), 0, stream, jobs, njobs,
-                         total_tiles, acc);
-      break;
-    case kSplitW2:
-      hipLaunchKernelGGL(k_head<kSplitW2>, dim3(grid), dim3(kBlock), 0, stream, jobs, njobs,
-                         total_tiles, acc);
-      break;
-    case kSplitW1:
-      hipLaunchKernelGGL(k_head<kSplitW1>, dim3(grid), dim3(kBlock), 0, stream, jobs, njobs,
-                         total_tiles, acc);
-      break;
-    case kPlainW3:
-      hipLaunchKernelGGL(k_head<kPlainW3>, dim3(grid), dim3(kBlock), 0, stream, jobs, njobs,
-                         total_tiles, acc);
-      break;
-    case kPlainWide:
-      hipLaunchKernelGGL(k_head<kPlainWide>, dim3(grid), dim3(kBlock), 0, stream, jobs, njobs,
-                         total_tiles, acc);
-      break;
-    default:
-      return hipErrorInvalidValue;
+    case kSplitW3: return resident_blocks<kSplitW3>();
+    case kSplitW2: return resident_blocks<kSplitW2>();
+    case kSplitW1: return resident_blocks<kSplitW1>();
+    case kPlainW3: return resident_blocks<kPlainW3>();
+    case kPlainWide: return resident_blocks<kPlainWide>();
+    default: return 0;
   }
+}
+
+hipError_t launch_head(int kind, const JobDev* jobs, int njobs, uint64_t total_tiles,
+                       uint32_t chunk, unsigned long long* counter, unsigned long long* acc,
+                       int grid, hipStream_t stream) {
+  if (njobs <= 0 || total_tiles == 0) return hipSuccess;
+#define CLB_LAUNCH_HEAD(K)                                                                   \
+  hipLaunchKernelGGL(k_head<K>, dim3(grid), dim3(kBlock), 0, stream, jobs, njobs, total_tiles, \
+                     chunk, counter, acc)
+  switch (kind) {
+    case kSplitW3: CLB_LAUNCH_HEAD(kSplitW3); break;
+    case kSplitW2: CLB_LAUNCH_HEAD(kSplitW2); break;
+    case kSplitW1: CLB_LAUNCH_HEAD(kSplitW1); break;
+    case kPlainW3: CLB_LAUNCH_HEAD(kPlainW3); break;
+    case kPlainWide: CLB_LAUNCH_HEAD(kPlainWide); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef CLB_LAUNCH_HEAD
   return hipGetLastError();
 }
 
