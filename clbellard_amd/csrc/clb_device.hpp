@@ -116,12 +116,16 @@ __device__ __forceinline__ double fpow2(uint64_t e, int s, double m, double minv
 // lanes of a wave have exponents E - 10 L, which this splits into a common high part H, whose
 // window digits are scalar, and a per-lane low part v.  Bit 10 of v is absorbed by the last
 // window of H (digit up to 2^W), then ten squares take the low ten bits of v.
-// Valid while m 2^(2^W) <= 2^50.
+// Valid while m 2^(2^W) <= 2^50.  The top `lead` (3 or 4, uniform) to lead + W - 1 bits of H form
+// t and give the start value 2^t - rint(2^t / m) m in one exact step; with 2^t / m <= 2^49 the
+// quotient is off by at most 0.5 + 1.5 2^-3, so |x| <= 0.69 m: lead 3 (t <= 32) for any m,
+// lead 4 (t <= 64) for m >= 2^15.
 template <int W>
-__device__ __forceinline__ double fpow2_split(uint64_t H, uint32_t v, double m, double minv) {
+__device__ __forceinline__ double fpow2_split(uint64_t H, uint32_t v, int lead, double m,
+                                              double minv) {
   constexpr uint32_t mask = (1u << W) - 1;
   const int bl = 64 - __builtin_clzll(H);
-  const int s = bl > 3 ? W * ((bl - 3) / W) : 0;  // H = t 2^s + (s window bits), t < 2^(W+2)
+  const int s = bl > lead ? W * ((bl - lead) / W) : 0;  // H = t 2^s + (s window bits)
   const uint32_t t = (uint32_t)(H >> s);
   const int carry = (int)(v >> 10);
   double x;

@@ -72,6 +72,7 @@ __device__ __forceinline__ void tile_split(const JobDev& jb, uint64_t kbase, int
   const bool newton1 = full && jb.den0 * (kbase + kWave) + jb.den1 >= (1ull << 37);
   const uint64_t E0 = 10ull * (jb.d - kbase);                    // exponent of lane 0, term 0
   const bool big = E0 - 10ull * kWave * (uint64_t)(nt - 1) >= 1664;  // every term in 1024 H + v form
+  const int lead = jb.den0 * kbase + jb.den1 >= (1u << 15) ? 4 : 3;  // smallest m of the tile
   for (int i = 0; i < nt; ++i) {
     const uint64_t k = kbase + (uint64_t)kWave * (uint64_t)i + (uint64_t)lane;
     if (full || (k >= jb.k0 && k < jb.k1)) {
@@ -86,7 +87,7 @@ __device__ __forceinline__ void tile_split(const JobDev& jb, uint64_t kbase, int
       double rem;
       if (big || E >= 1664) {
         const uint64_t F = E - 640;
-        rem = fpow2_split<W>(F >> 10, (uint32_t)(F & 1023) + 640u - ten_lane, m, minv);
+        rem = fpow2_split<W>(F >> 10, (uint32_t)(F & 1023) + 640u - ten_lane, lead, m, minv);
       } else {
         const int bl = 64 - __builtin_clzll(E);
         const int s = bl > 3 ? W * ((bl - 3) / W) : 0;
