@@ -68,6 +68,16 @@ def build_oracle() -> str:
     return os.path.join(ROOT, "oracle", "liboracle.so")
 
 
+def build_reference_kernels(ref_dir: str = "/root/reference") -> bool:
+    """The reference's OpenCL kernels compiled from their sources into oracle/_ref/, with the
+    replay host (GPU-side checker for the tests; not part of the product).  Skipped, returning
+    False, where the reference sources are absent."""
+    if not os.path.exists(os.path.join(ref_dir, "Bellard.cl")):
+        return False
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={ref_dir}"])
+    return True
+
+
 def main(argv=None) -> int:
     build_library()
     if os.path.exists(os.path.join(CSRC, "clb_cli.cpp")):

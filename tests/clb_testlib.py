@@ -7,6 +7,7 @@ from __future__ import annotations
 import ctypes
 import json
 import os
+import struct
 import subprocess
 from fractions import Fraction
 
@@ -216,3 +217,75 @@ def pi_series_check(terms: int = 12) -> Fraction:
             s += sign * Fraction(NUMER[j], DEN0[j] * n + DEN1[j])
         total += Fraction((-1) ** n, 1024 ** n) * s
     return total / 64
+
+
+# ----------------------------------------- the reference's own kernels (oracle/_ref, GPU only)
+
+REF_DIR = os.path.join(ORACLE_DIR, "_ref")
+REF_LIB = os.path.join(REF_DIR, "libref_runner.so")
+
+
+def loopmax32(j: int) -> int:
+    """First k whose modulus leaves 32 bits (KernelFuncs.as:31)."""
+    return ((1 << 32) - DEN1[j] - 1) // DEN0[j] + 1
+
+
+class RefKernels:
+    """ctypes view of oracle/ref_runner.c: the reference's OpenCL kernels (compiled from its
+    sources into oracle/_ref/) driven through the OpenCL runtime with the reference's launch
+    sequence.  precision 0 returns the double-double (hi, lo); 1 and 2 return 192-bit words in
+    bigSum order (most significant first)."""
+
+    def __init__(self, path: str = REF_LIB, co_dir: str = REF_DIR):
+        lib = ctypes.CDLL(path)
+        u64, c_int = ctypes.c_uint64, ctypes.c_int
+        p64 = ctypes.POINTER(ctypes.c_uint64)
+        lib.ref_init.argtypes, lib.ref_init.restype = [ctypes.c_char_p], c_int
+        lib.ref_normal.argtypes, lib.ref_normal.restype = [u64, c_int, p64], c_int
+        lib.ref_job.argtypes = [u64, u64, u64, c_int, c_int, p64]
+        lib.ref_job.restype = c_int
+        lib.ref_last_error.restype = ctypes.c_char_p
+        lib.ref_last_ms.restype = ctypes.c_double
+        lib.ref_shutdown.restype = None
+        self.lib = lib
+        self._check(lib.ref_init(co_dir.encode()))
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise RuntimeError("reference kernels: " + self.lib.ref_last_error().decode())
+
+    @staticmethod
+    def _value(w, precision: int):
+        if precision == 0:
+            hi, lo = (struct.unpack("<d", struct.pack("<Q", x))[0] for x in w[:2])
+            return hi, lo
+        return tuple(w)
+
+    def normal(self, d: int, precision: int):
+        w = U3()
+        self._check(self.lib.ref_normal(d, precision, w))
+        return self._value(w, precision)
+
+    def job(self, d: int, k0: int, k1: int, j: int, precision: int):
+        w = U3()
+        self._check(self.lib.ref_job(d, k0, k1, j, precision, w))
+        return self._value(w, precision)
+
+    def last_ms(self) -> float:
+        return self.lib.ref_last_ms()
+
+    def close(self) -> None:
+        self.lib.ref_shutdown()
+
+
+def dd_error_ulps(hi: float, lo: float, exact: int) -> int:
+    """|frac(hi + lo) - exact / 2^192| in units of 2^-192 (distance on the unit circle)."""
+    v = (Fraction(hi) + Fraction(lo)) * (1 << 192)
+    diff = (v - exact) % (1 << 192)
+    diff = min(diff, (1 << 192) - diff)
+    return int(diff) + (1 if diff.denominator != 1 else 0)
+
+
+def u192_error_ulps(w, exact: int) -> int:
+    diff = (w2i(w) - exact) % (1 << 192)
+    return min(diff, (1 << 192) - diff)
