@@ -20,8 +20,8 @@ import time
 import pytest
 
 import clbellard_amd as clb
-from clb_testlib import (NUMER, REF_LIB, RefKernels, dd_error_ulps, loopmax32, py_normalize,
-                         u192_error_ulps, w2i)
+from clb_testlib import (NUMER, REF_LIB, RefKernels, dd_error_ulps, i2w, loopmax32,
+                         py_normalize, u192_error_ulps, w2i)
 
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not os.path.exists(REF_LIB),
@@ -168,3 +168,34 @@ def test_reference_kernel_rate(ref, ctx):
         note(f"rate {k}: {v:.3e} term-formulas/s")
     assert rates["ours"] > rates["reference precision 2"]
     assert rates["ours"] > rates["reference precision 1"]
+
+
+@pytest.mark.skipif(not os.environ.get("CLB_REF_FULL_N"),
+                    reason="a full reference run takes minutes; set CLB_REF_FULL_N=<position>")
+@pytest.mark.parametrize("precision", [2, 1])
+def test_full_position_reference_kernels(ref, ctx, precision):
+    """A whole position on the reference kernels, formula by formula (Allloop,
+    Bellard_192.hsp:264-296; dev_ans accumulation done here as the mod 2^192 sum of the Sum7
+    outputs), against ours: 24 hex digits equal, error within the double-double bound, and both
+    wall times printed."""
+    n = int(os.environ["CLB_REF_FULL_N"])
+    nn, d = py_normalize(n)
+    t0 = time.perf_counter()
+    frac, hexs, _ = ctx.normal(n)
+    ours_s = time.perf_counter() - t0
+    note(f"ours N={nn}: {hexs} in {ours_s:.2f} s")
+    total, ref_s = 0, 0.0
+    for j in range(7):
+        t0 = time.perf_counter()
+        part = ref.job(d, 0, d, j, precision)
+        dt = time.perf_counter() - t0
+        ref_s += dt
+        total = (total + w2i(part)) % (1 << 192)
+        note(f"reference precision {precision} formula {j}: {dt:.2f} s")
+    ref_hex = f"{total:048x}"
+    err = u192_error_ulps(i2w(total), w2i(frac))
+    note(f"reference N={nn} precision {precision}: {ref_hex} in {ref_s:.2f} s, "
+         f"error 2^{log2(err):.1f} x 2^-192; {7 * (d + 20) / ref_s:.3e} vs "
+         f"{7 * (d + 20) / ours_s:.3e} term-formulas/s")
+    assert ref_hex[:24] == hexs[:24]
+    assert err <= dd192_bound(d + 20, sum(NUMER))
