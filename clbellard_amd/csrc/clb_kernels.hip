@@ -2,8 +2,8 @@
 //
 // Work decomposition.  A launch holds jobs (formula j, loop count d, head range [k0, k1)).  Each
 // job's range is cut into tiles of kTileK consecutive k aligned to multiples of kTileK; the tiles
-// of all jobs form one flat index space that a persistent grid walks round-robin, one tile per
-// wave at a time.  Lane L of a tile evaluates k = base + L + 64 i, i < kTermsPerLane, so every
+// of all jobs form one flat index space that a persistent grid consumes in chunks of consecutive
+// tiles taken from a global counter.  Lane L of a tile evaluates k = base + L + 64 i, i < kTermsPerLane, so every
 // lane sees one parity of k and hence one sign (KernelFuncs.as:10 and the (numesign + k) % 2 rule
 // of Bellard_div2by1.cl:335).  Each lane keeps the 192-bit sum Q of its terms; when the wave moves
 // to another job (and at the end) the lane applies its sign (-Q - n for n negative terms,
@@ -56,62 +56,77 @@ __device__ __forceinline__ void flush_job(U192 v, uint64_t nterms, uint32_t nume
 // Lane L's exponent is E - 10 L with E the exponent of lane 0 (uniform): for E >= 1664 it is
 // evaluated as 2^(1024 H + v) with H = (E - 640) >> 10 uniform and v = ((E - 640) & 1023) + 640
 // - 10 L in [10, 1663] per lane (fpow2_split); smaller exponents take per-lane windows.
-template <int W>
-__device__ __forceinline__ void tile_split(const JobDev& jb, uint64_t kbase, int lane, U192& acc,
-                                           uint64_t& nterms) {
-  double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0;
+// kFast: every term of the tile lies in the job and has E >= 1664, and m >= 2^37 from the second
+// term on (nearly all tiles), so the term loop carries no range check, no exponent-form test, no
+// per-term count and no choice of reciprocal update.
+template <int W, bool kFast>
+__device__ __forceinline__ void split_terms(const JobDev& jb, uint64_t kbase, int lane, int nt,
+                                            bool newton1, uint64_t E0, int lead, double (&f)[4],
+                                            uint32_t& corr, uint64_t& nterms) {
+  if (kFast) newton1 = true;
   const uint32_t ten_lane = 10u * (uint32_t)lane;
   double m = (double)(jb.den0 * (kbase + (uint64_t)lane) + jb.den1);
-  double minv = 0.0;
   const double mstep = (double)(jb.den0 * (uint64_t)kWave);
-  // Per-tile uniform facts, so that the term loop compares only 32-bit scalars:
-  const uint64_t left = (jb.k1 - kbase + kWave - 1) / kWave;  // terms of lane 0 before k1
-  const int nt = left < (uint64_t)kTermsPerLane ? (int)left : kTermsPerLane;
-  const bool full = kbase >= jb.k0 && kbase + kTileK <= jb.k1;  // every lane, every term valid
-  // one Newton step refreshes 1/m from the previous term once m >= 2^37 (64 den0 < 2^-27 m)
-  const bool newton1 = full && jb.den0 * (kbase + kWave) + jb.den1 >= (1ull << 37);
-  const uint64_t E0 = 10ull * (jb.d - kbase);                    // exponent of lane 0, term 0
-  const bool big = E0 - 10ull * kWave * (uint64_t)(nt - 1) >= 1664;  // every term in 1024 H + v form
-  const int lead = jb.den0 * kbase + jb.den1 >= (1u << 15) ? 4 : 3;  // smallest m of the tile
+  double minv = recip(m);
   for (int i = 0; i < nt; ++i) {
     const uint64_t k = kbase + (uint64_t)kWave * (uint64_t)i + (uint64_t)lane;
-    if (full || (k >= jb.k0 && k < jb.k1)) {
-      // 1/m: hardware estimate and two Newton steps, or one step from the previous term's.
-      if (newton1 && i > 0) {
-        const double e = __builtin_fma(-m, minv, 1.0);
-        minv = __builtin_fma(minv, e, minv);
-      } else {
-        minv = recip(m);
-      }
+    if (kFast || (k >= jb.k0 && k < jb.k1)) {
       const uint64_t E = E0 - 10ull * kWave * (uint64_t)i;
       double rem;
-      if (big || E >= 1664) {
+      if (kFast || E >= 1664) {
         const uint64_t F = E - 640;
-        rem = fpow2_split<W>(F >> 10, (uint32_t)(F & 1023) + 640u - ten_lane, lead, m, minv);
+        const uint32_t vs = (uint32_t)(F & 1023) + 640u;  // uniform part of v
+        rem = fpow2_split<W>(F >> 10, vs - ten_lane, lead, m, minv);
       } else {
         const int bl = 64 - __builtin_clzll(E);
         const int s = bl > 3 ? W * ((bl - 3) / W) : 0;
         rem = fpow2<W>(E - ten_lane, s, m, minv);
       }
-      f0 += fdigit(rem, m, minv);
-      f1 += fdigit(rem, m, minv);
-      f2 += fdigit(rem, m, minv);
-      const double q3 = fdigit(rem, m, minv);
-      f3 += q3 + (rem >= m ? 1.0 : 0.0);  // lazy remainder in [m, 2m): one more
-      if (!full) nterms += (m > 1.0);     // m = 1 gives r = 0 and a zero term of either sign
+      f[0] += fdigit(rem, m, minv);
+      f[1] += fdigit(rem, m, minv);
+      f[2] += fdigit(rem, m, minv);
+      f[3] += fdigit(rem, m, minv);
+      corr += rem >= m ? 1u : 0u;          // lazy remainder in [m, 2m): the last digit is one more
+      if (!kFast) nterms += (m > 1.0);     // m = 1 gives r = 0 and a zero term of either sign
     }
     m += mstep;
+    // 1/m of the next term: one Newton step from this one's once m >= 2^37 (the step
+    // 64 den0 < 2^-27 m), else the hardware estimate and two steps.
+    if (newton1) {
+      const double e = __builtin_fma(-m, minv, 1.0);
+      minv = __builtin_fma(minv, e, minv);
+    } else {
+      minv = recip(m);
+    }
   }
-  if (full) {  // every term counted; k = 0 with den1 = 1 is the only m = 1
-    nterms += kTermsPerLane;
+}
+
+template <int W>
+__device__ __forceinline__ void tile_split(const JobDev& jb, uint64_t kbase, int lane, U192& acc,
+                                           uint64_t& nterms) {
+  // Per-tile uniform facts, so that the term loop compares only 32-bit scalars:
+  const uint64_t left = (jb.k1 - kbase + kWave - 1) / kWave;  // terms of lane 0 before k1
+  const int nt = left < (uint64_t)kTermsPerLane ? (int)left : kTermsPerLane;
+  const bool full = kbase >= jb.k0 && kbase + kTileK <= jb.k1;  // every lane, every term valid
+  const bool newton1 = jb.den0 * (kbase + kWave) + jb.den1 >= (1ull << 37);  // m of terms i >= 1
+  const uint64_t E0 = 10ull * (jb.d - kbase);                    // exponent of lane 0, term 0
+  const bool big = E0 - 10ull * kWave * (uint64_t)(nt - 1) >= 1664;  // every term in 1024 H + v form
+  const int lead = jb.den0 * kbase + jb.den1 >= (1u << 15) ? 4 : 3;  // smallest m of the tile
+  double f[4] = {0.0, 0.0, 0.0, 0.0};
+  uint32_t corr = 0;
+  if (full && big && newton1) {
+    split_terms<W, true>(jb, kbase, lane, nt, newton1, E0, lead, f, corr, nterms);
+    nterms += kTermsPerLane;  // every term counted; k = 0 with den1 = 1 is the only m = 1
     if (kbase == 0 && lane == 0 && jb.den1 == 1) nterms -= 1;
+  } else {
+    split_terms<W, false>(jb, kbase, lane, nt, newton1, E0, lead, f, corr, nterms);
   }
-  // Digit i has weight 2^(144 - 48 i).  f0 may be negative (balanced residue); adding 2^52 keeps
-  // it non-negative and changes the sum by 2^196 = 0 mod 2^192.
-  const uint64_t u3 = f2u64(f3);
-  const uint64_t u2 = f2u64(f2);
-  const uint64_t u1 = f2u64(f1);
-  const uint64_t u0 = f2u64(f0 + 0x1p52);
+  // Digit i has weight 2^(144 - 48 i).  f[0] may be negative (balanced residue); adding 2^52
+  // keeps it non-negative and changes the sum by 2^196 = 0 mod 2^192.
+  const uint64_t u3 = f2u64(f[3]) + corr;
+  const uint64_t u2 = f2u64(f[2]);
+  const uint64_t u1 = f2u64(f[1]);
+  const uint64_t u0 = f2u64(f[0] + 0x1p52);
   add192(acc, u3, 0, 0);
   add192(acc, u2 << 48, u2 >> 16, 0);
   add192(acc, 0, u1 << 32, u1 >> 32);

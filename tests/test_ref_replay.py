@@ -111,7 +111,7 @@ def test_double_double_kernels_error(ref, ctx, precision):
     note(f"precision {precision}: worst error 2^{log2(worst):.1f} x 2^-192")
 
 
-POSITIONS = [1_000_000, 10_000_000, 100_000_000, 1_000_000_000, 10_000_000_000]
+POSITIONS = [1_000_000, 10_000_000, 100_000_000, 1_000_000_000, 2_000_000_000, 10_000_000_000]
 
 
 @pytest.mark.parametrize("n", POSITIONS)
