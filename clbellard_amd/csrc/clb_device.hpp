@@ -116,16 +116,23 @@ __device__ __forceinline__ double fpow2(uint64_t e, int s, double m, double minv
 // lanes of a wave have exponents E - 10 L, which this splits into a common high part H, whose
 // window digits are scalar, and a per-lane low part v.  Bit 10 of v is absorbed by the last
 // window of H (digit up to 2^W), then ten squares take the low ten bits of v.
-// Valid while m 2^(2^W) <= 2^50.  The top `lead` (3 or 4, uniform) to lead + W - 1 bits of H form
-// t and give the start value 2^t - rint(2^t / m) m in one exact step; with 2^t / m <= 2^49 the
-// quotient is off by at most 0.5 + 1.5 2^-3, so |x| <= 0.69 m: lead 3 (t <= 32) for any m,
-// lead 4 (t <= 64) for m >= 2^15.
+// Valid while m 2^(2^W) <= 2^50.  The leading bits t of H (everything above the lowest s bits, s
+// the smallest multiple of W that leaves t <= tmax) give the start value 2^t - rint(2^t / m) m in
+// one exact step: with 2^t / m <= 2^49 the quotient is off by at most 0.5 + 1.5 2^-3, so
+// |x| <= 0.69 m.  tmax (uniform) = 49 + floor(log2 m) of the smallest m the caller passes.
 template <int W>
-__device__ __forceinline__ double fpow2_split(uint64_t H, uint32_t v, int lead, double m,
+__device__ __forceinline__ double fpow2_split(uint64_t H, uint32_t v, int tmax, double m,
                                               double minv) {
   constexpr uint32_t mask = (1u << W) - 1;
+  // 32-bit compares only: gfx9 has no scalar 64-bit ordered compare, so a 64-bit one would cost
+  // VALU instructions in every term.
   const int bl = 64 - __builtin_clzll(H);
-  const int s = bl > lead ? W * ((bl - lead) / W) : 0;  // H = t 2^s + (s window bits)
+  const int bt = 32 - __builtin_clz((uint32_t)tmax);
+  int s = 0;
+  if (bl > bt || (bl == bt && (uint32_t)H >= (uint32_t)tmax)) {  // s = 0 needs H + 1 <= tmax
+    s = bl > bt + W ? W * ((bl - bt + W - 1) / W) : W;           // now H >> s < 2^bt
+    if ((uint32_t)(H >> s) > (uint32_t)tmax) s += W;
+  }
   const uint32_t t = (uint32_t)(H >> s);
   const int carry = (int)(v >> 10);
   double x;

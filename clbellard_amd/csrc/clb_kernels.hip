@@ -61,7 +61,7 @@ __device__ __forceinline__ void flush_job(U192 v, uint64_t nterms, uint32_t nume
 // per-term count and no choice of reciprocal update.
 template <int W, bool kFast>
 __device__ __forceinline__ void split_terms(const JobDev& jb, uint64_t kbase, int lane, int nt,
-                                            bool newton1, uint64_t E0, int lead, double (&f)[4],
+                                            bool newton1, uint64_t E0, int tmax, double (&f)[4],
                                             uint32_t& corr, uint64_t& nterms) {
   if (kFast) newton1 = true;
   const uint32_t ten_lane = 10u * (uint32_t)lane;
@@ -76,7 +76,7 @@ __device__ __forceinline__ void split_terms(const JobDev& jb, uint64_t kbase, in
       if (kFast || E >= 1664) {
         const uint64_t F = E - 640;
         const uint32_t vs = (uint32_t)(F & 1023) + 640u;  // uniform part of v
-        rem = fpow2_split<W>(F >> 10, vs - ten_lane, lead, m, minv);
+        rem = fpow2_split<W>(F >> 10, vs - ten_lane, tmax, m, minv);
       } else {
         const int bl = 64 - __builtin_clzll(E);
         const int s = bl > 3 ? W * ((bl - 3) / W) : 0;
@@ -111,15 +111,15 @@ __device__ __forceinline__ void tile_split(const JobDev& jb, uint64_t kbase, int
   const bool newton1 = jb.den0 * (kbase + kWave) + jb.den1 >= (1ull << 37);  // m of terms i >= 1
   const uint64_t E0 = 10ull * (jb.d - kbase);                    // exponent of lane 0, term 0
   const bool big = E0 - 10ull * kWave * (uint64_t)(nt - 1) >= 1664;  // every term in 1024 H + v form
-  const int lead = jb.den0 * kbase + jb.den1 >= (1u << 15) ? 4 : 3;  // smallest m of the tile
+  const int tmax = 49 + 63 - __builtin_clzll(jb.den0 * kbase + jb.den1);  // smallest m of the tile
   double f[4] = {0.0, 0.0, 0.0, 0.0};
   uint32_t corr = 0;
   if (full && big && newton1) {
-    split_terms<W, true>(jb, kbase, lane, nt, newton1, E0, lead, f, corr, nterms);
+    split_terms<W, true>(jb, kbase, lane, nt, newton1, E0, tmax, f, corr, nterms);
     nterms += kTermsPerLane;  // every term counted; k = 0 with den1 = 1 is the only m = 1
     if (kbase == 0 && lane == 0 && jb.den1 == 1) nterms -= 1;
   } else {
-    split_terms<W, false>(jb, kbase, lane, nt, newton1, E0, lead, f, corr, nterms);
+    split_terms<W, false>(jb, kbase, lane, nt, newton1, E0, tmax, f, corr, nterms);
   }
   // Digit i has weight 2^(144 - 48 i).  f[0] may be negative (balanced residue); adding 2^52
   // keeps it non-negative and changes the sum by 2^196 = 0 mod 2^192.
