@@ -8,7 +8,7 @@ one k over all seven formulas (seven modular exponentiations).
 
 Also reported, outside the timed steps:
   * the plain all-integer kernel and the split kernel on the same seeded windows (64 windows of
-    2^22 consecutive k per formula, starts from SplitMix64 seed 2), bit-compared;
+    2^26 consecutive k per formula, starts from SplitMix64 seed 2), bit-compared;
   * the same windows summed by the CPU oracle on a bounded sample, bit-compared, and its rate
     (cpu_baseline, kind "port");
   * roofline of the dominant kernel (the head-term kernel) from its HIP-event launch time.
@@ -86,7 +86,9 @@ def bench_batched(clb, args):
         pos.append(5 * round(10 ** (6 + 5 * u) / 5))
     terms = sum(clb.normalize(p)[1] for p in pos)
     ctx = clb.Context(0, clb.PREC_U192)
-    log(f"batched: {len(pos)} positions, {terms:.4g} terms")
+    ctx.set_graph(not args.no_graph)
+    log(f"batched: {len(pos)} positions, {terms:.4g} terms, "
+        f"{'eager launches' if args.no_graph else 'captured HIP graph'}")
     t0 = time.perf_counter()
     fracs = ctx.batch_positions(pos)
     dt = time.perf_counter() - t0
@@ -105,8 +107,49 @@ def bench_batched(clb, args):
         "data": "synthetic",
         "config": {"workload": f"{len(pos)} positions log-uniform in [1e6, 1e11], seed 0, one "
                                "persistent launch", "positions": len(pos), "terms": terms,
-                   "head_kernel_ms": head_ms, "checked_single": len(sample)}}), flush=True)
+                   "graph": not args.no_graph, "head_kernel_ms": head_ms,
+                   "checked_single": len(sample)}}), flush=True)
     ctx.close()
+    return 0
+
+
+def bench_small(clb, args):
+    """Call latency of small positions: back-to-back clb_normal calls at N = 10^5 (d = 39999),
+    10^6 and 10^7, with the tile length chosen per launch (auto) or forced to 16, 4 or 1 terms
+    per lane, and auto with the captured-graph replay; every mode's digits compared."""
+    modes = [("auto", 0, False), ("tpl16", 16, False), ("tpl4", 4, False), ("tpl1", 1, False),
+             ("auto+graph", 0, True)]
+    table = {}
+    for n in (100000, 1000000, 10000000):
+        calls = max(5, args.calls * 100000 // n)
+        ref = None
+        row = {}
+        for name, tpl, graph in modes:
+            ctx = clb.Context(0, clb.PREC_U192)
+            ctx.set_tiling(tpl)
+            ctx.set_graph(graph)
+            ctx.normal(n)  # warm-up (and capture)
+            t0 = time.perf_counter()
+            for _ in range(calls):
+                frac = ctx.normal(n)[0]
+            dt = time.perf_counter() - t0
+            ctx.close()
+            if ref is None:
+                ref = frac
+            elif frac != ref:
+                raise SystemExit(f"N={n} {name}: digits differ from the first mode")
+            row[name] = round(1e6 * dt / calls, 1)
+        table[str(n)] = row
+        log(f"N={n}: " + ", ".join(f"{k} {v} us" for k, v in row.items()))
+    best = table["100000"]["auto"]
+    print(json.dumps({
+        "metric": "clb_normal call latency at N=1e5", "value": best, "unit": "us/call",
+        "n_gpus": 1, "steps": args.calls, "warmup": 1, "ms_per_step": best / 1e3,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "back-to-back normal-mode calls at N=1e5, 1e6, 1e7 per tile "
+                               "length and launch mode (us per call)", "table": table}}),
+          flush=True)
     return 0
 
 
@@ -161,12 +204,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--position", type=int, default=POSITION)
     ap.add_argument("--windows", type=int, default=64)
-    ap.add_argument("--window-log2", type=int, default=22)
+    ap.add_argument("--window-log2", type=int, default=26)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["normal", "batched", "worklist"], default="normal",
+    ap.add_argument("--workload", choices=["normal", "batched", "worklist", "small"],
+                    default="normal",
                     help="normal: the metric (N = 10^12); batched: many positions in one launch; "
-                         "worklist: uneven term-range jobs written as work files and summed")
+                         "worklist: uneven term-range jobs written as work files and summed; "
+                         "small: call latency of small positions, eager vs captured graph")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched: launch the kernels one by one instead of a captured graph")
+    ap.add_argument("--calls", type=int, default=200, help="small: calls per mode")
     ap.add_argument("--positions", type=int, default=4096, help="batched: number of positions")
     ap.add_argument("--check", type=int, default=8,
                     help="batched: positions re-run alone and compared bit for bit")
@@ -195,6 +243,8 @@ def main():
         return bench_batched(clb, args)
     if args.workload == "worklist":
         return bench_worklist(clb, args)
+    if args.workload == "small":
+        return bench_small(clb, args)
 
     n_pos, d, _ = clb.normalize(args.position)
     ctx = clb.Context(0, clb.PREC_U192, kernel=clb.KERNEL_SPLIT)

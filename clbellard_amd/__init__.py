@@ -73,6 +73,8 @@ _SIGNATURES = {
     "clb_ctx_destroy": ([_CTX], ctypes.c_int),
     "clb_set_kernel": ([_CTX, ctypes.c_int], ctypes.c_int),
     "clb_set_launch": ([_CTX, ctypes.c_int], ctypes.c_int),
+    "clb_set_graph": ([_CTX, ctypes.c_int], ctypes.c_int),
+    "clb_set_tiling": ([_CTX, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "clb_normalize": ([_U64, _P64, _P64, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "clb_formula": ([ctypes.c_int, _P64, ctypes.POINTER(ctypes.c_int64), _P64, _P64],
                     ctypes.c_int),
@@ -257,6 +259,15 @@ class Context:
 
     def set_launch(self, blocks_per_cu: int) -> None:
         _check(lib.clb_set_launch(self._ctx, blocks_per_cu))
+
+    def set_tiling(self, terms_per_lane: int = 0, min_tiles_per_wave: int = 0) -> None:
+        """Terms per lane per tile (16, 4, 1; 0 = chosen per launch by min_tiles_per_wave tiles
+        per resident wave, 0 = always 16)."""
+        _check(lib.clb_set_tiling(self._ctx, terms_per_lane, min_tiles_per_wave))
+
+    def set_graph(self, enable: bool) -> None:
+        """Replay each call's kernel sequence as a captured HIP graph (identical results)."""
+        _check(lib.clb_set_graph(self._ctx, 1 if enable else 0))
 
     # --- KernelFuncs.as surface
     def set0mem(self) -> None:

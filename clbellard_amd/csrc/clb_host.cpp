@@ -64,6 +64,8 @@ struct HostJob {
   bool tail;
 };
 
+// Device buffer that grows geometrically, so that its address (which a captured graph records)
+// changes rarely.
 template <class T>
 struct DevBuf {
   T* p = nullptr;
@@ -72,8 +74,9 @@ struct DevBuf {
     if (n <= cap) return CLB_OK;
     if (p) (void)hipFree(p);
     p = nullptr;
-    cap = 0;
     size_t want = n < 64 ? 64 : n;
+    if (want < 2 * cap) want = 2 * cap;
+    cap = 0;
     hipError_t e = hipMalloc((void**)&p, want * sizeof(T));
     if (e != hipSuccess) return fail(CLB_ENOMEM, "hipMalloc(%zu bytes): %s", want * sizeof(T),
                                      hipGetErrorString(e));
@@ -82,6 +85,30 @@ struct DevBuf {
   }
   void release() {
     if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// Pinned host staging (hipHostMalloc), so that the per-call copies are single DMA transfers.
+struct HostBuf {
+  uint8_t* p = nullptr;
+  size_t cap = 0;
+  int reserve(size_t n) {
+    if (n <= cap) return CLB_OK;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    size_t want = n < 4096 ? 4096 : n;
+    if (want < 2 * cap) want = 2 * cap;
+    cap = 0;
+    hipError_t e = hipHostMalloc((void**)&p, want, hipHostMallocDefault);
+    if (e != hipSuccess)
+      return fail(CLB_ENOMEM, "hipHostMalloc(%zu bytes): %s", want, hipGetErrorString(e));
+    cap = want;
+    return CLB_OK;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
     p = nullptr;
     cap = 0;
   }
@@ -97,18 +124,27 @@ struct clb_ctx {
   int num_cus = 0;
   char device_name[256] = {0};
   hipStream_t stream = nullptr;
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   uint64_t* partial = nullptr;  // bigSumMem[0..2] (most significant word first)
   uint64_t* ans = nullptr;      // dev_ans[0..2]
-  DevBuf<JobDev> jobs;
-  DevBuf<TailDev> tails;
-  DevBuf<GroupDev> groups;
-  DevBuf<uint8_t> shifts;
+  DevBuf<uint8_t> block;        // the call's table block: LaunchDev header + tables
+  HostBuf block_host;           // its pinned staging copy
   DevBuf<unsigned long long> acc;
   DevBuf<uint64_t> out;
   DevBuf<uint64_t> slot_out;
+  HostBuf result_host;          // pinned landing area of out (+ slot_out)
   float head_ms = 0.f, total_ms = 0.f;
-  int resident[5] = {0, 0, 0, 0, 0};  // resident k_head workgroups per CU, per kernel kind
+  int resident[5][kNumTpl] = {};  // resident k_head workgroups per CU, per kind and tile length
+  int tpl_index = -1;            // index into kTplChoices, or -1: chosen per launch
+  int min_tiles_per_wave = 0;    // launch-size threshold of that choice (0: always the longest)
+  // Graph mode: the kernel sequence of a call, captured once per buffer layout and kernel choice
+  // (every argument the kernels see), replayed afterwards.  Counts travel in hdr.
+  bool use_graph = false;
+  struct Graph {
+    uintptr_t key[8];
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
 };
 
 namespace {
@@ -134,6 +170,55 @@ int choose_kind(int pref, uint64_t max_m, int* kind) {
   return CLB_OK;
 }
 
+// Persistent grid: every workgroup resident at once (one wave of blocks), unless overridden.
+int blocks_per_cu(clb_ctx* ctx, int kind, int tpl) {
+  if (ctx->blocks_per_cu > 0) return ctx->blocks_per_cu;
+  if (ctx->resident[kind][tpl] == 0) {
+    const int r = head_blocks_per_cu(kind, tpl);
+    ctx->resident[kind][tpl] = r > 0 ? r : 8;
+  }
+  return ctx->resident[kind][tpl];
+}
+
+// The captured launch sequence (clear, head, tail, finalize) for this call's kernel arguments,
+// captured on first use.  Kernels read counts and tables from the block, so a graph stays valid
+// while the buffers it names are not reallocated; at most 8 are kept.
+int graph_for(clb_ctx* ctx, int kind, int tpl, int grid, uint64_t* dst, bool accumulate,
+              uint64_t* slot_dev, hipGraphExec_t* exec) {
+  LaunchDev* hdr = reinterpret_cast<LaunchDev*>(ctx->block.p);
+  const uintptr_t key[8] = {(uintptr_t)kind,       (uintptr_t)tpl, (uintptr_t)grid,
+                            (uintptr_t)hdr,        (uintptr_t)ctx->acc.p, (uintptr_t)dst,
+                            (uintptr_t)accumulate, (uintptr_t)slot_dev};
+  for (const auto& g : ctx->graphs)
+    if (memcmp(g.key, key, sizeof key) == 0) {
+      *exec = g.exec;
+      return CLB_OK;
+    }
+  hipStream_t s = ctx->stream;
+  HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  hipError_t e = launch_clear(ctx->acc.p, hdr, s);
+  if (e == hipSuccess) e = launch_head(kind, tpl, hdr, ctx->acc.p, grid, s);
+  if (e == hipSuccess) e = launch_tail(hdr, ctx->acc.p, s);
+  if (e == hipSuccess) e = launch_finalize(ctx->acc.p, hdr, dst, accumulate, slot_dev, s);
+  hipGraph_t graph = nullptr;
+  const hipError_t e_end = hipStreamEndCapture(s, &graph);
+  if (e == hipSuccess) e = e_end;
+  hipGraphExec_t ex = nullptr;
+  if (e == hipSuccess) e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(CLB_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+  if (ctx->graphs.size() >= 8) {
+    (void)hipGraphExecDestroy(ctx->graphs.front().exec);
+    ctx->graphs.erase(ctx->graphs.begin());
+  }
+  clb_ctx::Graph g;
+  memcpy(g.key, key, sizeof key);
+  g.exec = ex;
+  ctx->graphs.push_back(g);
+  *exec = ex;
+  return CLB_OK;
+}
+
 // Runs head jobs + tails into slots [0, nslots), then combines the slots by groups.
 // out_host receives 3 words per group unless accumulate_dev is given, in which case the single
 // group's value is added into that device buffer.  slot_host (optional) receives each slot's
@@ -146,86 +231,143 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
   const size_t nslots = shift.size();
   std::vector<JobDev> jobs;
   std::vector<TailDev> tails;
-  uint64_t flat = 0, max_m = 1;
+  uint64_t max_m = 1;
+  size_t nhead = 0;
   for (const HostJob& h : hjobs) {
     if (h.tail) tails.push_back({h.d, h.den0, h.den1, h.numesign, h.slot});
+    if (h.k0 >= h.k1) continue;
+    const uint64_t m = h.den0 * (h.k1 - 1) + h.den1;
+    if (m > max_m) max_m = m;
+    ++nhead;
+  }
+  int kind = kSplitW3;
+  if (nhead && (rc = choose_kind(ctx->kernel, max_m, &kind))) return rc;
+  if (nhead > 0x7fffffff || tails.size() > 0x7fffffff || groups.size() > 0x7fffffff)
+    return fail(CLB_EINVAL, "too many jobs");
+  // Terms per lane and grid: the longest tiles that still give every resident wave
+  // ctx->min_tiles_per_wave tiles to take (small launches spread over more, shorter tiles).
+  int tpl = ctx->tpl_index;
+  uint64_t grid = 0, flat = 0;
+  for (int t = (tpl >= 0 ? tpl : 0); t < kNumTpl; ++t) {
+    grid = (uint64_t)ctx->num_cus * (uint64_t)blocks_per_cu(ctx, kind, t);
+    const uint64_t tk = tile_k(kTplChoices[t]);
+    flat = 0;
+    for (const HostJob& h : hjobs)
+      if (h.k0 < h.k1) flat += (h.k1 + tk - 1) / tk - h.k0 / tk;
+    tpl = t;
+    if (ctx->tpl_index >= 0 ||
+        flat >= (uint64_t)ctx->min_tiles_per_wave * grid * (uint64_t)(kBlock / kWave))
+      break;
+  }
+  const uint64_t tk = tile_k(kTplChoices[tpl]);
+  flat = 0;
+  for (const HostJob& h : hjobs) {
     if (h.k0 >= h.k1) continue;
     JobDev j{};
     j.d = h.d;
     j.k0 = h.k0;
     j.k1 = h.k1;
-    j.tile_first = h.k0 / kTileK;
+    j.tile_first = h.k0 / tk;
     j.flat_begin = flat;
     j.den0 = h.den0;
     j.den1 = h.den1;
     j.numesign = h.numesign;
     j.slot = h.slot;
-    flat += (h.k1 + kTileK - 1) / kTileK - j.tile_first;
-    const uint64_t m = h.den0 * (h.k1 - 1) + h.den1;
-    if (m > max_m) max_m = m;
+    flat += (h.k1 + tk - 1) / tk - j.tile_first;
     jobs.push_back(j);
   }
-  int kind = kSplitW3;
-  if (!jobs.empty() && (rc = choose_kind(ctx->kernel, max_m, &kind))) return rc;
-  if (jobs.size() > 0x7fffffff || tails.size() > 0x7fffffff || groups.size() > 0x7fffffff)
-    return fail(CLB_EINVAL, "too many jobs");
-  if ((rc = ctx->jobs.reserve(jobs.size() + 1)) || (rc = ctx->tails.reserve(tails.size() + 1)) ||
-      (rc = ctx->groups.reserve(groups.size())) || (rc = ctx->shifts.reserve(nslots)) ||
-      (rc = ctx->acc.reserve((nslots + 1) * kAccWords)) || (rc = ctx->out.reserve(3 * groups.size())) ||
-      (rc = ctx->slot_out.reserve(3 * nslots)))
+  // The table block: header, jobs, tails, groups, shifts, each 64-byte aligned.
+  auto align64 = [](size_t x) { return (x + 63) & ~(size_t)63; };
+  const size_t off_jobs = align64(sizeof(LaunchDev));
+  const size_t off_tails = off_jobs + align64(jobs.size() * sizeof(JobDev));
+  const size_t off_groups = off_tails + align64(tails.size() * sizeof(TailDev));
+  const size_t off_shifts = off_groups + align64(groups.size() * sizeof(GroupDev));
+  const size_t bytes = off_shifts + nslots;
+  if (bytes > 0xffffffffull) return fail(CLB_EINVAL, "too many jobs for one call");
+  const size_t res_words = 3 * groups.size() + (slot_host ? 3 * nslots : 0);
+  if ((rc = ctx->block.reserve(bytes)) || (rc = ctx->block_host.reserve(bytes)) ||
+      (rc = ctx->out.reserve(3 * groups.size())) ||
+      (rc = ctx->slot_out.reserve(3 * nslots)) ||
+      (rc = ctx->result_host.reserve(res_words * sizeof(uint64_t))))
     return rc;
+  // Tile schedule.  Up to 64 tiles per wave: each wave takes one static range of consecutive
+  // tiles (no atomics; atomics on one address serialise at ~35 ns each).  Larger launches: all
+  // tiles from the queue in chunks, at least 8 per wave and at most 256 tiles (~15 ms of one
+  // wave's work at N = 10^12, the granularity of the final balance).
+  const uint64_t nwaves = grid * (uint64_t)(kBlock / kWave);
+  uint64_t static_tiles = 0, chunk = 1;
+  if (flat <= 64 * nwaves) {
+    static_tiles = (flat + nwaves - 1) / nwaves;
+  } else {
+    const uint64_t per = flat / (8 * nwaves);
+    chunk = per < 1 ? 1 : per > 256 ? 256 : per;
+  }
+  // Copies per accumulator slot: enough that the waves' flushes spread over about nwaves / 16
+  // addresses per limb (a power of 2, at most 64), so short launches do not queue on one slot.
+  uint32_t replicas = 1;
+  while (replicas < 64 && (uint64_t)replicas * 2 * nslots * 16 <= nwaves) replicas *= 2;
+  if ((rc = ctx->acc.reserve(nslots * replicas * kAccWords))) return rc;
+  LaunchDev h{};
+  h.counter = 0;
+  h.total_tiles = flat;
+  h.nzero = nslots * replicas * kAccWords;
+  h.replicas = replicas;
+  h.nslots = (uint32_t)nslots;
+  h.static_tiles = static_tiles;
+  h.njobs = (uint32_t)jobs.size();
+  h.chunk = (uint32_t)chunk;
+  h.ntails = (uint32_t)tails.size();
+  h.ngroups = (uint32_t)groups.size();
+  h.off_jobs = (uint32_t)off_jobs;
+  h.off_tails = (uint32_t)off_tails;
+  h.off_groups = (uint32_t)off_groups;
+  h.off_shifts = (uint32_t)off_shifts;
+  uint8_t* stage = ctx->block_host.p;
+  memcpy(stage, &h, sizeof h);
+  if (!jobs.empty()) memcpy(stage + off_jobs, jobs.data(), jobs.size() * sizeof(JobDev));
+  if (!tails.empty()) memcpy(stage + off_tails, tails.data(), tails.size() * sizeof(TailDev));
+  if (!groups.empty()) memcpy(stage + off_groups, groups.data(), groups.size() * sizeof(GroupDev));
+  if (nslots) memcpy(stage + off_shifts, shift.data(), nslots);
+  LaunchDev* hdr = reinterpret_cast<LaunchDev*>(ctx->block.p);
+  uint64_t* dst = accumulate_dev ? accumulate_dev : ctx->out.p;
+  uint64_t* slot_dev = slot_host ? ctx->slot_out.p : nullptr;
   hipStream_t s = ctx->stream;
   HIP_TRY(hipEventRecord(ctx->ev[0], s));
-  if (!jobs.empty())
-    HIP_TRY(hipMemcpyAsync(ctx->jobs.p, jobs.data(), jobs.size() * sizeof(JobDev),
-                           hipMemcpyHostToDevice, s));
-  if (!tails.empty())
-    HIP_TRY(hipMemcpyAsync(ctx->tails.p, tails.data(), tails.size() * sizeof(TailDev),
-                           hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(ctx->groups.p, groups.data(), groups.size() * sizeof(GroupDev),
-                         hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(ctx->shifts.p, shift.data(), nslots, hipMemcpyHostToDevice, s));
-  HIP_TRY(launch_zero(ctx->acc.p, (nslots + 1) * kAccWords, s));  // slots + the tile counter
-  // persistent grid: every workgroup resident at once (one wave of blocks), unless overridden
-  int bpc = ctx->blocks_per_cu;
-  if (bpc <= 0) {
-    if (ctx->resident[kind] == 0) {
-      const int r = head_blocks_per_cu(kind);
-      ctx->resident[kind] = r > 0 ? r : 8;
-    }
-    bpc = ctx->resident[kind];
+  HIP_TRY(hipMemcpyAsync(ctx->block.p, stage, bytes, hipMemcpyHostToDevice, s));
+  if (ctx->use_graph) {
+    hipGraphExec_t exec = nullptr;
+    if ((rc = graph_for(ctx, kind, tpl, (int)grid, dst, accumulate_dev != nullptr, slot_dev,
+                        &exec)))
+      return rc;
+    HIP_TRY(hipEventRecord(ctx->ev[1], s));
+    HIP_TRY(hipGraphLaunch(exec, s));
+    HIP_TRY(hipEventRecord(ctx->ev[2], s));
+  } else {
+    HIP_TRY(launch_clear(ctx->acc.p, hdr, s));
+    HIP_TRY(hipEventRecord(ctx->ev[1], s));
+    HIP_TRY(launch_head(kind, tpl, hdr, ctx->acc.p, (int)grid, s));
+    HIP_TRY(hipEventRecord(ctx->ev[2], s));
+    HIP_TRY(launch_tail(hdr, ctx->acc.p, s));
+    HIP_TRY(launch_finalize(ctx->acc.p, hdr, dst, accumulate_dev != nullptr, slot_dev, s));
   }
-  HIP_TRY(hipEventRecord(ctx->ev[1], s));
-  const uint64_t grid = (uint64_t)ctx->num_cus * (uint64_t)bpc;
-  // Chunks of up to 256 tiles (~18 ms of one wave's work at N = 10^12), at least 8 chunks per
-  // wave.  Device-scope atomics go past the per-XCD L2, so fewer chunks keep the counter's
-  // memory transactions negligible.
-  const uint64_t per_wave = flat / (grid * (kBlock / kWave) * 8 + 1);
-  const uint32_t chunk = (uint32_t)(per_wave < 1 ? 1 : per_wave > 256 ? 256 : per_wave);
-  HIP_TRY(launch_head(kind, ctx->jobs.p, (int)jobs.size(), flat, chunk,
-                      ctx->acc.p + nslots * kAccWords, ctx->acc.p, (int)grid, s));
-  HIP_TRY(hipEventRecord(ctx->ev[2], s));
-  HIP_TRY(launch_tail(ctx->tails.p, (int)tails.size(), ctx->acc.p, s));
-  uint64_t* dst = accumulate_dev ? accumulate_dev : ctx->out.p;
-  HIP_TRY(launch_finalize(ctx->acc.p, ctx->shifts.p, (int)nslots, ctx->groups.p,
-                          (int)groups.size(), dst, accumulate_dev != nullptr,
-                          slot_host ? ctx->slot_out.p : nullptr, s));
-  if (out_host && !accumulate_dev)
-    HIP_TRY(hipMemcpyAsync(out_host, ctx->out.p, 3 * groups.size() * sizeof(uint64_t),
+  uint64_t* res = reinterpret_cast<uint64_t*>(ctx->result_host.p);
+  if (out_host && !accumulate_dev && !groups.empty())
+    HIP_TRY(hipMemcpyAsync(res, ctx->out.p, 3 * groups.size() * sizeof(uint64_t),
                            hipMemcpyDeviceToHost, s));
-  if (slot_host)
-    HIP_TRY(hipMemcpyAsync(slot_host, ctx->slot_out.p, 3 * nslots * sizeof(uint64_t),
-                           hipMemcpyDeviceToHost, s));
-  hipEvent_t end;
-  HIP_TRY(hipEventCreate(&end));
-  hipError_t e1 = hipEventRecord(end, s);
-  hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(s) : e1;
+  if (slot_host && nslots)
+    HIP_TRY(hipMemcpyAsync(res + 3 * groups.size(), ctx->slot_out.p,
+                           3 * nslots * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  hipError_t e2 = hipEventRecord(ctx->ev[3], s);
+  if (e2 == hipSuccess) e2 = hipStreamSynchronize(s);
   float head = 0.f, total = 0.f;
   if (e2 == hipSuccess) e2 = hipEventElapsedTime(&head, ctx->ev[1], ctx->ev[2]);
-  if (e2 == hipSuccess) e2 = hipEventElapsedTime(&total, ctx->ev[0], end);
-  (void)hipEventDestroy(end);
+  if (e2 == hipSuccess) e2 = hipEventElapsedTime(&total, ctx->ev[0], ctx->ev[3]);
   if (e2 != hipSuccess)
     return fail(CLB_EDEVICE, "term launch failed: %s", hipGetErrorString(e2));
+  if (out_host && !accumulate_dev && !groups.empty())
+    memcpy(out_host, res, 3 * groups.size() * sizeof(uint64_t));
+  if (slot_host && nslots)
+    memcpy(slot_host, res + 3 * groups.size(), 3 * nslots * sizeof(uint64_t));
   ctx->head_ms = head;
   ctx->total_ms = total;
   return CLB_OK;
@@ -570,13 +712,14 @@ int clb_ctx_destroy(clb_ctx* ctx) {
   if (!ctx) return CLB_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  ctx->jobs.release();
-  ctx->tails.release();
-  ctx->groups.release();
-  ctx->shifts.release();
+  ctx->block.release();
+  ctx->block_host.release();
   ctx->acc.release();
   ctx->out.release();
   ctx->slot_out.release();
+  ctx->result_host.release();
+  for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
+  ctx->graphs.clear();
   if (ctx->partial) (void)hipFree(ctx->partial);
   if (ctx->ans) (void)hipFree(ctx->ans);
   for (auto& ev : ctx->ev)
@@ -591,6 +734,27 @@ int clb_set_kernel(clb_ctx* ctx, int kernel) {
   if (kernel < CLB_KERNEL_AUTO || kernel > CLB_KERNEL_SPLIT)
     return fail(CLB_EINVAL, "kernel must be 0 (auto), 1 (plain) or 2 (split)");
   ctx->kernel = kernel;
+  return CLB_OK;
+}
+
+int clb_set_tiling(clb_ctx* ctx, int terms_per_lane, int min_tiles_per_wave) {
+  if (int rc = check_ctx(ctx)) return rc;
+  int idx = -1;
+  for (int t = 0; t < kNumTpl; ++t)
+    if (kTplChoices[t] == terms_per_lane) idx = t;
+  if (terms_per_lane != 0 && idx < 0)
+    return fail(CLB_EINVAL, "terms per lane must be 0, 16, 4 or 1");
+  if (min_tiles_per_wave < 0 || min_tiles_per_wave > 1024)
+    return fail(CLB_EINVAL, "min_tiles_per_wave 0..1024");
+  ctx->tpl_index = idx;
+  ctx->min_tiles_per_wave = min_tiles_per_wave;
+  return CLB_OK;
+}
+
+int clb_set_graph(clb_ctx* ctx, int enable) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (enable != 0 && enable != 1) return fail(CLB_EINVAL, "graph mode must be 0 or 1");
+  ctx->use_graph = enable != 0;
   return CLB_OK;
 }
 

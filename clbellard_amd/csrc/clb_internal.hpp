@@ -7,8 +7,13 @@
 namespace clb {
 
 constexpr int kWave = 64;
-constexpr int kTermsPerLane = 16;                          // terms per lane per tile
-constexpr uint64_t kTileK = (uint64_t)kWave * kTermsPerLane;  // consecutive k per tile (even)
+// Terms per lane per tile (TPL): a tile is kWave * TPL consecutive k, lane L taking k = base + L +
+// 64 i, i < TPL.  16 is the most the fp64 engine's digit sums hold exactly; launches too small to
+// fill the GPU with 16-term tiles use 4 or 1 so that more waves run side by side.
+constexpr int kMaxTermsPerLane = 16;
+constexpr int kNumTpl = 3;
+constexpr int kTplChoices[kNumTpl] = {16, 4, 1};
+__host__ __device__ constexpr uint64_t tile_k(int tpl) { return (uint64_t)kWave * (uint64_t)tpl; }
 constexpr int kBlock = 256;                                // threads per workgroup
 constexpr int kAccWords = 8;                               // u64 words per accumulator slot
 constexpr int kTailTerms = 20;                             // k = d .. d+19 (Bellard.cl:1164-1239)
@@ -22,12 +27,13 @@ enum Kind : int {
   kPlainWide = 4 // int engine, reduced residues, 1-bit windows: m < 2^63
 };
 
-// One head-term job in the flat tile space.  Tiles are aligned to absolute multiples of kTileK,
-// so lane L of every tile handles k = L (mod 64) and keeps one sign for the whole job.
+// One head-term job in the flat tile space.  Tiles are aligned to absolute multiples of the tile
+// size (a multiple of 64), so lane L of every tile handles k = L (mod 64) and keeps one sign for
+// the whole job.
 struct JobDev {
   uint64_t d;           // loop count
   uint64_t k0, k1;      // head range [k0, k1), k1 <= d, k0 < k1
-  uint64_t tile_first;  // k0 / kTileK
+  uint64_t tile_first;  // k0 / tile size
   uint64_t flat_begin;  // first flat tile index of the job (prefix sum, ascending)
   uint64_t den0, den1;  // modulus m = den0 * k + den1
   uint32_t numesign;    // 1: the term of even k is negative (KernelFuncs.as:10)
@@ -47,23 +53,59 @@ struct GroupDev {
   uint32_t count;
 };
 
+// Per-call header at offset 0 of the call's table block (header, job table, tail table, groups,
+// shifts, packed at the byte offsets below).  The kernels read counts and table offsets from it,
+// so one captured HIP graph serves every call while the block's address is unchanged.  The host
+// copies the block before each launch sequence, which also resets the tile queue.
+struct LaunchDev {
+  unsigned long long counter;  // tile queue: tiles handed out in the dynamic phase
+  uint64_t total_tiles;        // flat tiles of all jobs
+  uint64_t nzero;              // accumulator words to clear
+  uint64_t static_tiles;       // static phase: wave w takes tiles [w T, (w + 1) T), T = this
+  uint32_t njobs;              // entries of the job table
+  uint32_t chunk;              // consecutive tiles per queue grab in the dynamic phase
+  uint32_t ntails;             // entries of the tail table
+  uint32_t ngroups;            // results (groups of slots)
+  uint32_t replicas;           // copies per accumulator slot (a power of 2, at most 64)
+  uint32_t nslots;             // accumulator slots
+  uint32_t off_jobs, off_tails, off_groups, off_shifts;  // byte offsets in the block
+};
+
+__host__ __device__ inline const JobDev* hdr_jobs(const LaunchDev* h) {
+  return reinterpret_cast<const JobDev*>(reinterpret_cast<const uint8_t*>(h) + h->off_jobs);
+}
+__host__ __device__ inline const TailDev* hdr_tails(const LaunchDev* h) {
+  return reinterpret_cast<const TailDev*>(reinterpret_cast<const uint8_t*>(h) + h->off_tails);
+}
+__host__ __device__ inline const GroupDev* hdr_groups(const LaunchDev* h) {
+  return reinterpret_cast<const GroupDev*>(reinterpret_cast<const uint8_t*>(h) + h->off_groups);
+}
+__host__ __device__ inline const uint8_t* hdr_shifts(const LaunchDev* h) {
+  return reinterpret_cast<const uint8_t*>(h) + h->off_shifts;
+}
+
+constexpr int kTailBlocks = 128;     // k_tail grid (one wave per block, grid-stride over tails)
+constexpr int kFinalizeBlocks = 64;  // k_finalize grid (64 threads per block, grid-stride)
+constexpr int kClearBlocks = 256;    // k_clear grid (256 threads per block, grid-stride)
+constexpr int kFoldBlocks = 256;     // k_fold grid (one wave per block, grid-stride over slots)
+
 // Workgroups of k_head<kind> that fit on one compute unit at once (0 if unknown).
-int head_blocks_per_cu(int kind);
+int head_blocks_per_cu(int kind, int tpl_index);
 
 // Kernel launch wrappers (clb_kernels.hip).  All asynchronous on `stream`.
 // counter: one zeroed u64 the waves take chunks of `chunk` tiles from.
-hipError_t launch_head(int kind, const JobDev* jobs, int njobs, uint64_t total_tiles,
-                       uint32_t chunk, unsigned long long* counter, unsigned long long* acc,
+// The launch sequence of one call: clear the accumulator words, the head terms (persistent grid of
+// `grid` workgroups), the tails, the combine (replicas folded, then the groups) into out (or
+// added into it when accumulate).
+// slot_out (optional) receives every slot's shifted value.  hdr is the device copy of the call's
+// table block.
+hipError_t launch_clear(unsigned long long* acc, const LaunchDev* hdr, hipStream_t stream);
+hipError_t launch_head(int kind, int tpl_index, LaunchDev* hdr, unsigned long long* acc,
                        int grid, hipStream_t stream);
-hipError_t launch_tail(const TailDev* tails, int ntails, unsigned long long* acc,
-                       hipStream_t stream);
+hipError_t launch_tail(const LaunchDev* hdr, unsigned long long* acc, hipStream_t stream);
+hipError_t launch_finalize(unsigned long long* acc, const LaunchDev* hdr, uint64_t* out,
+                           int accumulate, uint64_t* slot_out, hipStream_t stream);
 hipError_t launch_zero(unsigned long long* p, uint64_t nwords, hipStream_t stream);
-// out[g] (3 words, most significant first) = sum of shifted slots of group g; accumulate adds
-// into out instead of overwriting.  slot_out (may be null) receives each shifted slot value.
-hipError_t launch_finalize(const unsigned long long* acc, const uint8_t* shift, int nslots,
-                           const GroupDev* groups, int ngroups, uint64_t* out, int accumulate,
-                           uint64_t* slot_out, hipStream_t stream);
-// Sum7_192 (Bellard.cl:1359-1398): partial <<= shift (written back), ans += partial.
 hipError_t launch_sum7(uint64_t* ans, uint64_t* partial, int shift, hipStream_t stream);
 
 }  // namespace clb

@@ -1,15 +1,15 @@
 // clb_kernels.hip -- HIP kernels for gfx950: Bellard head terms, tail terms and the combine.
 //
 // Work decomposition.  A launch holds jobs (formula j, loop count d, head range [k0, k1)).  Each
-// job's range is cut into tiles of kTileK consecutive k aligned to multiples of kTileK; the tiles
-// of all jobs form one flat index space that a persistent grid consumes in chunks of consecutive
-// tiles taken from a global counter.  Lane L of a tile evaluates k = base + L + 64 i, i < kTermsPerLane, so every
-// lane sees one parity of k and hence one sign (KernelFuncs.as:10 and the (numesign + k) % 2 rule
-// of Bellard_div2by1.cl:335).  Each lane keeps the 192-bit sum Q of its terms; when the wave moves
-// to another job (and at the end) the lane applies its sign (-Q - n for n negative terms,
-// Bellard_div2by1.cl:335-341), the wave sums its 64 lanes with cross-lane shuffles, and lane 0
-// adds the six 32-bit limbs into the job's accumulator slot with 64-bit atomics.  Integer sums
-// are exact, so the result does not depend on the schedule.
+// job's range is cut into tiles of 64 * TPL consecutive k aligned to multiples of the tile size;
+// the tiles of all jobs form one flat index space that a persistent grid consumes in chunks of
+// consecutive tiles taken from a global counter.  Lane L of a tile evaluates k = base + L + 64 i,
+// i < TPL, so every lane sees one parity of k and hence one sign (KernelFuncs.as:10 and the
+// (numesign + k) % 2 rule of Bellard_div2by1.cl:335).  Each lane keeps the 192-bit sum Q of its
+// terms; when the wave moves to another job (and at the end) the lane applies its sign (-Q - n for
+// n negative terms, Bellard_div2by1.cl:335-341), the wave sums its 64 lanes with cross-lane
+// shuffles, and lane 0 adds the six 32-bit limbs into the job's accumulator slot with 64-bit
+// atomics.  Integer sums are exact, so the result does not depend on the schedule.
 #include "clb_device.hpp"
 #include "clb_internal.hpp"
 
@@ -51,8 +51,8 @@ __device__ __forceinline__ void flush_job(U192 v, uint64_t nterms, uint32_t nume
 
 // ------------------------------------------------------------------ per-tile term evaluation
 
-// fp64 engine.  Per lane per tile at most kTermsPerLane = 16 terms, so the four digit sums stay
-// exact integers below 2^53 and are converted to the lane's 192-bit sum once per tile.
+// fp64 engine.  Per lane per tile at most TPL <= 16 terms, so the four digit sums stay exact
+// integers below 2^53 and are converted to the lane's 192-bit sum once per tile.
 // Lane L's exponent is E - 10 L with E the exponent of lane 0 (uniform): for E >= 1664 it is
 // evaluated as 2^(1024 H + v) with H = (E - 640) >> 10 uniform and v = ((E - 640) & 1023) + 640
 // - 10 L in [10, 1663] per lane (fpow2_split); smaller exponents take per-lane windows.
@@ -101,13 +101,14 @@ __device__ __forceinline__ void split_terms(const JobDev& jb, uint64_t kbase, in
   }
 }
 
-template <int W>
+template <int W, int TPL>
 __device__ __forceinline__ void tile_split(const JobDev& jb, uint64_t kbase, int lane, U192& acc,
                                            uint64_t& nterms) {
+  static_assert(TPL >= 1 && TPL <= kMaxTermsPerLane, "digit sums must stay below 2^53");
   // Per-tile uniform facts, so that the term loop compares only 32-bit scalars:
   const uint64_t left = (jb.k1 - kbase + kWave - 1) / kWave;  // terms of lane 0 before k1
-  const int nt = left < (uint64_t)kTermsPerLane ? (int)left : kTermsPerLane;
-  const bool full = kbase >= jb.k0 && kbase + kTileK <= jb.k1;  // every lane, every term valid
+  const int nt = left < (uint64_t)TPL ? (int)left : TPL;
+  const bool full = kbase >= jb.k0 && kbase + tile_k(TPL) <= jb.k1;  // every lane and term valid
   const bool newton1 = jb.den0 * (kbase + kWave) + jb.den1 >= (1ull << 37);  // m of terms i >= 1
   const uint64_t E0 = 10ull * (jb.d - kbase);                    // exponent of lane 0, term 0
   const bool big = E0 - 10ull * kWave * (uint64_t)(nt - 1) >= 1664;  // every term in 1024 H + v form
@@ -116,7 +117,7 @@ __device__ __forceinline__ void tile_split(const JobDev& jb, uint64_t kbase, int
   uint32_t corr = 0;
   if (full && big && newton1) {
     split_terms<W, true>(jb, kbase, lane, nt, newton1, E0, tmax, f, corr, nterms);
-    nterms += kTermsPerLane;  // every term counted; k = 0 with den1 = 1 is the only m = 1
+    nterms += TPL;  // every term counted; k = 0 with den1 = 1 is the only m = 1
     if (kbase == 0 && lane == 0 && jb.den1 == 1) nterms -= 1;
   } else {
     split_terms<W, false>(jb, kbase, lane, nt, newton1, E0, tmax, f, corr, nterms);
@@ -134,14 +135,14 @@ __device__ __forceinline__ void tile_split(const JobDev& jb, uint64_t kbase, int
 }
 
 // int engine: Montgomery form of 2^(e+192), then three exact-division quotient words.
-template <bool kWide>
+template <bool kWide, int TPL>
 __device__ __forceinline__ void tile_plain(const JobDev& jb, uint64_t kbase, int lane, U192& acc,
                                            uint64_t& nterms) {
   uint64_t k = kbase + (uint64_t)lane;
   uint64_t m = jb.den0 * k + jb.den1;
   uint64_t e = 10ull * (jb.d - k);
   const uint64_t mstep = jb.den0 * (uint64_t)kWave;
-  for (int i = 0; i < kTermsPerLane; ++i) {
+  for (int i = 0; i < TPL; ++i) {
     const uint64_t kw = kbase + (uint64_t)kWave * (uint64_t)i;
     if (kw >= jb.k1) break;
     if (k >= jb.k0 && k < jb.k1) {
@@ -166,101 +167,196 @@ __device__ __forceinline__ void tile_plain(const JobDev& jb, uint64_t kbase, int
   }
 }
 
-template <int KIND>
+template <int KIND, int TPL>
 __device__ __forceinline__ void tile_terms(const JobDev& jb, uint64_t kbase, int lane, U192& acc,
                                            uint64_t& nterms) {
-  if constexpr (KIND == kSplitW3) tile_split<3>(jb, kbase, lane, acc, nterms);
-  if constexpr (KIND == kSplitW2) tile_split<2>(jb, kbase, lane, acc, nterms);
-  if constexpr (KIND == kSplitW1) tile_split<1>(jb, kbase, lane, acc, nterms);
-  if constexpr (KIND == kPlainW3) tile_plain<false>(jb, kbase, lane, acc, nterms);
-  if constexpr (KIND == kPlainWide) tile_plain<true>(jb, kbase, lane, acc, nterms);
+  if constexpr (KIND == kSplitW3) tile_split<3, TPL>(jb, kbase, lane, acc, nterms);
+  if constexpr (KIND == kSplitW2) tile_split<2, TPL>(jb, kbase, lane, acc, nterms);
+  if constexpr (KIND == kSplitW1) tile_split<1, TPL>(jb, kbase, lane, acc, nterms);
+  if constexpr (KIND == kPlainW3) tile_plain<false, TPL>(jb, kbase, lane, acc, nterms);
+  if constexpr (KIND == kPlainWide) tile_plain<true, TPL>(jb, kbase, lane, acc, nterms);
 }
 
-// Persistent head-term kernel.  Waves take chunks of `chunk` consecutive flat tiles from a
-// global counter (one atomic per chunk), so the launch balances itself whatever the cost of a
-// tile (bitlen of its exponents) and however many waves are resident.  A wave's chunks increase
-// monotonically, so its job index only moves forward.  Flat tile f belongs to the last job with
-// flat_begin <= f.
-template <int KIND>
-__global__ void __launch_bounds__(kBlock) k_head(const JobDev* __restrict__ jobs, int njobs,
-                                                 uint64_t total_tiles, uint32_t chunk,
-                                                 unsigned long long* __restrict__ counter,
-                                                 unsigned long long* __restrict__ acc) {
+// Persistent head-term kernel.  Flat tile f belongs to the last job with flat_begin <= f.  Wave w
+// first takes the static range [w T, (w + 1) T) of T = static_tiles consecutive tiles (no
+// atomics; small launches are all static), then chunks of `chunk` consecutive tiles after the
+// nwaves T static ones from a global counter (one atomic per chunk), so the launch balances
+// itself whatever a tile costs.  A wave's tiles only increase, so its job index only moves
+// forward.
+template <int KIND, int TPL>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8)))
+k_head(LaunchDev* __restrict__ hdr, unsigned long long* __restrict__ acc) {
+  const JobDev* __restrict__ jobs = hdr_jobs(hdr);
+  const int njobs = (int)hdr->njobs;
+  const uint64_t total_tiles = hdr->total_tiles;
+  const uint64_t chunk = hdr->chunk;
+  const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
+  // readfirstlane: the compiler does not know threadIdx.x / 64 is uniform, and a divergent tile
+  // index would move the job table loads and all per-tile values into vector registers.
+  const uint64_t wave_id = (uint64_t)blockIdx.x * (kBlock / kWave) +
+                           (uint64_t)__builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint64_t static_tiles = hdr->static_tiles;
+  const uint64_t dyn0 = static_tiles * nwaves;  // first tile of the dynamic phase
   const int lane = (int)(threadIdx.x & (kWave - 1));
+  // Slot s has `replicas` copies; wave w adds into copy w mod replicas, which spreads the atomics
+  // of many short flushes over more addresses (atomics on one address serialise).
+  const uint32_t replicas = hdr->replicas;
+  const uint32_t rep = (uint32_t)wave_id & (replicas - 1);
   int cur = -1;
   JobDev jb{};
   U192 sum = {0, 0, 0};
   uint64_t nterms = 0;
-  for (;;) {
-    unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd(counter, (unsigned long long)chunk);
-    const uint64_t f0 =
-        ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32) |
-        (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)base);
-    if (f0 >= total_tiles) break;
-    const uint64_t f1 = f0 + chunk < total_tiles ? f0 + chunk : total_tiles;
-    for (uint64_t f = f0; f < f1; ++f) {
-      int lo = (cur < 0) ? 0 : cur, hi = njobs - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].flat_begin <= f)
-          lo = mid;
-        else
-          hi = mid - 1;
-      }
-      if (lo != cur) {
-        if (cur >= 0) flush_job(sum, nterms, jb.numesign, lane, acc + (size_t)jb.slot * kAccWords);
-        cur = lo;
-        jb = jobs[lo];
-        sum = {0, 0, 0};
-        nterms = 0;
-      }
-      const uint64_t kbase = (jb.tile_first + (f - jb.flat_begin)) * kTileK;
-      tile_terms<KIND>(jb, kbase, lane, sum, nterms);
+  auto tile = [&](uint64_t f) {
+    int lo = (cur < 0) ? 0 : cur, hi = njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].flat_begin <= f)
+        lo = mid;
+      else
+        hi = mid - 1;
     }
+    if (lo != cur) {
+      if (cur >= 0)
+        flush_job(sum, nterms, jb.numesign, lane,
+                  acc + ((size_t)jb.slot * replicas + rep) * kAccWords);
+      cur = lo;
+      jb = jobs[lo];
+      sum = {0, 0, 0};
+      nterms = 0;
+    }
+    const uint64_t kbase = (jb.tile_first + (f - jb.flat_begin)) * tile_k(TPL);
+    tile_terms<KIND, TPL>(jb, kbase, lane, sum, nterms);
+  };
+  // One loop and one call site of tile(), so the term code is emitted once (two copies cost
+  // registers and occupancy).  All branches here are wave-uniform.
+  uint64_t f = wave_id * static_tiles;
+  uint64_t f_end = f + static_tiles < total_tiles ? f + static_tiles : total_tiles;
+  for (;;) {
+    if (f >= f_end) {
+      if (dyn0 >= total_tiles) break;
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(&hdr->counter, (unsigned long long)chunk);
+      f = dyn0 + (((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32) |
+                  (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)base));
+      if (f >= total_tiles) break;
+      f_end = f + chunk < total_tiles ? f + chunk : total_tiles;
+    }
+    tile(f);
+    ++f;
   }
-  if (cur >= 0) flush_job(sum, nterms, jb.numesign, lane, acc + (size_t)jb.slot * kAccWords);
+  if (cur >= 0)
+    flush_job(sum, nterms, jb.numesign, lane, acc + ((size_t)jb.slot * replicas + rep) * kAccWords);
 }
 
 // ------------------------------------------------------------------ tail terms
 
 // Tail k = d + i, i = lane < 20: value 2^(-10i)/m (Bellard.cl:1164-1239).  Term =
-// floor(2^(192-10i) / m) by bit-serial long division (exact for m < 2^63), complemented when
-// negative.  One wave per tail job.
-__global__ void __launch_bounds__(kWave) k_tail(const TailDev* __restrict__ tails,
-                                                unsigned long long* __restrict__ acc) {
-  const TailDev tj = tails[blockIdx.x];
+// floor(2^(192-10i) / m), complemented when negative.  For m < 2^48 the quotient is the fp64 long
+// division of the head terms' digits (first digit E mod 48 bits, then 48 bits each); larger m take
+// bit-serial long division (exact for m < 2^63).
+__device__ __forceinline__ U192 quot_pow2_f(int E, uint64_t mi) {
+  const double m = (double)mi;
+  const double minv = recip(m);
+  const int a = E % 48 == 0 ? 48 : E % 48;
+  const int c = (E - a) / 48;  // digits after the first; digit j has weight 2^(48 (c - j))
+  U192 q = {0, 0, 0};
+  double rem = 1.0;
+  for (int j = 0; j <= c; ++j) {
+    const double n = __builtin_amdgcn_ldexp(rem, j == 0 ? a : 48);
+    // The estimate may be -1 only when n < m / 4 (the first digit, 2^a < m / 4): then the digit
+    // is 0 and the remainder n.  Later remainders are nonzero residues of 2^(a + 48 j), so their
+    // digits are at least 2^48 / m - 1 >= 0.
+    const double qd = __builtin_fmax(__builtin_floor(__builtin_fma(n, minv, -0.25)), 0.0);
+    rem = __builtin_fma(-qd, m, n);
+    const uint64_t u = f2u64(qd);
+    switch (c - j) {
+      case 0: add192(q, u, 0, 0); break;
+      case 1: add192(q, u << 48, u >> 16, 0); break;
+      case 2: add192(q, 0, u << 32, u >> 32); break;
+      default: add192(q, 0, 0, u << 16); break;
+    }
+  }
+  if (rem >= m) add192(q, 1, 0, 0);  // lazy remainder in [m, 2m)
+  return q;
+}
+
+__device__ __forceinline__ U192 quot_pow2_bits(int E, uint64_t m) {
+  U192 q = {0, 0, 0};
+  uint64_t rem = 1;  // numerator 2^E; its leading step leaves remainder 1
+  for (int b = E - 1; b >= 0; --b) {
+    rem <<= 1;
+    if (rem >= m) {
+      rem -= m;
+      if (b >= 128)
+        q.w2 |= 1ull << (b - 128);
+      else if (b >= 64)
+        q.w1 |= 1ull << (b - 64);
+      else
+        q.w0 |= 1ull << b;
+    }
+  }
+  return q;
+}
+
+__device__ __forceinline__ void tail_job(const TailDev tj, uint32_t replicas,
+                                         unsigned long long* __restrict__ acc) {
   const int lane = (int)threadIdx.x;
   U192 q = {0, 0, 0};
   if (lane < kTailTerms) {
     const uint64_t k = tj.d + (uint64_t)lane;
     const uint64_t m = tj.den0 * k + tj.den1;
-    const int top = 192 - 10 * lane;  // numerator 2^top; its leading step leaves remainder 1
-    uint64_t rem = 1;
-    for (int b = top - 1; b >= 0; --b) {
-      rem <<= 1;
-      if (rem >= m) {
-        rem -= m;
-        if (b >= 128)
-          q.w2 |= 1ull << (b - 128);
-        else if (b >= 64)
-          q.w1 |= 1ull << (b - 64);
-        else
-          q.w0 |= 1ull << b;
-      }
-    }
+    const int E = 192 - 10 * lane;
+    q = m < (1ull << 48) ? quot_pow2_f(E, m) : quot_pow2_bits(E, m);
     if ((tj.numesign + (uint32_t)(k & 1)) & 1u) q = {~q.w0, ~q.w1, ~q.w2};
   }
   wave_sum192(q);
-  if (lane == 0) atomic_add192(acc + (size_t)tj.slot * kAccWords, q);
+  if (lane == 0)
+    atomic_add192(acc + ((size_t)tj.slot * replicas + (blockIdx.x & (replicas - 1))) * kAccWords,
+                  q);
+}
+
+// One wave per tail job, grid-stride over the tail table.
+__global__ void __launch_bounds__(kWave) k_tail(const LaunchDev* __restrict__ hdr,
+                                                unsigned long long* __restrict__ acc) {
+  const TailDev* __restrict__ tails = hdr_tails(hdr);
+  for (uint32_t t = blockIdx.x; t < hdr->ntails; t += gridDim.x)
+    tail_job(tails[t], hdr->replicas, acc);
 }
 
 // ------------------------------------------------------------------ combine
+
+__global__ void k_clear(unsigned long long* __restrict__ p, const LaunchDev* __restrict__ hdr) {
+  const uint64_t n = hdr->nzero;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
 
 __global__ void k_zero(unsigned long long* p, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = 0;
+}
+
+// Sum of a slot's replicas into its first copy: one wave per slot, lane r loading copy r.
+__global__ void __launch_bounds__(kWave) k_fold(unsigned long long* __restrict__ acc,
+                                                const LaunchDev* __restrict__ hdr) {
+  const uint32_t replicas = hdr->replicas;
+  if (replicas == 1) return;
+  const int lane = (int)threadIdx.x;
+  for (uint32_t s = blockIdx.x; s < hdr->nslots; s += gridDim.x) {
+    unsigned long long* base = acc + (size_t)s * replicas * kAccWords;
+    uint64_t v[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) v[i] = (uint32_t)lane < replicas ? base[lane * kAccWords + i] : 0;
+#pragma unroll
+    for (int mask = 32; mask >= 1; mask >>= 1)
+#pragma unroll
+      for (int i = 0; i < 6; ++i) v[i] += shfl_xor64(v[i], mask);
+    if (lane == 0)
+#pragma unroll
+      for (int i = 0; i < 6; ++i) base[i] = v[i];
+  }
 }
 
 // Carry-propagate the six 32-bit limb sums of a slot into a 192-bit value.
@@ -280,18 +376,16 @@ __device__ __forceinline__ U192 shl192(U192 v, int sh) {
   return {v.w0 << sh, (v.w1 << sh) | (v.w0 >> (64 - sh)), (v.w2 << sh) | (v.w1 >> (64 - sh))};
 }
 
-__global__ void k_finalize(const unsigned long long* __restrict__ acc,
-                           const uint8_t* __restrict__ shift, const GroupDev* __restrict__ groups,
-                           int ngroups, uint64_t* __restrict__ out, int accumulate,
-                           uint64_t* __restrict__ slot_out) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ngroups) return;
-  const GroupDev gr = groups[g];
+__device__ __forceinline__ void finalize_group(const unsigned long long* __restrict__ acc,
+                                               uint32_t replicas,
+                                               const uint8_t* __restrict__ shift, GroupDev gr,
+                                               uint32_t g, uint64_t* __restrict__ out,
+                                               int accumulate, uint64_t* __restrict__ slot_out) {
   U192 total = {0, 0, 0};
   if (accumulate) total = {out[3 * g + 2], out[3 * g + 1], out[3 * g + 0]};
   for (uint32_t i = 0; i < gr.count; ++i) {
     const uint32_t s = gr.first + i;
-    const U192 v = shl192(slot_value(acc + (size_t)s * kAccWords), shift[s]);
+    const U192 v = shl192(slot_value(acc + (size_t)s * replicas * kAccWords), shift[s]);
     if (slot_out) {
       slot_out[3 * s + 0] = v.w2;
       slot_out[3 * s + 1] = v.w1;
@@ -302,6 +396,16 @@ __global__ void k_finalize(const unsigned long long* __restrict__ acc,
   out[3 * g + 0] = total.w2;
   out[3 * g + 1] = total.w1;
   out[3 * g + 2] = total.w0;
+}
+
+__global__ void k_finalize(const unsigned long long* __restrict__ acc,
+                           const LaunchDev* __restrict__ hdr, uint64_t* __restrict__ out,
+                           int accumulate, uint64_t* __restrict__ slot_out) {
+  const GroupDev* __restrict__ groups = hdr_groups(hdr);
+  const uint8_t* __restrict__ shift = hdr_shifts(hdr);
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < hdr->ngroups;
+       g += gridDim.x * blockDim.x)
+    finalize_group(acc, hdr->replicas, shift, groups[g], g, out, accumulate, slot_out);
 }
 
 __global__ void k_sum7(uint64_t* ans, uint64_t* partial, int shift) {
@@ -319,48 +423,75 @@ __global__ void k_sum7(uint64_t* ans, uint64_t* partial, int shift) {
 
 // ------------------------------------------------------------------ launch wrappers
 
-template <int KIND>
+template <int KIND, int TPL>
 static int resident_blocks() {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_head<KIND>, kBlock, 0) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_head<KIND, TPL>, kBlock, 0) != hipSuccess)
     n = 0;
   return n;
 }
 
-int head_blocks_per_cu(int kind) {
-  switch (kind) {
-    case kSplitW3: return resident_blocks<kSplitW3>();
-    case kSplitW2: return resident_blocks<kSplitW2>();
-    case kSplitW1: return resident_blocks<kSplitW1>();
-    case kPlainW3: return resident_blocks<kPlainW3>();
-    case kPlainWide: return resident_blocks<kPlainWide>();
+template <int KIND>
+static int resident_blocks_tpl(int tpl_index) {
+  switch (tpl_index) {
+    case 0: return resident_blocks<KIND, kTplChoices[0]>();
+    case 1: return resident_blocks<KIND, kTplChoices[1]>();
+    case 2: return resident_blocks<KIND, kTplChoices[2]>();
     default: return 0;
   }
 }
 
-hipError_t launch_head(int kind, const JobDev* jobs, int njobs, uint64_t total_tiles,
-                       uint32_t chunk, unsigned long long* counter, unsigned long long* acc,
-                       int grid, hipStream_t stream) {
-  if (njobs <= 0 || total_tiles == 0) return hipSuccess;
-#define CLB_LAUNCH_HEAD(K)                                                                   \
-  hipLaunchKernelGGL(k_head<K>, dim3(grid), dim3(kBlock), 0, stream, jobs, njobs, total_tiles, \
-                     chunk, counter, acc)
+int head_blocks_per_cu(int kind, int tpl_index) {
   switch (kind) {
-    case kSplitW3: CLB_LAUNCH_HEAD(kSplitW3); break;
-    case kSplitW2: CLB_LAUNCH_HEAD(kSplitW2); break;
-    case kSplitW1: CLB_LAUNCH_HEAD(kSplitW1); break;
-    case kPlainW3: CLB_LAUNCH_HEAD(kPlainW3); break;
-    case kPlainWide: CLB_LAUNCH_HEAD(kPlainWide); break;
+    case kSplitW3: return resident_blocks_tpl<kSplitW3>(tpl_index);
+    case kSplitW2: return resident_blocks_tpl<kSplitW2>(tpl_index);
+    case kSplitW1: return resident_blocks_tpl<kSplitW1>(tpl_index);
+    case kPlainW3: return resident_blocks_tpl<kPlainW3>(tpl_index);
+    case kPlainWide: return resident_blocks_tpl<kPlainWide>(tpl_index);
+    default: return 0;
+  }
+}
+
+template <int KIND>
+static hipError_t launch_head_tpl(int tpl_index, LaunchDev* hdr, unsigned long long* acc, int grid,
+                                  hipStream_t stream) {
+  switch (tpl_index) {
+    case 0:
+      hipLaunchKernelGGL((k_head<KIND, kTplChoices[0]>), dim3(grid), dim3(kBlock), 0, stream, hdr,
+                         acc);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_head<KIND, kTplChoices[1]>), dim3(grid), dim3(kBlock), 0, stream, hdr,
+                         acc);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_head<KIND, kTplChoices[2]>), dim3(grid), dim3(kBlock), 0, stream, hdr,
+                         acc);
+      break;
     default: return hipErrorInvalidValue;
   }
-#undef CLB_LAUNCH_HEAD
   return hipGetLastError();
 }
 
-hipError_t launch_tail(const TailDev* tails, int ntails, unsigned long long* acc,
-                       hipStream_t stream) {
-  if (ntails <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tail, dim3(ntails), dim3(kWave), 0, stream, tails, acc);
+hipError_t launch_head(int kind, int tpl_index, LaunchDev* hdr, unsigned long long* acc,
+                       int grid, hipStream_t stream) {
+  switch (kind) {
+    case kSplitW3: return launch_head_tpl<kSplitW3>(tpl_index, hdr, acc, grid, stream);
+    case kSplitW2: return launch_head_tpl<kSplitW2>(tpl_index, hdr, acc, grid, stream);
+    case kSplitW1: return launch_head_tpl<kSplitW1>(tpl_index, hdr, acc, grid, stream);
+    case kPlainW3: return launch_head_tpl<kPlainW3>(tpl_index, hdr, acc, grid, stream);
+    case kPlainWide: return launch_head_tpl<kPlainWide>(tpl_index, hdr, acc, grid, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_tail(const LaunchDev* hdr, unsigned long long* acc, hipStream_t stream) {
+  hipLaunchKernelGGL(k_tail, dim3(kTailBlocks), dim3(kWave), 0, stream, hdr, acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_clear(unsigned long long* acc, const LaunchDev* hdr, hipStream_t stream) {
+  hipLaunchKernelGGL(k_clear, dim3(kClearBlocks), dim3(256), 0, stream, acc, hdr);
   return hipGetLastError();
 }
 
@@ -372,13 +503,13 @@ hipError_t launch_zero(unsigned long long* p, uint64_t nwords, hipStream_t strea
   return hipGetLastError();
 }
 
-hipError_t launch_finalize(const unsigned long long* acc, const uint8_t* shift, int nslots,
-                           const GroupDev* groups, int ngroups, uint64_t* out, int accumulate,
-                           uint64_t* slot_out, hipStream_t stream) {
-  (void)nslots;
-  if (ngroups <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_finalize, dim3((ngroups + 63) / 64), dim3(64), 0, stream, acc, shift,
-                     groups, ngroups, out, accumulate, slot_out);
+hipError_t launch_finalize(unsigned long long* acc, const LaunchDev* hdr, uint64_t* out,
+                           int accumulate, uint64_t* slot_out, hipStream_t stream) {
+  hipLaunchKernelGGL(k_fold, dim3(kFoldBlocks), dim3(kWave), 0, stream, acc, hdr);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_finalize, dim3(kFinalizeBlocks), dim3(64), 0, stream, acc, hdr, out,
+                     accumulate, slot_out);
   return hipGetLastError();
 }
 

@@ -82,6 +82,16 @@ int clb_ctx_destroy(clb_ctx *ctx);
 int clb_set_kernel(clb_ctx *ctx, int kernel);
 /* Launch shape: persistent workgroups per compute unit (0 = library default). */
 int clb_set_launch(clb_ctx *ctx, int blocks_per_cu);
+/* Tile length (no reference counterpart; the reference's work-items take GWS-strided terms,
+ * KernelFuncs.as:38-52): terms per lane per tile, 16, 4 or 1; or 0 to choose per launch the
+ * longest whose tiles give every resident wave at least min_tiles_per_wave tiles (0: always 16,
+ * the default, which measured fastest at every size).  Results are identical for every choice. */
+int clb_set_tiling(clb_ctx *ctx, int terms_per_lane, int min_tiles_per_wave);
+/* Launch mode (no reference counterpart; the reference enqueues every kernel from the HSP host,
+ * KernelFuncs.as:25-74): 1 = run each call's kernel sequence (clear, head terms, tails, combine)
+ * as a HIP graph, captured on the first call with a given buffer layout and kernel choice and
+ * replayed by later calls; 0 (default) = launch the kernels one by one.  Results are identical. */
+int clb_set_graph(clb_ctx *ctx, int enable);
 
 /* BuildOptions core (Bellard_192.hsp:449-450, :468-474): N -> odd loop count d and N' (a multiple
  * of 5, the 1-based hex position of the first output digit); overdouble = (10d + 9 >= 2^53).
@@ -152,7 +162,8 @@ int clb_write_work_file(const char *work_dir, int j, int64_t b, const uint64_t w
 int clb_work_sum(const char *work_dir, char hex[49], int counts[7]);
 
 /* Timing of the last clb_normal / clb_batch_positions / clb_run_jobs / clb_batch_calc call on
- * the context's stream (HIP events): the head-term kernel and the whole call, in ms. */
+ * the context's stream (HIP events): the head-term kernel (in graph mode: the whole replayed
+ * graph) and the whole call, in ms. */
 int clb_last_timing(clb_ctx *ctx, double *head_kernel_ms, double *total_ms);
 
 #ifdef __cplusplus

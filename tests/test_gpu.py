@@ -233,3 +233,57 @@ def test_worklist_end_to_end(tmp_path, precision):
         assert total == hexs
     else:
         assert line0.endswith("0" * 20)
+
+
+def test_graph_mode_replays_match_eager(ctx, oracle):
+    """Graph mode (clb_set_graph): every replay of a captured sequence, with eager work on another
+    context and synchronisations in between, equals the eager result; calls of other sizes reuse
+    or recapture the graph as their buffers require."""
+    g = clb.Context(0, clb.PREC_U192)
+    g.set_graph(True)
+    try:
+        rng = random.Random(7)
+        for rep in range(3):
+            for n in (5, 100000, 65540, 10**7):
+                note(f"graph replay {rep}: normal N={n}")
+                want = ctx.normal(n)
+                assert g.normal(n) == want, (rep, n)
+            pos = [5 * round(10 ** (3 + 4 * rng.random()) / 5) for _ in range(12 + 20 * rep)]
+            note(f"graph replay {rep}: batch of {len(pos)} positions")
+            assert g.batch_positions(pos) == ctx.batch_positions(pos), rep
+            _, d = py_normalize(10**8)
+            jobs = [(d, 1000 * rep, d // 3 + rep, j) for j in range(7)]
+            assert g.run_jobs(jobs) == ctx.run_jobs(jobs), rep
+            # the accumulating path (BatchCalc into the partial, AddAns)
+            g.set0mem()
+            g.clear_ans()
+            ctx.set0mem()
+            ctx.clear_ans()
+            for j in range(7):
+                g.set0mem()
+                g.batch_calc(d, 0, d, j)
+                g.add_ans(j)
+                ctx.set0mem()
+                ctx.batch_calc(d, 0, d, j)
+                ctx.add_ans(j)
+            assert g.read_ans() == ctx.read_ans(), rep
+        assert w2i(g.normal(250000)[0]) == oracle.normal(250000)
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("tpl", [16, 4, 1])
+def test_tile_lengths_give_identical_bits(oracle, tpl):
+    """Every terms-per-lane choice, both engines, against the oracle and the automatic choice."""
+    for kernel in (clb.KERNEL_SPLIT, clb.KERNEL_PLAIN):
+        with clb.Context(0, clb.PREC_U192, kernel=kernel) as c:
+            c.set_tiling(tpl)
+            for n in (5, 1005, 65540, 250000):
+                note(f"tiles of {tpl} terms per lane, kernel {kernel}: N={n}")
+                assert w2i(c.normal(n)[0]) == oracle.normal(n), (tpl, kernel, n)
+            _, d = py_normalize(10**9)
+            jobs = [(d, 12345 + 777 * j, 12345 + 777 * j + 3 * (1 << 20) + j, j) for j in range(7)]
+            jobs += [(d, d - 5000 - j, d + 3, j) for j in range(7)]
+            got = c.run_jobs(jobs)
+            c.set_tiling(0)
+            assert got == c.run_jobs(jobs), (tpl, kernel)
