@@ -86,14 +86,20 @@ def bench_batched(clb, args):
         pos.append(5 * round(10 ** (6 + 5 * u) / 5))
     terms = sum(clb.normalize(p)[1] for p in pos)
     ctx = clb.Context(0, clb.PREC_U192)
+    median = sorted(pos)[len(pos) // 2]
+    tpl, bpc = ctx.autotune(median, 3000.0)
+    log(f"autotune at N={median}: {tpl} terms per lane, {bpc} workgroups per CU")
     ctx.set_graph(not args.no_graph)
     log(f"batched: {len(pos)} positions, {terms:.4g} terms, "
         f"{'eager launches' if args.no_graph else 'captured HIP graph'}")
-    t0 = time.perf_counter()
-    fracs = ctx.batch_positions(pos)
-    dt = time.perf_counter() - t0
-    head_ms = ctx.last_timing()[0]
-    log(f"batched launch: {dt:.2f}s")
+    fracs, dt, head_ms = [], 0.0, 0.0
+    per = -(-len(pos) // args.batches)
+    for b in range(0, len(pos), per):
+        t0 = time.perf_counter()
+        fracs += ctx.batch_positions(pos[b:b + per])
+        dt += time.perf_counter() - t0
+        head_ms += ctx.last_timing()[0]
+        log(f"launch of positions {b}..{min(b + per, len(pos)) - 1}: {dt:.2f}s so far")
     pick = sorted(range(len(pos)), key=lambda i: pos[i])
     sample = [pick[int(i * (len(pick) - 1) / max(1, args.check - 1))] for i in range(args.check)]
     for i in sample:
@@ -105,9 +111,11 @@ def bench_batched(clb, args):
         "unit": "terms/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": 1e3 * dt,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{len(pos)} positions log-uniform in [1e6, 1e11], seed 0, one "
-                               "persistent launch", "positions": len(pos), "terms": terms,
-                   "graph": not args.no_graph, "head_kernel_ms": head_ms,
+        "config": {"workload": f"{len(pos)} positions log-uniform in [1e6, 1e11], seed 0, "
+                               f"{args.batches} persistent launch(es)", "positions": len(pos),
+                   "terms": terms,
+                   "graph": not args.no_graph, "terms_per_lane": tpl,
+                   "workgroups_per_cu": bpc, "head_kernel_ms": head_ms,
                    "checked_single": len(sample)}}), flush=True)
     ctx.close()
     return 0
@@ -212,6 +220,8 @@ def main():
                     help="normal: the metric (N = 10^12); batched: many positions in one launch; "
                          "worklist: uneven term-range jobs written as work files and summed; "
                          "small: call latency of small positions, eager vs captured graph")
+    ap.add_argument("--batches", type=int, default=1,
+                    help="batched: split the positions over this many launches")
     ap.add_argument("--no-graph", action="store_true",
                     help="batched: launch the kernels one by one instead of a captured graph")
     ap.add_argument("--calls", type=int, default=200, help="small: calls per mode")

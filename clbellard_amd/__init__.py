@@ -75,6 +75,8 @@ _SIGNATURES = {
     "clb_set_launch": ([_CTX, ctypes.c_int], ctypes.c_int),
     "clb_set_graph": ([_CTX, ctypes.c_int], ctypes.c_int),
     "clb_set_tiling": ([_CTX, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "clb_autotune": ([_CTX, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(ctypes.c_int),
+                      ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "clb_normalize": ([_U64, _P64, _P64, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "clb_formula": ([ctypes.c_int, _P64, ctypes.POINTER(ctypes.c_int64), _P64, _P64],
                     ctypes.c_int),
@@ -264,6 +266,13 @@ class Context:
         """Terms per lane per tile (16, 4, 1; 0 = chosen per launch by min_tiles_per_wave tiles
         per resident wave, 0 = always 16)."""
         _check(lib.clb_set_tiling(self._ctx, terms_per_lane, min_tiles_per_wave))
+
+    def autotune(self, n: int, budget_ms: float = 2000.0) -> Tuple[int, int]:
+        """Time tile lengths and grid sizes on a sample of position n; keep and return the
+        fastest (terms per lane, workgroups per CU)."""
+        tpl, bpc = ctypes.c_int(), ctypes.c_int()
+        _check(lib.clb_autotune(self._ctx, n, budget_ms, ctypes.byref(tpl), ctypes.byref(bpc)))
+        return tpl.value, bpc.value
 
     def set_graph(self, enable: bool) -> None:
         """Replay each call's kernel sequence as a captured HIP graph (identical results)."""

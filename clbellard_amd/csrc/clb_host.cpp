@@ -751,6 +751,83 @@ int clb_set_tiling(clb_ctx* ctx, int terms_per_lane, int min_tiles_per_wave) {
   return CLB_OK;
 }
 
+int clb_autotune(clb_ctx* ctx, uint64_t n, double budget_ms, int* terms_per_lane,
+                 int* blocks_per_cu_out) {
+  if (int rc = check_ctx(ctx)) return rc;
+  uint64_t d;
+  if (int rc = normalize(n, nullptr, &d, nullptr)) return rc;
+  if (!(budget_ms > 0.0)) return fail(CLB_EINVAL, "budget_ms must be positive");
+  const int saved_tpl = ctx->tpl_index, saved_bpc = ctx->blocks_per_cu;
+  const int saved_min = ctx->min_tiles_per_wave;
+  const bool saved_graph = ctx->use_graph;
+  ctx->use_graph = false;
+  // The sample: the seven formulas' head terms from k0, s terms each (all of [0, d) when that
+  // fits the budget), no tails.
+  auto jobs_for = [&](uint64_t k0, uint64_t s) {
+    std::vector<HostJob> jobs;
+    for (int j = 0; j < 7; ++j) {
+      HostJob h =
+          batch_job(d, k0, k0 + s, kDen0[j], kDen1[j], (uint32_t)kNumeSign[j], (uint32_t)j);
+      h.tail = false;
+      jobs.push_back(h);
+    }
+    return jobs;
+  };
+  std::vector<uint8_t> shift(kShift, kShift + 7);
+  std::vector<GroupDev> groups{{0, 7}};
+  uint64_t res[3], best_res[3] = {0, 0, 0};
+  auto restore = [&](int rc) {
+    ctx->tpl_index = saved_tpl;
+    ctx->blocks_per_cu = saved_bpc;
+    ctx->min_tiles_per_wave = saved_min;
+    ctx->use_graph = saved_graph;
+    return rc;
+  };
+  // probe at the default settings to size the sample
+  ctx->tpl_index = 0;
+  ctx->blocks_per_cu = 0;
+  const uint64_t probe = d < (1ull << 20) ? d : (1ull << 20);
+  const uint64_t k_mid = d > probe ? (d - probe) / 2 : 0;
+  if (int rc = run(ctx, jobs_for(k_mid, probe), shift, groups, res, nullptr, nullptr))
+    return restore(rc);
+  const double per_term = (ctx->head_ms > 0.f ? ctx->head_ms : 1e-3) / (double)probe;
+  const int resident = blocks_per_cu(ctx, kSplitW3, 0);
+  const int bpc_choices[3] = {resident, (3 * resident + 3) / 4, (resident + 1) / 2};
+  const int ncand = kNumTpl * 3, reps = 2;
+  uint64_t s = (uint64_t)(budget_ms / (ncand * reps + 1) / per_term);
+  if (s < probe) s = probe;
+  if (s > d) s = d;
+  const uint64_t k0 = (d - s) / 2;
+  const std::vector<HostJob> sample = jobs_for(k0, s);
+  double best = -1.0;
+  int best_tpl = 0, best_bpc = resident;
+  for (int t = 0; t < kNumTpl; ++t)
+    for (int b = 0; b < 3; ++b) {
+      if (b > 0 && bpc_choices[b] == bpc_choices[b - 1]) continue;
+      ctx->tpl_index = t;
+      ctx->blocks_per_cu = bpc_choices[b];
+      double ms = -1.0;
+      for (int r = 0; r < reps; ++r) {
+        if (int rc = run(ctx, sample, shift, groups, res, nullptr, nullptr)) return restore(rc);
+        if (ms < 0.0 || ctx->head_ms < ms) ms = ctx->head_ms;
+      }
+      if (best >= 0.0 && memcmp(res, best_res, sizeof res) != 0)
+        return restore(fail(CLB_EDEVICE, "autotune: results differ between launch shapes"));
+      memcpy(best_res, res, sizeof res);
+      if (best < 0.0 || ms < best) {
+        best = ms;
+        best_tpl = t;
+        best_bpc = bpc_choices[b];
+      }
+    }
+  restore(CLB_OK);
+  ctx->tpl_index = best_tpl;
+  ctx->blocks_per_cu = best_bpc == resident ? 0 : best_bpc;
+  if (terms_per_lane) *terms_per_lane = kTplChoices[best_tpl];
+  if (blocks_per_cu_out) *blocks_per_cu_out = best_bpc;
+  return CLB_OK;
+}
+
 int clb_set_graph(clb_ctx* ctx, int enable) {
   if (int rc = check_ctx(ctx)) return rc;
   if (enable != 0 && enable != 1) return fail(CLB_EINVAL, "graph mode must be 0 or 1");

@@ -287,3 +287,15 @@ def test_tile_lengths_give_identical_bits(oracle, tpl):
             got = c.run_jobs(jobs)
             c.set_tiling(0)
             assert got == c.run_jobs(jobs), (tpl, kernel)
+
+
+def test_autotune_keeps_bits(oracle):
+    """clb_autotune picks a valid tile length and grid, and results stay bit-identical."""
+    with clb.Context(0, clb.PREC_U192) as c:
+        for n, budget in ((250000, 50.0), (10**9, 300.0)):
+            tpl, bpc = c.autotune(n, budget)
+            note(f"autotune N={n}: {tpl} terms per lane, {bpc} workgroups per CU")
+            assert tpl in (16, 4, 1) and bpc >= 1
+        assert w2i(c.normal(250000)[0]) == oracle.normal(250000)
+        with clb.Context(0, clb.PREC_U192) as ref:
+            assert c.normal(10**8) == ref.normal(10**8)
