@@ -9,6 +9,7 @@
 #include <sys/types.h>
 
 #include <cerrno>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -127,12 +128,8 @@ struct clb_ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   uint64_t* partial = nullptr;  // bigSumMem[0..2] (most significant word first)
   uint64_t* ans = nullptr;      // dev_ans[0..2]
-  DevBuf<uint8_t> block;        // the call's table block: LaunchDev header + tables
-  HostBuf block_host;           // its pinned staging copy
-  DevBuf<unsigned long long> acc;
-  DevBuf<uint64_t> out;
-  DevBuf<uint64_t> slot_out;
-  HostBuf result_host;          // pinned landing area of out (+ slot_out)
+  DevBuf<uint8_t> block;        // the call's block: header, tables, accumulator, results
+  HostBuf block_host;           // its pinned staging copy (results land here too)
   float head_ms = 0.f, total_ms = 0.f;
   int resident[5][kNumTpl] = {};  // resident k_head workgroups per CU, per kind and tile length
   int tpl_index = -1;            // index into kTplChoices, or -1: chosen per launch
@@ -141,7 +138,7 @@ struct clb_ctx {
   // (every argument the kernels see), replayed afterwards.  Counts travel in hdr.
   bool use_graph = false;
   struct Graph {
-    uintptr_t key[8];
+    uintptr_t key[4];
     hipGraphExec_t exec;
   };
   std::vector<Graph> graphs;
@@ -180,15 +177,12 @@ int blocks_per_cu(clb_ctx* ctx, int kind, int tpl) {
   return ctx->resident[kind][tpl];
 }
 
-// The captured launch sequence (clear, head, tail, finalize) for this call's kernel arguments,
-// captured on first use.  Kernels read counts and tables from the block, so a graph stays valid
-// while the buffers it names are not reallocated; at most 8 are kept.
-int graph_for(clb_ctx* ctx, int kind, int tpl, int grid, uint64_t* dst, bool accumulate,
-              uint64_t* slot_dev, hipGraphExec_t* exec) {
+// The captured launch sequence (head with tails, combine) for a kernel choice and block address,
+// captured on first use.  Kernels read everything else from the block, so a graph stays valid
+// while the block is not reallocated; at most 8 are kept.
+int graph_for(clb_ctx* ctx, int kind, int tpl, int grid, hipGraphExec_t* exec) {
   LaunchDev* hdr = reinterpret_cast<LaunchDev*>(ctx->block.p);
-  const uintptr_t key[8] = {(uintptr_t)kind,       (uintptr_t)tpl, (uintptr_t)grid,
-                            (uintptr_t)hdr,        (uintptr_t)ctx->acc.p, (uintptr_t)dst,
-                            (uintptr_t)accumulate, (uintptr_t)slot_dev};
+  const uintptr_t key[4] = {(uintptr_t)kind, (uintptr_t)tpl, (uintptr_t)grid, (uintptr_t)hdr};
   for (const auto& g : ctx->graphs)
     if (memcmp(g.key, key, sizeof key) == 0) {
       *exec = g.exec;
@@ -196,10 +190,8 @@ int graph_for(clb_ctx* ctx, int kind, int tpl, int grid, uint64_t* dst, bool acc
     }
   hipStream_t s = ctx->stream;
   HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  hipError_t e = launch_clear(ctx->acc.p, hdr, s);
-  if (e == hipSuccess) e = launch_head(kind, tpl, hdr, ctx->acc.p, grid, s);
-  if (e == hipSuccess) e = launch_tail(hdr, ctx->acc.p, s);
-  if (e == hipSuccess) e = launch_finalize(ctx->acc.p, hdr, dst, accumulate, slot_dev, s);
+  hipError_t e = launch_head(kind, tpl, hdr, grid, s);
+  if (e == hipSuccess) e = launch_combine(hdr, s);
   hipGraph_t graph = nullptr;
   const hipError_t e_end = hipStreamEndCapture(s, &graph);
   if (e == hipSuccess) e = e_end;
@@ -219,13 +211,21 @@ int graph_for(clb_ctx* ctx, int kind, int tpl, int grid, uint64_t* dst, bool acc
   return CLB_OK;
 }
 
+// Device-side copies a call leaves behind (clb_normal keeps the reference's Allloop state).
+struct Mirrors {
+  uint64_t* total = nullptr;  // receives group 0's result
+  uint64_t* slot = nullptr;   // receives slot `index`'s shifted value
+  uint32_t index = 0;
+};
+
 // Runs head jobs + tails into slots [0, nslots), then combines the slots by groups.
 // out_host receives 3 words per group unless accumulate_dev is given, in which case the single
 // group's value is added into that device buffer.  slot_host (optional) receives each slot's
 // shifted value.
 int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8_t>& shift,
         const std::vector<GroupDev>& groups, uint64_t* out_host, uint64_t* accumulate_dev,
-        uint64_t* slot_host) {
+        uint64_t* slot_host, const Mirrors& mirrors = Mirrors()) {
+  const auto t_call = std::chrono::steady_clock::now();
   int rc = set_device(ctx);
   if (rc) return rc;
   const size_t nslots = shift.size();
@@ -276,25 +276,28 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
     flat += (h.k1 + tk - 1) / tk - j.tile_first;
     jobs.push_back(j);
   }
-  // The table block: header, jobs, tails, groups, shifts, each 64-byte aligned.
+  const uint64_t nwaves = grid * (uint64_t)(kBlock / kWave);
+  // Copies per accumulator slot: enough that the waves' flushes spread over about nwaves / 16
+  // addresses per limb (a power of 2, at most 64), so short launches do not queue on one slot.
+  uint32_t replicas = 1;
+  while (replicas < 64 && (uint64_t)replicas * 2 * nslots * 16 <= nwaves) replicas *= 2;
+  // The block: header, jobs, tails, groups, shifts, accumulator (copied in as zeros), results
+  // (group results, then slot values), each 64-byte aligned.
   auto align64 = [](size_t x) { return (x + 63) & ~(size_t)63; };
   const size_t off_jobs = align64(sizeof(LaunchDev));
   const size_t off_tails = off_jobs + align64(jobs.size() * sizeof(JobDev));
   const size_t off_groups = off_tails + align64(tails.size() * sizeof(TailDev));
   const size_t off_shifts = off_groups + align64(groups.size() * sizeof(GroupDev));
-  const size_t bytes = off_shifts + nslots;
+  const size_t off_acc = off_shifts + align64(nslots);
+  const size_t off_out = off_acc + nslots * replicas * kAccWords * sizeof(uint64_t);
+  const size_t out_words = 3 * groups.size() + (slot_host ? 3 * nslots : 0);
+  const size_t bytes = off_out + out_words * sizeof(uint64_t);
   if (bytes > 0xffffffffull) return fail(CLB_EINVAL, "too many jobs for one call");
-  const size_t res_words = 3 * groups.size() + (slot_host ? 3 * nslots : 0);
-  if ((rc = ctx->block.reserve(bytes)) || (rc = ctx->block_host.reserve(bytes)) ||
-      (rc = ctx->out.reserve(3 * groups.size())) ||
-      (rc = ctx->slot_out.reserve(3 * nslots)) ||
-      (rc = ctx->result_host.reserve(res_words * sizeof(uint64_t))))
-    return rc;
+  if ((rc = ctx->block.reserve(bytes)) || (rc = ctx->block_host.reserve(bytes))) return rc;
   // Tile schedule.  Up to 64 tiles per wave: each wave takes one static range of consecutive
   // tiles (no atomics; atomics on one address serialise at ~35 ns each).  Larger launches: all
   // tiles from the queue in chunks, at least 8 per wave and at most 256 tiles (~15 ms of one
   // wave's work at N = 10^12, the granularity of the final balance).
-  const uint64_t nwaves = grid * (uint64_t)(kBlock / kWave);
   uint64_t static_tiles = 0, chunk = 1;
   if (flat <= 64 * nwaves) {
     static_tiles = (flat + nwaves - 1) / nwaves;
@@ -302,74 +305,69 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
     const uint64_t per = flat / (8 * nwaves);
     chunk = per < 1 ? 1 : per > 256 ? 256 : per;
   }
-  // Copies per accumulator slot: enough that the waves' flushes spread over about nwaves / 16
-  // addresses per limb (a power of 2, at most 64), so short launches do not queue on one slot.
-  uint32_t replicas = 1;
-  while (replicas < 64 && (uint64_t)replicas * 2 * nslots * 16 <= nwaves) replicas *= 2;
-  if ((rc = ctx->acc.reserve(nslots * replicas * kAccWords))) return rc;
+  uint8_t* dev = ctx->block.p;
   LaunchDev h{};
   h.counter = 0;
   h.total_tiles = flat;
-  h.nzero = nslots * replicas * kAccWords;
-  h.replicas = replicas;
-  h.nslots = (uint32_t)nslots;
   h.static_tiles = static_tiles;
+  h.dst = (uint64_t)(uintptr_t)(accumulate_dev ? accumulate_dev
+                                               : reinterpret_cast<uint64_t*>(dev + off_out));
+  h.slot_out = slot_host ? (uint64_t)(uintptr_t)(dev + off_out + 3 * groups.size() * 8) : 0;
+  h.mirror_total = (uint64_t)(uintptr_t)mirrors.total;
+  h.mirror_slot = (uint64_t)(uintptr_t)mirrors.slot;
+  h.mirror_index = mirrors.index;
+  h.accumulate = accumulate_dev != nullptr;
   h.njobs = (uint32_t)jobs.size();
   h.chunk = (uint32_t)chunk;
   h.ntails = (uint32_t)tails.size();
   h.ngroups = (uint32_t)groups.size();
+  h.replicas = replicas;
+  h.nslots = (uint32_t)nslots;
   h.off_jobs = (uint32_t)off_jobs;
   h.off_tails = (uint32_t)off_tails;
   h.off_groups = (uint32_t)off_groups;
   h.off_shifts = (uint32_t)off_shifts;
+  h.off_acc = (uint32_t)off_acc;
+  h.off_out = (uint32_t)off_out;
   uint8_t* stage = ctx->block_host.p;
   memcpy(stage, &h, sizeof h);
   if (!jobs.empty()) memcpy(stage + off_jobs, jobs.data(), jobs.size() * sizeof(JobDev));
   if (!tails.empty()) memcpy(stage + off_tails, tails.data(), tails.size() * sizeof(TailDev));
   if (!groups.empty()) memcpy(stage + off_groups, groups.data(), groups.size() * sizeof(GroupDev));
   if (nslots) memcpy(stage + off_shifts, shift.data(), nslots);
-  LaunchDev* hdr = reinterpret_cast<LaunchDev*>(ctx->block.p);
-  uint64_t* dst = accumulate_dev ? accumulate_dev : ctx->out.p;
-  uint64_t* slot_dev = slot_host ? ctx->slot_out.p : nullptr;
+  memset(stage + off_acc, 0, off_out - off_acc);
+  LaunchDev* hdr = reinterpret_cast<LaunchDev*>(dev);
   hipStream_t s = ctx->stream;
-  HIP_TRY(hipEventRecord(ctx->ev[0], s));
-  HIP_TRY(hipMemcpyAsync(ctx->block.p, stage, bytes, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dev, stage, off_out, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipEventRecord(ctx->ev[1], s));
   if (ctx->use_graph) {
     hipGraphExec_t exec = nullptr;
-    if ((rc = graph_for(ctx, kind, tpl, (int)grid, dst, accumulate_dev != nullptr, slot_dev,
-                        &exec)))
-      return rc;
-    HIP_TRY(hipEventRecord(ctx->ev[1], s));
+    if ((rc = graph_for(ctx, kind, tpl, (int)grid, &exec))) return rc;
     HIP_TRY(hipGraphLaunch(exec, s));
     HIP_TRY(hipEventRecord(ctx->ev[2], s));
   } else {
-    HIP_TRY(launch_clear(ctx->acc.p, hdr, s));
-    HIP_TRY(hipEventRecord(ctx->ev[1], s));
-    HIP_TRY(launch_head(kind, tpl, hdr, ctx->acc.p, (int)grid, s));
+    HIP_TRY(launch_head(kind, tpl, hdr, (int)grid, s));
     HIP_TRY(hipEventRecord(ctx->ev[2], s));
-    HIP_TRY(launch_tail(hdr, ctx->acc.p, s));
-    HIP_TRY(launch_finalize(ctx->acc.p, hdr, dst, accumulate_dev != nullptr, slot_dev, s));
+    HIP_TRY(launch_combine(hdr, s));
   }
-  uint64_t* res = reinterpret_cast<uint64_t*>(ctx->result_host.p);
-  if (out_host && !accumulate_dev && !groups.empty())
-    HIP_TRY(hipMemcpyAsync(res, ctx->out.p, 3 * groups.size() * sizeof(uint64_t),
-                           hipMemcpyDeviceToHost, s));
-  if (slot_host && nslots)
-    HIP_TRY(hipMemcpyAsync(res + 3 * groups.size(), ctx->slot_out.p,
-                           3 * nslots * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  hipError_t e2 = hipEventRecord(ctx->ev[3], s);
-  if (e2 == hipSuccess) e2 = hipStreamSynchronize(s);
-  float head = 0.f, total = 0.f;
+  const size_t back = accumulate_dev ? (slot_host ? 3 * nslots : 0) : out_words;
+  const size_t back_off = accumulate_dev ? off_out + 3 * groups.size() * 8 : off_out;
+  if (back) HIP_TRY(hipMemcpyAsync(stage + back_off, dev + back_off, back * 8,
+                                   hipMemcpyDeviceToHost, s));
+  hipError_t e2 = hipStreamSynchronize(s);
+  float head = 0.f;
   if (e2 == hipSuccess) e2 = hipEventElapsedTime(&head, ctx->ev[1], ctx->ev[2]);
-  if (e2 == hipSuccess) e2 = hipEventElapsedTime(&total, ctx->ev[0], ctx->ev[3]);
   if (e2 != hipSuccess)
     return fail(CLB_EDEVICE, "term launch failed: %s", hipGetErrorString(e2));
+  const uint64_t* res = reinterpret_cast<const uint64_t*>(stage + off_out);
   if (out_host && !accumulate_dev && !groups.empty())
     memcpy(out_host, res, 3 * groups.size() * sizeof(uint64_t));
   if (slot_host && nslots)
     memcpy(slot_host, res + 3 * groups.size(), 3 * nslots * sizeof(uint64_t));
   ctx->head_ms = head;
-  ctx->total_ms = total;
+  ctx->total_ms = (float)std::chrono::duration<double, std::milli>(
+                      std::chrono::steady_clock::now() - t_call)
+                      .count();
   return CLB_OK;
 }
 
@@ -714,10 +712,6 @@ int clb_ctx_destroy(clb_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   ctx->block.release();
   ctx->block_host.release();
-  ctx->acc.release();
-  ctx->out.release();
-  ctx->slot_out.release();
-  ctx->result_host.release();
   for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
   ctx->graphs.clear();
   if (ctx->partial) (void)hipFree(ctx->partial);
@@ -793,7 +787,7 @@ int clb_autotune(clb_ctx* ctx, uint64_t n, double budget_ms, int* terms_per_lane
   const double per_term = (ctx->head_ms > 0.f ? ctx->head_ms : 1e-3) / (double)probe;
   const int resident = blocks_per_cu(ctx, kSplitW3, 0);
   const int bpc_choices[3] = {resident, (3 * resident + 3) / 4, (resident + 1) / 2};
-  const int ncand = kNumTpl * 3, reps = 2;
+  const int ncand = kNumTpl * 3, reps = 3;
   uint64_t s = (uint64_t)(budget_ms / (ncand * reps + 1) / per_term);
   if (s < probe) s = probe;
   if (s > d) s = d;
@@ -814,7 +808,9 @@ int clb_autotune(clb_ctx* ctx, uint64_t n, double budget_ms, int* terms_per_lane
       if (best >= 0.0 && memcmp(res, best_res, sizeof res) != 0)
         return restore(fail(CLB_EDEVICE, "autotune: results differ between launch shapes"));
       memcpy(best_res, res, sizeof res);
-      if (best < 0.0 || ms < best) {
+      // The first candidate is the default (16 terms per lane, full residency); another must
+      // beat the best so far by 2% to replace it, so timing noise does not move the choice.
+      if (best < 0.0 || ms < best * 0.98) {
         best = ms;
         best_tpl = t;
         best_bpc = bpc_choices[b];
@@ -957,13 +953,13 @@ int clb_normal(clb_ctx* ctx, uint64_t n, uint64_t frac[3], char hex[49], uint64_
     shift[j] = kShift[j];
   }
   std::vector<GroupDev> groups{{0, 7}};
-  uint64_t res[3], slots[21];
-  if (int rc = run(ctx, jobs, shift, groups, res, nullptr, slots)) return rc;
+  uint64_t res[3];
   // Leave the device state as Allloop does: dev_ans = the answer, bigSum = formula 6's partial.
-  HIP_TRY(hipMemcpyAsync(ctx->ans, res, sizeof res, hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(hipMemcpyAsync(ctx->partial, slots + 18, 3 * sizeof(uint64_t), hipMemcpyHostToDevice,
-                         ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  Mirrors mirrors;
+  mirrors.total = ctx->ans;
+  mirrors.slot = ctx->partial;
+  mirrors.index = 6;
+  if (int rc = run(ctx, jobs, shift, groups, res, nullptr, nullptr, mirrors)) return rc;
   if (frac) memcpy(frac, res, sizeof res);
   if (hex) format_result(ctx->precision, res, hex);
   if (n_out) *n_out = nn;

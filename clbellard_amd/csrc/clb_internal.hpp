@@ -53,58 +53,58 @@ struct GroupDev {
   uint32_t count;
 };
 
-// Per-call header at offset 0 of the call's table block (header, job table, tail table, groups,
-// shifts, packed at the byte offsets below).  The kernels read counts and table offsets from it,
-// so one captured HIP graph serves every call while the block's address is unchanged.  The host
-// copies the block before each launch sequence, which also resets the tile queue.
+// Per-call header at offset 0 of the call's block.  The block holds, at the byte offsets below,
+// the job table, the tail table, the groups, the shifts, the accumulator (zeroed by the copy that
+// brings the block in) and room for the results.  The kernels read everything from the block, so
+// their only argument is its address, and one captured HIP graph serves every call while the
+// block is not reallocated.  The host copies the block before each launch sequence, which also
+// resets the tile queue.
 struct LaunchDev {
   unsigned long long counter;  // tile queue: tiles handed out in the dynamic phase
   uint64_t total_tiles;        // flat tiles of all jobs
-  uint64_t nzero;              // accumulator words to clear
   uint64_t static_tiles;       // static phase: wave w takes tiles [w T, (w + 1) T), T = this
+  uint64_t dst;                // device address of the group results (3 words each, MSW first)
+  uint64_t slot_out;           // 0, or the device address of every slot's shifted value
+  uint64_t mirror_total;       // 0, or a device address that also receives group 0's result
+  uint64_t mirror_slot;        // 0, or a device address that also receives slot mirror_index
+  uint32_t mirror_index;
+  uint32_t accumulate;         // 1: dst += result (the AddAns-style accumulation)
   uint32_t njobs;              // entries of the job table
   uint32_t chunk;              // consecutive tiles per queue grab in the dynamic phase
   uint32_t ntails;             // entries of the tail table
   uint32_t ngroups;            // results (groups of slots)
   uint32_t replicas;           // copies per accumulator slot (a power of 2, at most 64)
   uint32_t nslots;             // accumulator slots
-  uint32_t off_jobs, off_tails, off_groups, off_shifts;  // byte offsets in the block
+  uint32_t off_jobs, off_tails, off_groups, off_shifts, off_acc, off_out;  // byte offsets
 };
 
+__host__ __device__ inline const uint8_t* hdr_at(const LaunchDev* h, uint32_t off) {
+  return reinterpret_cast<const uint8_t*>(h) + off;
+}
 __host__ __device__ inline const JobDev* hdr_jobs(const LaunchDev* h) {
-  return reinterpret_cast<const JobDev*>(reinterpret_cast<const uint8_t*>(h) + h->off_jobs);
+  return reinterpret_cast<const JobDev*>(hdr_at(h, h->off_jobs));
 }
 __host__ __device__ inline const TailDev* hdr_tails(const LaunchDev* h) {
-  return reinterpret_cast<const TailDev*>(reinterpret_cast<const uint8_t*>(h) + h->off_tails);
+  return reinterpret_cast<const TailDev*>(hdr_at(h, h->off_tails));
 }
 __host__ __device__ inline const GroupDev* hdr_groups(const LaunchDev* h) {
-  return reinterpret_cast<const GroupDev*>(reinterpret_cast<const uint8_t*>(h) + h->off_groups);
+  return reinterpret_cast<const GroupDev*>(hdr_at(h, h->off_groups));
 }
 __host__ __device__ inline const uint8_t* hdr_shifts(const LaunchDev* h) {
-  return reinterpret_cast<const uint8_t*>(h) + h->off_shifts;
+  return hdr_at(h, h->off_shifts);
+}
+__host__ __device__ inline unsigned long long* hdr_acc(LaunchDev* h) {
+  return reinterpret_cast<unsigned long long*>(reinterpret_cast<uint8_t*>(h) + h->off_acc);
 }
 
-constexpr int kTailBlocks = 128;     // k_tail grid (one wave per block, grid-stride over tails)
-constexpr int kFinalizeBlocks = 64;  // k_finalize grid (64 threads per block, grid-stride)
-constexpr int kClearBlocks = 256;    // k_clear grid (256 threads per block, grid-stride)
-constexpr int kFoldBlocks = 256;     // k_fold grid (one wave per block, grid-stride over slots)
+constexpr int kCombineBlocks = 256;  // k_combine grid (one wave per block, grid-stride over groups)
 
-// Workgroups of k_head<kind> that fit on one compute unit at once (0 if unknown).
 int head_blocks_per_cu(int kind, int tpl_index);
 
-// Kernel launch wrappers (clb_kernels.hip).  All asynchronous on `stream`.
-// counter: one zeroed u64 the waves take chunks of `chunk` tiles from.
-// The launch sequence of one call: clear the accumulator words, the head terms (persistent grid of
-// `grid` workgroups), the tails, the combine (replicas folded, then the groups) into out (or
-// added into it when accumulate).
-// slot_out (optional) receives every slot's shifted value.  hdr is the device copy of the call's
-// table block.
-hipError_t launch_clear(unsigned long long* acc, const LaunchDev* hdr, hipStream_t stream);
-hipError_t launch_head(int kind, int tpl_index, LaunchDev* hdr, unsigned long long* acc,
-                       int grid, hipStream_t stream);
-hipError_t launch_tail(const LaunchDev* hdr, unsigned long long* acc, hipStream_t stream);
-hipError_t launch_finalize(unsigned long long* acc, const LaunchDev* hdr, uint64_t* out,
-                           int accumulate, uint64_t* slot_out, hipStream_t stream);
+// The launch sequence of one call: the head and tail terms (persistent grid of `grid`
+// workgroups), then the combine (slot replicas folded, shifted, summed per group).
+hipError_t launch_head(int kind, int tpl_index, LaunchDev* hdr, int grid, hipStream_t stream);
+hipError_t launch_combine(LaunchDev* hdr, hipStream_t stream);
 hipError_t launch_zero(unsigned long long* p, uint64_t nwords, hipStream_t stream);
 hipError_t launch_sum7(uint64_t* ans, uint64_t* partial, int shift, hipStream_t stream);
 

@@ -177,77 +177,6 @@ __device__ __forceinline__ void tile_terms(const JobDev& jb, uint64_t kbase, int
   if constexpr (KIND == kPlainWide) tile_plain<true, TPL>(jb, kbase, lane, acc, nterms);
 }
 
-// Persistent head-term kernel.  Flat tile f belongs to the last job with flat_begin <= f.  Wave w
-// first takes the static range [w T, (w + 1) T) of T = static_tiles consecutive tiles (no
-// atomics; small launches are all static), then chunks of `chunk` consecutive tiles after the
-// nwaves T static ones from a global counter (one atomic per chunk), so the launch balances
-// itself whatever a tile costs.  A wave's tiles only increase, so its job index only moves
-// forward.
-template <int KIND, int TPL>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8)))
-k_head(LaunchDev* __restrict__ hdr, unsigned long long* __restrict__ acc) {
-  const JobDev* __restrict__ jobs = hdr_jobs(hdr);
-  const int njobs = (int)hdr->njobs;
-  const uint64_t total_tiles = hdr->total_tiles;
-  const uint64_t chunk = hdr->chunk;
-  const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
-  // readfirstlane: the compiler does not know threadIdx.x / 64 is uniform, and a divergent tile
-  // index would move the job table loads and all per-tile values into vector registers.
-  const uint64_t wave_id = (uint64_t)blockIdx.x * (kBlock / kWave) +
-                           (uint64_t)__builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const uint64_t static_tiles = hdr->static_tiles;
-  const uint64_t dyn0 = static_tiles * nwaves;  // first tile of the dynamic phase
-  const int lane = (int)(threadIdx.x & (kWave - 1));
-  // Slot s has `replicas` copies; wave w adds into copy w mod replicas, which spreads the atomics
-  // of many short flushes over more addresses (atomics on one address serialise).
-  const uint32_t replicas = hdr->replicas;
-  const uint32_t rep = (uint32_t)wave_id & (replicas - 1);
-  int cur = -1;
-  JobDev jb{};
-  U192 sum = {0, 0, 0};
-  uint64_t nterms = 0;
-  auto tile = [&](uint64_t f) {
-    int lo = (cur < 0) ? 0 : cur, hi = njobs - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (jobs[mid].flat_begin <= f)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    if (lo != cur) {
-      if (cur >= 0)
-        flush_job(sum, nterms, jb.numesign, lane,
-                  acc + ((size_t)jb.slot * replicas + rep) * kAccWords);
-      cur = lo;
-      jb = jobs[lo];
-      sum = {0, 0, 0};
-      nterms = 0;
-    }
-    const uint64_t kbase = (jb.tile_first + (f - jb.flat_begin)) * tile_k(TPL);
-    tile_terms<KIND, TPL>(jb, kbase, lane, sum, nterms);
-  };
-  // One loop and one call site of tile(), so the term code is emitted once (two copies cost
-  // registers and occupancy).  All branches here are wave-uniform.
-  uint64_t f = wave_id * static_tiles;
-  uint64_t f_end = f + static_tiles < total_tiles ? f + static_tiles : total_tiles;
-  for (;;) {
-    if (f >= f_end) {
-      if (dyn0 >= total_tiles) break;
-      unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(&hdr->counter, (unsigned long long)chunk);
-      f = dyn0 + (((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32) |
-                  (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)base));
-      if (f >= total_tiles) break;
-      f_end = f + chunk < total_tiles ? f + chunk : total_tiles;
-    }
-    tile(f);
-    ++f;
-  }
-  if (cur >= 0)
-    flush_job(sum, nterms, jb.numesign, lane, acc + ((size_t)jb.slot * replicas + rep) * kAccWords);
-}
-
 // ------------------------------------------------------------------ tail terms
 
 // Tail k = d + i, i = lane < 20: value 2^(-10i)/m (Bellard.cl:1164-1239).  Term =
@@ -298,9 +227,10 @@ __device__ __forceinline__ U192 quot_pow2_bits(int E, uint64_t m) {
   return q;
 }
 
-__device__ __forceinline__ void tail_job(const TailDev tj, uint32_t replicas,
-                                         unsigned long long* __restrict__ acc) {
-  const int lane = (int)threadIdx.x;
+// One tail job on one wave: lane i < 20 takes k = d + i; the wave sum goes to copy `rep` of the
+// job's slot.
+__device__ __forceinline__ void tail_job(const TailDev tj, int lane, uint32_t replicas,
+                                         uint32_t rep, unsigned long long* __restrict__ acc) {
   U192 q = {0, 0, 0};
   if (lane < kTailTerms) {
     const uint64_t k = tj.d + (uint64_t)lane;
@@ -310,27 +240,87 @@ __device__ __forceinline__ void tail_job(const TailDev tj, uint32_t replicas,
     if ((tj.numesign + (uint32_t)(k & 1)) & 1u) q = {~q.w0, ~q.w1, ~q.w2};
   }
   wave_sum192(q);
-  if (lane == 0)
-    atomic_add192(acc + ((size_t)tj.slot * replicas + (blockIdx.x & (replicas - 1))) * kAccWords,
-                  q);
+  if (lane == 0) atomic_add192(acc + ((size_t)tj.slot * replicas + rep) * kAccWords, q);
 }
 
-// One wave per tail job, grid-stride over the tail table.
-__global__ void __launch_bounds__(kWave) k_tail(const LaunchDev* __restrict__ hdr,
-                                                unsigned long long* __restrict__ acc) {
-  const TailDev* __restrict__ tails = hdr_tails(hdr);
-  for (uint32_t t = blockIdx.x; t < hdr->ntails; t += gridDim.x)
-    tail_job(tails[t], hdr->replicas, acc);
+// Persistent head-term kernel; it also takes the tail jobs (wave w the tails w, w + nwaves, ...,
+// before its tiles).  Flat tile f belongs to the last job with flat_begin <= f.  Wave w first
+// takes the static range [w T, (w + 1) T) of T = static_tiles consecutive tiles (no atomics;
+// small launches are all static), then chunks of `chunk` consecutive tiles after the nwaves T
+// static ones from a global counter (one atomic per chunk), so the launch balances itself
+// whatever a tile costs.  A wave's tiles only increase, so its job index only moves forward.
+template <int KIND, int TPL>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8)))
+k_head(LaunchDev* __restrict__ hdr) {
+  unsigned long long* __restrict__ acc = hdr_acc(hdr);
+  const JobDev* __restrict__ jobs = hdr_jobs(hdr);
+  const int njobs = (int)hdr->njobs;
+  const uint64_t total_tiles = hdr->total_tiles;
+  const uint64_t chunk = hdr->chunk;
+  const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
+  // readfirstlane: the compiler does not know threadIdx.x / 64 is uniform, and a divergent tile
+  // index would move the job table loads and all per-tile values into vector registers.
+  const uint64_t wave_id = (uint64_t)blockIdx.x * (kBlock / kWave) +
+                           (uint64_t)__builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint64_t static_tiles = hdr->static_tiles;
+  const uint64_t dyn0 = static_tiles * nwaves;  // first tile of the dynamic phase
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  // Slot s has `replicas` copies; wave w adds into copy w mod replicas, which spreads the atomics
+  // of many short flushes over more addresses (atomics on one address serialise).
+  const uint32_t replicas = hdr->replicas;
+  const uint32_t rep = (uint32_t)wave_id & (replicas - 1);
+  {
+    const TailDev* __restrict__ tails = hdr_tails(hdr);
+    for (uint64_t t = wave_id; t < hdr->ntails; t += nwaves)
+      tail_job(tails[t], lane, replicas, rep, acc);
+  }
+  int cur = -1;
+  JobDev jb{};
+  U192 sum = {0, 0, 0};
+  uint64_t nterms = 0;
+  auto tile = [&](uint64_t f) {
+    int lo = (cur < 0) ? 0 : cur, hi = njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].flat_begin <= f)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    if (lo != cur) {
+      if (cur >= 0)
+        flush_job(sum, nterms, jb.numesign, lane,
+                  acc + ((size_t)jb.slot * replicas + rep) * kAccWords);
+      cur = lo;
+      jb = jobs[lo];
+      sum = {0, 0, 0};
+      nterms = 0;
+    }
+    const uint64_t kbase = (jb.tile_first + (f - jb.flat_begin)) * tile_k(TPL);
+    tile_terms<KIND, TPL>(jb, kbase, lane, sum, nterms);
+  };
+  // One loop and one call site of tile(), so the term code is emitted once (two copies cost
+  // registers and occupancy).  All branches here are wave-uniform.
+  uint64_t f = wave_id * static_tiles;
+  uint64_t f_end = f + static_tiles < total_tiles ? f + static_tiles : total_tiles;
+  for (;;) {
+    if (f >= f_end) {
+      if (dyn0 >= total_tiles) break;
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(&hdr->counter, (unsigned long long)chunk);
+      f = dyn0 + (((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32) |
+                  (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)base));
+      if (f >= total_tiles) break;
+      f_end = f + chunk < total_tiles ? f + chunk : total_tiles;
+    }
+    tile(f);
+    ++f;
+  }
+  if (cur >= 0)
+    flush_job(sum, nterms, jb.numesign, lane, acc + ((size_t)jb.slot * replicas + rep) * kAccWords);
 }
 
 // ------------------------------------------------------------------ combine
-
-__global__ void k_clear(unsigned long long* __restrict__ p, const LaunchDev* __restrict__ hdr) {
-  const uint64_t n = hdr->nzero;
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    p[i] = 0;
-}
 
 __global__ void k_zero(unsigned long long* p, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
@@ -338,29 +328,8 @@ __global__ void k_zero(unsigned long long* p, uint64_t n) {
     p[i] = 0;
 }
 
-// Sum of a slot's replicas into its first copy: one wave per slot, lane r loading copy r.
-__global__ void __launch_bounds__(kWave) k_fold(unsigned long long* __restrict__ acc,
-                                                const LaunchDev* __restrict__ hdr) {
-  const uint32_t replicas = hdr->replicas;
-  if (replicas == 1) return;
-  const int lane = (int)threadIdx.x;
-  for (uint32_t s = blockIdx.x; s < hdr->nslots; s += gridDim.x) {
-    unsigned long long* base = acc + (size_t)s * replicas * kAccWords;
-    uint64_t v[6];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) v[i] = (uint32_t)lane < replicas ? base[lane * kAccWords + i] : 0;
-#pragma unroll
-    for (int mask = 32; mask >= 1; mask >>= 1)
-#pragma unroll
-      for (int i = 0; i < 6; ++i) v[i] += shfl_xor64(v[i], mask);
-    if (lane == 0)
-#pragma unroll
-      for (int i = 0; i < 6; ++i) base[i] = v[i];
-  }
-}
-
-// Carry-propagate the six 32-bit limb sums of a slot into a 192-bit value.
-__device__ __forceinline__ U192 slot_value(const unsigned long long* s) {
+// Carry-propagate six 32-bit limb sums into a 192-bit value.
+__device__ __forceinline__ U192 limbs_value(const uint64_t (&s)[6]) {
   uint64_t limb[6];
   uint64_t c = 0;
   for (int i = 0; i < 6; ++i) {
@@ -376,36 +345,54 @@ __device__ __forceinline__ U192 shl192(U192 v, int sh) {
   return {v.w0 << sh, (v.w1 << sh) | (v.w0 >> (64 - sh)), (v.w2 << sh) | (v.w1 >> (64 - sh))};
 }
 
-__device__ __forceinline__ void finalize_group(const unsigned long long* __restrict__ acc,
-                                               uint32_t replicas,
-                                               const uint8_t* __restrict__ shift, GroupDev gr,
-                                               uint32_t g, uint64_t* __restrict__ out,
-                                               int accumulate, uint64_t* __restrict__ slot_out) {
-  U192 total = {0, 0, 0};
-  if (accumulate) total = {out[3 * g + 2], out[3 * g + 1], out[3 * g + 0]};
-  for (uint32_t i = 0; i < gr.count; ++i) {
-    const uint32_t s = gr.first + i;
-    const U192 v = shl192(slot_value(acc + (size_t)s * replicas * kAccWords), shift[s]);
-    if (slot_out) {
-      slot_out[3 * s + 0] = v.w2;
-      slot_out[3 * s + 1] = v.w1;
-      slot_out[3 * s + 2] = v.w0;
-    }
-    add192(total, v.w0, v.w1, v.w2);
-  }
-  out[3 * g + 0] = total.w2;
-  out[3 * g + 1] = total.w1;
-  out[3 * g + 2] = total.w0;
+__device__ __forceinline__ void store192(uint64_t* p, const U192& v) {  // MSW first
+  p[0] = v.w2;
+  p[1] = v.w1;
+  p[2] = v.w0;
 }
 
-__global__ void k_finalize(const unsigned long long* __restrict__ acc,
-                           const LaunchDev* __restrict__ hdr, uint64_t* __restrict__ out,
-                           int accumulate, uint64_t* __restrict__ slot_out) {
+// Sum7_192-style combine (Bellard.cl:1359-1398): per group, every slot's replicas summed (lane r
+// loads copy r), carried into 192 bits, shifted by log2 c_j and added to the group's total.
+// One wave per group, grid-stride over the groups.
+__global__ void __launch_bounds__(kWave) k_combine(LaunchDev* __restrict__ hdr) {
+  const unsigned long long* __restrict__ acc = hdr_acc(hdr);
   const GroupDev* __restrict__ groups = hdr_groups(hdr);
   const uint8_t* __restrict__ shift = hdr_shifts(hdr);
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < hdr->ngroups;
-       g += gridDim.x * blockDim.x)
-    finalize_group(acc, hdr->replicas, shift, groups[g], g, out, accumulate, slot_out);
+  uint64_t* dst = reinterpret_cast<uint64_t*>(hdr->dst);
+  uint64_t* slot_out = reinterpret_cast<uint64_t*>(hdr->slot_out);
+  const uint32_t replicas = hdr->replicas;
+  const int lane = (int)threadIdx.x;
+  for (uint32_t g = blockIdx.x; g < hdr->ngroups; g += gridDim.x) {
+    const GroupDev gr = groups[g];
+    U192 total = {0, 0, 0};
+    if (hdr->accumulate) total = {dst[3 * g + 2], dst[3 * g + 1], dst[3 * g + 0]};
+    for (uint32_t i = 0; i < gr.count; ++i) {
+      const uint32_t s = gr.first + i;
+      const unsigned long long* base = acc + (size_t)s * replicas * kAccWords;
+      uint64_t v[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        v[k] = (uint32_t)lane < replicas ? base[(size_t)lane * kAccWords + k] : 0;
+      if (replicas > 1) {
+#pragma unroll
+        for (int mask = 32; mask >= 1; mask >>= 1)
+#pragma unroll
+          for (int k = 0; k < 6; ++k) v[k] += shfl_xor64(v[k], mask);
+      }
+      const U192 val = shl192(limbs_value(v), shift[s]);
+      if (lane == 0) {
+        if (slot_out) store192(slot_out + 3 * (size_t)s, val);
+        if (hdr->mirror_slot && s == hdr->mirror_index)
+          store192(reinterpret_cast<uint64_t*>(hdr->mirror_slot), val);
+      }
+      add192(total, val.w0, val.w1, val.w2);
+    }
+    if (lane == 0) {
+      store192(dst + 3 * (size_t)g, total);
+      if (hdr->mirror_total && g == 0)
+        store192(reinterpret_cast<uint64_t*>(hdr->mirror_total), total);
+    }
+  }
 }
 
 __global__ void k_sum7(uint64_t* ans, uint64_t* partial, int shift) {
@@ -453,45 +440,35 @@ int head_blocks_per_cu(int kind, int tpl_index) {
 }
 
 template <int KIND>
-static hipError_t launch_head_tpl(int tpl_index, LaunchDev* hdr, unsigned long long* acc, int grid,
-                                  hipStream_t stream) {
+static hipError_t launch_head_tpl(int tpl_index, LaunchDev* hdr, int grid, hipStream_t stream) {
   switch (tpl_index) {
     case 0:
-      hipLaunchKernelGGL((k_head<KIND, kTplChoices[0]>), dim3(grid), dim3(kBlock), 0, stream, hdr,
-                         acc);
+      hipLaunchKernelGGL((k_head<KIND, kTplChoices[0]>), dim3(grid), dim3(kBlock), 0, stream, hdr);
       break;
     case 1:
-      hipLaunchKernelGGL((k_head<KIND, kTplChoices[1]>), dim3(grid), dim3(kBlock), 0, stream, hdr,
-                         acc);
+      hipLaunchKernelGGL((k_head<KIND, kTplChoices[1]>), dim3(grid), dim3(kBlock), 0, stream, hdr);
       break;
     case 2:
-      hipLaunchKernelGGL((k_head<KIND, kTplChoices[2]>), dim3(grid), dim3(kBlock), 0, stream, hdr,
-                         acc);
+      hipLaunchKernelGGL((k_head<KIND, kTplChoices[2]>), dim3(grid), dim3(kBlock), 0, stream, hdr);
       break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_head(int kind, int tpl_index, LaunchDev* hdr, unsigned long long* acc,
-                       int grid, hipStream_t stream) {
+hipError_t launch_head(int kind, int tpl_index, LaunchDev* hdr, int grid, hipStream_t stream) {
   switch (kind) {
-    case kSplitW3: return launch_head_tpl<kSplitW3>(tpl_index, hdr, acc, grid, stream);
-    case kSplitW2: return launch_head_tpl<kSplitW2>(tpl_index, hdr, acc, grid, stream);
-    case kSplitW1: return launch_head_tpl<kSplitW1>(tpl_index, hdr, acc, grid, stream);
-    case kPlainW3: return launch_head_tpl<kPlainW3>(tpl_index, hdr, acc, grid, stream);
-    case kPlainWide: return launch_head_tpl<kPlainWide>(tpl_index, hdr, acc, grid, stream);
+    case kSplitW3: return launch_head_tpl<kSplitW3>(tpl_index, hdr, grid, stream);
+    case kSplitW2: return launch_head_tpl<kSplitW2>(tpl_index, hdr, grid, stream);
+    case kSplitW1: return launch_head_tpl<kSplitW1>(tpl_index, hdr, grid, stream);
+    case kPlainW3: return launch_head_tpl<kPlainW3>(tpl_index, hdr, grid, stream);
+    case kPlainWide: return launch_head_tpl<kPlainWide>(tpl_index, hdr, grid, stream);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_tail(const LaunchDev* hdr, unsigned long long* acc, hipStream_t stream) {
-  hipLaunchKernelGGL(k_tail, dim3(kTailBlocks), dim3(kWave), 0, stream, hdr, acc);
-  return hipGetLastError();
-}
-
-hipError_t launch_clear(unsigned long long* acc, const LaunchDev* hdr, hipStream_t stream) {
-  hipLaunchKernelGGL(k_clear, dim3(kClearBlocks), dim3(256), 0, stream, acc, hdr);
+hipError_t launch_combine(LaunchDev* hdr, hipStream_t stream) {
+  hipLaunchKernelGGL(k_combine, dim3(kCombineBlocks), dim3(kWave), 0, stream, hdr);
   return hipGetLastError();
 }
 
@@ -500,16 +477,6 @@ hipError_t launch_zero(unsigned long long* p, uint64_t nwords, hipStream_t strea
   const uint64_t blocks = (nwords + 255) / 256;
   hipLaunchKernelGGL(k_zero, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0, stream,
                      p, nwords);
-  return hipGetLastError();
-}
-
-hipError_t launch_finalize(unsigned long long* acc, const LaunchDev* hdr, uint64_t* out,
-                           int accumulate, uint64_t* slot_out, hipStream_t stream) {
-  hipLaunchKernelGGL(k_fold, dim3(kFoldBlocks), dim3(kWave), 0, stream, acc, hdr);
-  const hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_finalize, dim3(kFinalizeBlocks), dim3(64), 0, stream, acc, hdr, out,
-                     accumulate, slot_out);
   return hipGetLastError();
 }
 

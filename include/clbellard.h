@@ -89,8 +89,10 @@ int clb_set_launch(clb_ctx *ctx, int blocks_per_cu);
 int clb_set_tiling(clb_ctx *ctx, int terms_per_lane, int min_tiles_per_wave);
 /* Autotuner (no reference counterpart): times the tile lengths 16, 4, 1 against 1, 3/4 and 1/2
  * of the resident workgroups per CU on a sample of the normal-mode computation at position n
- * (the seven formulas' head terms around d/2, sized to spend about budget_ms), keeps the fastest
- * on the context (as clb_set_tiling / clb_set_launch would) and returns it (outputs may be NULL).
+ * (the seven formulas' head terms around d/2, sized to spend about budget_ms; best of 3 runs
+ * each), keeps the fastest on the context (as clb_set_tiling / clb_set_launch would; a candidate
+ * must beat the default, 16 terms and full residency, by 2%) and returns it (outputs may be
+ * NULL).
  * Every candidate's sum is compared; a difference returns CLB_EDEVICE. */
 int clb_autotune(clb_ctx *ctx, uint64_t n, double budget_ms, int *terms_per_lane,
                  int *blocks_per_cu);
