@@ -125,7 +125,7 @@ struct clb_ctx {
   int num_cus = 0;
   char device_name[256] = {0};
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};  // around the head kernel (or the replayed graph)
   uint64_t* partial = nullptr;  // bigSumMem[0..2] (most significant word first)
   uint64_t* ans = nullptr;      // dev_ans[0..2]
   DevBuf<uint8_t> block;        // the call's block: header, tables, accumulator, results
@@ -339,15 +339,15 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
   LaunchDev* hdr = reinterpret_cast<LaunchDev*>(dev);
   hipStream_t s = ctx->stream;
   HIP_TRY(hipMemcpyAsync(dev, stage, off_out, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipEventRecord(ctx->ev[1], s));
+  HIP_TRY(hipEventRecord(ctx->ev[0], s));
   if (ctx->use_graph) {
     hipGraphExec_t exec = nullptr;
     if ((rc = graph_for(ctx, kind, tpl, (int)grid, &exec))) return rc;
     HIP_TRY(hipGraphLaunch(exec, s));
-    HIP_TRY(hipEventRecord(ctx->ev[2], s));
+    HIP_TRY(hipEventRecord(ctx->ev[1], s));
   } else {
     HIP_TRY(launch_head(kind, tpl, hdr, (int)grid, s));
-    HIP_TRY(hipEventRecord(ctx->ev[2], s));
+    HIP_TRY(hipEventRecord(ctx->ev[1], s));
     HIP_TRY(launch_combine(hdr, s));
   }
   const size_t back = accumulate_dev ? (slot_host ? 3 * nslots : 0) : out_words;
@@ -356,7 +356,7 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
                                    hipMemcpyDeviceToHost, s));
   hipError_t e2 = hipStreamSynchronize(s);
   float head = 0.f;
-  if (e2 == hipSuccess) e2 = hipEventElapsedTime(&head, ctx->ev[1], ctx->ev[2]);
+  if (e2 == hipSuccess) e2 = hipEventElapsedTime(&head, ctx->ev[0], ctx->ev[1]);
   if (e2 != hipSuccess)
     return fail(CLB_EDEVICE, "term launch failed: %s", hipGetErrorString(e2));
   const uint64_t* res = reinterpret_cast<const uint64_t*>(stage + off_out);
