@@ -144,6 +144,12 @@ def test_null_and_range_errors():
     assert clb.lib.clb_hex48(None, None) == -1
     assert clb.lib.clb_batch_calc(None, 10, 0, 10, 0) == -1
     assert clb.lib.clb_last_error()
+    # launch knobs: no context is an argument error, before any device work
+    assert clb.lib.clb_set_graph(None, 1) == -1
+    assert clb.lib.clb_set_tiling(None, 16, 0) == -1
+    assert clb.lib.clb_set_launch(None, 4) == -1
+    assert clb.lib.clb_autotune(None, 10**6, 100.0, None, None) == -1
+    assert b"context" in clb.lib.clb_last_error()
 
 
 def _wl(tmp_path, text, name="wl.txt"):
