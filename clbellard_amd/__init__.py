@@ -263,8 +263,8 @@ class Context:
         _check(lib.clb_set_launch(self._ctx, blocks_per_cu))
 
     def set_tiling(self, terms_per_lane: int = 0, min_tiles_per_wave: int = 0) -> None:
-        """Terms per lane per tile (16, 4, 1; 0 = chosen per launch by min_tiles_per_wave tiles
-        per resident wave, 0 = always 16)."""
+        """Terms per lane per tile (16, 4, 1; 0 = chosen per launch: the longest giving every
+        resident wave min_tiles_per_wave tiles, or with 0 one tile to every 8th wave)."""
         _check(lib.clb_set_tiling(self._ctx, terms_per_lane, min_tiles_per_wave))
 
     def autotune(self, n: int, budget_ms: float = 2000.0) -> Tuple[int, int]:

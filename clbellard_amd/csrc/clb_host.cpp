@@ -133,7 +133,7 @@ struct clb_ctx {
   float head_ms = 0.f, total_ms = 0.f;
   int resident[5][kNumTpl] = {};  // resident k_head workgroups per CU, per kind and tile length
   int tpl_index = -1;            // index into kTplChoices, or -1: chosen per launch
-  int min_tiles_per_wave = 0;    // launch-size threshold of that choice (0: always the longest)
+  int min_tiles_per_wave = 0;    // launch-size threshold of that choice (0: a tile per 8 waves)
   // Graph mode: the kernel sequence of a call, captured once per buffer layout and kernel choice
   // (every argument the kernels see), replayed afterwards.  Counts travel in hdr.
   bool use_graph = false;
@@ -245,7 +245,8 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
   if (nhead > 0x7fffffff || tails.size() > 0x7fffffff || groups.size() > 0x7fffffff)
     return fail(CLB_EINVAL, "too many jobs");
   // Terms per lane and grid: the longest tiles that still give every resident wave
-  // ctx->min_tiles_per_wave tiles to take (small launches spread over more, shorter tiles).
+  // ctx->min_tiles_per_wave tiles, or by default one tile to every 8th wave (only the smallest
+  // launches, e.g. N = 10^5, take shorter tiles: 4 terms per lane measured 11% faster there).
   int tpl = ctx->tpl_index;
   uint64_t grid = 0, flat = 0;
   for (int t = (tpl >= 0 ? tpl : 0); t < kNumTpl; ++t) {
@@ -255,9 +256,10 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
     for (const HostJob& h : hjobs)
       if (h.k0 < h.k1) flat += (h.k1 + tk - 1) / tk - h.k0 / tk;
     tpl = t;
-    if (ctx->tpl_index >= 0 ||
-        flat >= (uint64_t)ctx->min_tiles_per_wave * grid * (uint64_t)(kBlock / kWave))
-      break;
+    const uint64_t waves = grid * (uint64_t)(kBlock / kWave);
+    const uint64_t need =
+        ctx->min_tiles_per_wave > 0 ? (uint64_t)ctx->min_tiles_per_wave * waves : waves / 8;
+    if (ctx->tpl_index >= 0 || flat >= need) break;
   }
   const uint64_t tk = tile_k(kTplChoices[tpl]);
   flat = 0;

@@ -97,7 +97,8 @@ __host__ __device__ inline unsigned long long* hdr_acc(LaunchDev* h) {
   return reinterpret_cast<unsigned long long*>(reinterpret_cast<uint8_t*>(h) + h->off_acc);
 }
 
-constexpr int kCombineBlocks = 256;  // k_combine grid (one wave per block, grid-stride over groups)
+constexpr int kCombineBlocks = 256;  // k_combine grid (one block per group, grid-stride)
+constexpr int kCombineWaves = 8;     // k_combine block: waves sharing a group's slots
 
 int head_blocks_per_cu(int kind, int tpl_index);
 

@@ -352,21 +352,23 @@ __device__ __forceinline__ void store192(uint64_t* p, const U192& v) {  // MSW f
 }
 
 // Sum7_192-style combine (Bellard.cl:1359-1398): per group, every slot's replicas summed (lane r
-// loads copy r), carried into 192 bits, shifted by log2 c_j and added to the group's total.
-// One wave per group, grid-stride over the groups.
-__global__ void __launch_bounds__(kWave) k_combine(LaunchDev* __restrict__ hdr) {
+// loads copy r), carried into 192 bits, shifted by log2 c_j and added to the group's total.  One
+// block of kCombineWaves waves per group (grid-stride over the groups); wave w takes the group's
+// slots w, w + kCombineWaves, ..., and the waves' partial totals meet in LDS.
+__global__ void __launch_bounds__(kCombineWaves * kWave) k_combine(LaunchDev* __restrict__ hdr) {
+  __shared__ uint64_t part[kCombineWaves][3];
   const unsigned long long* __restrict__ acc = hdr_acc(hdr);
   const GroupDev* __restrict__ groups = hdr_groups(hdr);
   const uint8_t* __restrict__ shift = hdr_shifts(hdr);
   uint64_t* dst = reinterpret_cast<uint64_t*>(hdr->dst);
   uint64_t* slot_out = reinterpret_cast<uint64_t*>(hdr->slot_out);
   const uint32_t replicas = hdr->replicas;
-  const int lane = (int)threadIdx.x;
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  const uint32_t wave = threadIdx.x / kWave;
   for (uint32_t g = blockIdx.x; g < hdr->ngroups; g += gridDim.x) {
     const GroupDev gr = groups[g];
     U192 total = {0, 0, 0};
-    if (hdr->accumulate) total = {dst[3 * g + 2], dst[3 * g + 1], dst[3 * g + 0]};
-    for (uint32_t i = 0; i < gr.count; ++i) {
+    for (uint32_t i = wave; i < gr.count; i += kCombineWaves) {
       const uint32_t s = gr.first + i;
       const unsigned long long* base = acc + (size_t)s * replicas * kAccWords;
       uint64_t v[6];
@@ -388,10 +390,20 @@ __global__ void __launch_bounds__(kWave) k_combine(LaunchDev* __restrict__ hdr) 
       add192(total, val.w0, val.w1, val.w2);
     }
     if (lane == 0) {
-      store192(dst + 3 * (size_t)g, total);
-      if (hdr->mirror_total && g == 0)
-        store192(reinterpret_cast<uint64_t*>(hdr->mirror_total), total);
+      part[wave][0] = total.w0;
+      part[wave][1] = total.w1;
+      part[wave][2] = total.w2;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      U192 sum = {0, 0, 0};
+      if (hdr->accumulate) sum = {dst[3 * g + 2], dst[3 * g + 1], dst[3 * g + 0]};
+      for (int w = 0; w < kCombineWaves; ++w) add192(sum, part[w][0], part[w][1], part[w][2]);
+      store192(dst + 3 * (size_t)g, sum);
+      if (hdr->mirror_total && g == 0)
+        store192(reinterpret_cast<uint64_t*>(hdr->mirror_total), sum);
+    }
+    __syncthreads();
   }
 }
 
@@ -468,7 +480,8 @@ hipError_t launch_head(int kind, int tpl_index, LaunchDev* hdr, int grid, hipStr
 }
 
 hipError_t launch_combine(LaunchDev* hdr, hipStream_t stream) {
-  hipLaunchKernelGGL(k_combine, dim3(kCombineBlocks), dim3(kWave), 0, stream, hdr);
+  hipLaunchKernelGGL(k_combine, dim3(kCombineBlocks), dim3(kCombineWaves * kWave), 0, stream,
+                     hdr);
   return hipGetLastError();
 }
 

@@ -83,9 +83,10 @@ int clb_set_kernel(clb_ctx *ctx, int kernel);
 /* Launch shape: persistent workgroups per compute unit (0 = library default). */
 int clb_set_launch(clb_ctx *ctx, int blocks_per_cu);
 /* Tile length (no reference counterpart; the reference's work-items take GWS-strided terms,
- * KernelFuncs.as:38-52): terms per lane per tile, 16, 4 or 1; or 0 to choose per launch the
- * longest whose tiles give every resident wave at least min_tiles_per_wave tiles (0: always 16,
- * the default, which measured fastest at every size).  Results are identical for every choice. */
+ * KernelFuncs.as:38-52): terms per lane per tile, 16, 4 or 1; or 0 (default) to choose per
+ * launch the longest whose tiles give every resident wave at least min_tiles_per_wave tiles
+ * (0, the default: one tile to every 8th wave, so 16 except for the smallest launches).  Results
+ * are identical for every choice. */
 int clb_set_tiling(clb_ctx *ctx, int terms_per_lane, int min_tiles_per_wave);
 /* Autotuner (no reference counterpart): times the tile lengths 16, 4, 1 against 1, 3/4 and 1/2
  * of the resident workgroups per CU on a sample of the normal-mode computation at position n
