@@ -25,11 +25,14 @@ from typing import Iterable, List, Optional, Sequence, Tuple
 
 __all__ = [
     "ClbError", "lib", "LIB_PATH", "normalize", "formula", "hex48", "hex_dd", "u192_to_dd",
-    "work_sum", "parse_worklist", "write_work_file", "device_count", "device_name", "Context", "u192_to_int", "int_to_u192",
+    "work_sum", "parse_worklist", "write_work_file", "device_count", "device_name", "Context",
+    "u192_to_int", "int_to_u192",
+    "OK", "EINVAL", "EDEVICE", "ERANGE", "EIO", "ENOMEM", "EFORMAT",
     "PREC_DD106", "PREC_U192", "PREC_U192_BETA", "KERNEL_AUTO", "KERNEL_PLAIN", "KERNEL_SPLIT",
     "NUMER", "NUMESIGN", "DEN0", "DEN1",
 ]
 
+OK, EINVAL, EDEVICE, ERANGE, EIO, ENOMEM, EFORMAT = 0, -1, -2, -3, -4, -5, -6
 PREC_DD106, PREC_U192, PREC_U192_BETA = 0, 1, 2
 KERNEL_AUTO, KERNEL_PLAIN, KERNEL_SPLIT = 0, 1, 2
 

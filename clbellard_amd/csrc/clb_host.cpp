@@ -234,7 +234,15 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
   uint64_t max_m = 1;
   size_t nhead = 0;
   for (const HostJob& h : hjobs) {
-    if (h.tail) tails.push_back({h.d, h.den0, h.den1, h.numesign, h.slot});
+    // exponents 10 (d - k) must fit 64 bits; tail moduli den0 (d + 19) + den1 below 2^63 (the
+    // bit-serial division's bound)
+    if (h.d > UINT64_MAX / 10) return fail(CLB_ERANGE, "loop count %llu too large",
+                                           (unsigned long long)h.d);
+    if (h.tail) {
+      if (h.d + kTailTerms > ((1ull << 63) - h.den1) / h.den0)
+        return fail(CLB_ERANGE, "tail modulus of d = %llu exceeds 2^63", (unsigned long long)h.d);
+      tails.push_back({h.d, h.den0, h.den1, h.numesign, h.slot});
+    }
     if (h.k0 >= h.k1) continue;
     const uint64_t m = h.den0 * (h.k1 - 1) + h.den1;
     if (m > max_m) max_m = m;

@@ -299,3 +299,11 @@ def test_autotune_keeps_bits(oracle):
         assert w2i(c.normal(250000)[0]) == oracle.normal(250000)
         with clb.Context(0, clb.PREC_U192) as ref:
             assert c.normal(10**8) == ref.normal(10**8)
+
+
+def test_out_of_range_moduli_are_refused(ctx):
+    """Loop counts whose moduli leave the supported range fail before any launch (CLB_ERANGE)."""
+    with pytest.raises(clb.ClbError) as e:
+        ctx.run_jobs([(10**18 - 1, 10**18 - 10, 10**18, 2)])  # tail and head moduli near 10^19
+    assert e.value.code == clb.ERANGE
+    assert w2i(ctx.normal(1000)[0]) != 0  # the context stays usable
