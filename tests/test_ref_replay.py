@@ -199,3 +199,37 @@ def test_full_position_reference_kernels(ref, ctx, precision):
          f"{7 * (d + 20) / ours_s:.3e} term-formulas/s")
     assert ref_hex[:24] == hexs[:24]
     assert err <= dd192_bound(d + 20, sum(NUMER))
+
+
+def test_work_files_match_reference_kernels(ref, tmp_path):
+    """Work mode (Batchloop + fsave, Bellard_192.hsp:298-333) in precision 2 against the
+    reference kernels' job results (Set0mem, BatchCalc, AddAns, then bigSum as fsave writes it):
+    every job file whose range only Sglobal64mtg_192_Refresh covers is identical; the others are
+    within the double-double bound; the files sum (workmodeSum) to the normal-mode digits."""
+    n = 25_000_000_000
+    _, d = py_normalize(n)
+    batch = 1 << 30
+    nb = d // batch + 1
+    wl = tmp_path / "worklist.txt"
+    lines = [f"digits={n}", "precision=2"] + [f"{j}_{b}" for j in range(7) for b in range(nb)]
+    wl.write_bytes(("\r\n".join(lines) + "\r\n").encode())
+    work = str(tmp_path / "work")
+    with clb.Context(0, clb.PREC_U192_BETA) as c:
+        assert c.run_worklist(str(wl), work, batch_size=batch) == 7 * nb
+        digits = c.normal(n)[1]
+    exact = 0
+    for j in range(7):
+        for b in range(nb):
+            k0, k1 = b * batch, (b + 1) * batch
+            line0 = open(os.path.join(work, str(j), f"{j}_{b}.txt"), "rb").read().split(b"\r\n")[0]
+            ours = int(line0, 16)
+            theirs = w2i(ref.job(d, k0, k1, j, 2))
+            if k0 >= loopmax32(j) and k1 < d:
+                assert ours == theirs, (j, b)
+                exact += 1
+            else:
+                err = min((ours - theirs) % (1 << 192), (theirs - ours) % (1 << 192))
+                assert err <= dd192_bound(min(k1, d + 20) - k0, NUMER[j]), (j, b, err)
+    note(f"{exact} of {7 * nb} work files bit-identical to the reference kernels' jobs")
+    assert exact >= 7 * (nb - 3)
+    assert clb.work_sum(work) == (digits, [nb] * 7)
