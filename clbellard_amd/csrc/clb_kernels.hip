@@ -59,11 +59,14 @@ __device__ __forceinline__ void flush_job(U192 v, uint64_t nterms, uint32_t nume
 // kFast: every term of the tile lies in the job and has E >= 1664, and m >= 2^37 from the second
 // term on (nearly all tiles), so the term loop carries no range check, no exponent-form test, no
 // per-term count and no choice of reciprocal update.
-template <int W, bool kFast>
+template <int W, int TPL, bool kFast>
 __device__ __forceinline__ void split_terms(const JobDev& jb, uint64_t kbase, int lane, int nt,
                                             bool newton1, uint64_t E0, int tmax, double (&f)[4],
                                             uint32_t& corr, uint64_t& nterms) {
-  if (kFast) newton1 = true;
+  if (kFast) {
+    newton1 = true;
+    nt = TPL;  // a constant trip count keeps the loop test scalar
+  }
   const uint32_t ten_lane = 10u * (uint32_t)lane;
   double m = (double)(jb.den0 * (kbase + (uint64_t)lane) + jb.den1);
   const double mstep = (double)(jb.den0 * (uint64_t)kWave);
@@ -116,11 +119,11 @@ __device__ __forceinline__ void tile_split(const JobDev& jb, uint64_t kbase, int
   double f[4] = {0.0, 0.0, 0.0, 0.0};
   uint32_t corr = 0;
   if (full && big && newton1) {
-    split_terms<W, true>(jb, kbase, lane, nt, newton1, E0, tmax, f, corr, nterms);
+    split_terms<W, TPL, true>(jb, kbase, lane, nt, newton1, E0, tmax, f, corr, nterms);
     nterms += TPL;  // every term counted; k = 0 with den1 = 1 is the only m = 1
     if (kbase == 0 && lane == 0 && jb.den1 == 1) nterms -= 1;
   } else {
-    split_terms<W, false>(jb, kbase, lane, nt, newton1, E0, tmax, f, corr, nterms);
+    split_terms<W, TPL, false>(jb, kbase, lane, nt, newton1, E0, tmax, f, corr, nterms);
   }
   // Digit i has weight 2^(144 - 48 i).  f[0] may be negative (balanced residue); adding 2^52
   // keeps it non-negative and changes the sum by 2^196 = 0 mod 2^192.
