@@ -21,9 +21,15 @@ ARCH = os.environ.get("CLB_OFFLOAD_ARCH", "gfx950")
 # -ffp-contract=off: every fused multiply-add in the kernels is written explicitly; the exact
 # FP64 engine relies on the other products and sums being rounded on their own.
 # -packed-fp32-ops off: keep the compiler from emitting packed FP32 instructions.
+# Device code runs with the IEEE mode bit clear and FP64 denormals flushed: the hardware applies
+# output modifiers (the mul:2 of the doubling square) only then.  Neither matters otherwise here:
+# no value in the kernels is a NaN or a denormal (every one is an integer, 0, or 1/m for
+# m < 2^64), and clang accepts the IEEE mode change only with NaNs declared absent.
 HIP_FLAGS = [
     "-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Xarch_device", "-fdenormal-fp-math=preserve-sign",
+    "-Xarch_device", "-mno-amdgpu-ieee", "-Xarch_device", "-fno-honor-nans",
     "-Wall", "-Wno-unused-result",
 ]
 

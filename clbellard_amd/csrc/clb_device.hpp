@@ -57,6 +57,13 @@ __device__ __forceinline__ uint64_t f2u64(double f) {
   return ((uint64_t)(uint32_t)h << 32) | (uint64_t)(uint32_t)l;
 }
 
+// The two's complement of an exact integer |f| < 2^53 that may be negative.
+__device__ __forceinline__ uint64_t f2i64(double f) {
+  double h = __builtin_floor(f * 0x1p-32);
+  double l = __builtin_fma(h, -0x1p32, f);
+  return ((uint64_t)(int64_t)(int32_t)h << 32) | (uint64_t)(uint32_t)l;
+}
+
 // Reciprocal of m (an exact integer >= 1) to within about one ulp: hardware estimate plus two
 // Newton steps.
 __device__ __forceinline__ double recip(double m) {
@@ -112,14 +119,54 @@ __device__ __forceinline__ double fpow2(uint64_t e, int s, double m, double minv
   return x;
 }
 
+// x <- 2^(H's bits pos..1) applied by squaring: for pos = npos..1, x <- x^2 mod m, or 2 x^2 mod m
+// where bit pos of H is set.  fsq's six instructions; the doubling is the output modifier
+// (mul:2) of the final add, so it costs nothing, and bit pos is a scalar test and branch (H and
+// pos are wave-uniform).  The hardware applies an output modifier only with the IEEE mode bit
+// clear and FP64 denormals flushed, which the build sets for the device code (no value here is a
+// NaN or a denormal: every one is an integer, 0, or 1/m for m < 2^64).
+// Bounds, with |x| <= y m and the estimate of x^2 / m off by at most y^2 m 1.5 2^-52: for
+// m < 2^42, 2^46, 2^48 the result stays within y = 1.02, 1.06, 1.34 m, and fsq_shift (m 2^c <=
+// 2^50.25) on such an input returns at most 0.97, 1.0, 1.3 m; every intermediate is an exact
+// integer below 2^53.  The compiler folds no multiplication by 2.0 into an FP64 output modifier,
+// and a branch between two squares made it copy x between registers on every bit, hence one
+// block of assembly for the whole loop: ALU instructions and branches only, x kept in one
+// register, every output used, all paths leaving at the end.  Needs npos >= 1.
+__device__ __forceinline__ double fsq_bits(double x, uint64_t H, uint32_t npos, double m,
+                                           double minv) {
+  double hi, q;  // x's register takes lo, then the result
+  asm volatile(
+      "1:\n\t"
+      "s_bitcmp1_b64 %[H], %[pos]\n\t"
+      "v_mul_f64 %[hi], %[x], %[x]\n\t"
+      "v_fma_f64 %[x], %[x], %[x], -%[hi]\n\t"
+      "v_mul_f64 %[q], %[hi], %[minv]\n\t"
+      "v_rndne_f64 %[q], %[q]\n\t"
+      "v_fma_f64 %[q], -%[q], %[m], %[hi]\n\t"
+      "s_cbranch_scc1 2f\n\t"
+      "v_add_f64 %[x], %[q], %[x]\n\t"
+      "s_branch 3f\n"
+      "2:\n\t"
+      "v_add_f64 %[x], %[q], %[x] mul:2\n"
+      "3:\n\t"
+      "s_sub_u32 %[pos], %[pos], 1\n\t"
+      "s_cmp_lg_u32 %[pos], 0\n\t"
+      "s_cbranch_scc1 1b"
+      : [x] "+v"(x), [hi] "=&v"(hi), [q] "=&v"(q), [pos] "+s"(npos)
+      : [H] "s"(H), [m] "v"(m), [minv] "v"(minv)
+      : "scc");
+  return x;
+}
+
 // 2^(1024 H + v) mod m (balanced) for a wave-uniform H >= 1 and a per-lane v in [0, 2048): the
-// lanes of a wave have exponents E - 10 L, which this splits into a common high part H, whose
-// window digits are scalar, and a per-lane low part v.  Bit 10 of v is absorbed by the last
-// window of H (digit up to 2^W), then ten squares take the low ten bits of v.
-// Valid while m 2^(2^W) <= 2^50.  The leading bits t of H (everything above the lowest s bits, s
-// the smallest multiple of W that leaves t <= tmax) give the start value 2^t - rint(2^t / m) m in
-// one exact step: with 2^t / m <= 2^49 the quotient is off by at most 0.5 + 1.5 2^-3, so
-// |x| <= 0.69 m.  tmax (uniform) = 49 + floor(log2 m) of the smallest m the caller passes.
+// lanes of a wave have exponents E - 10 L, which this splits into a common high part H and a
+// per-lane low part v.  The leading bits t of H (everything above the lowest s bits, s the
+// smallest count that leaves t <= tmax) give the start value 2^t - rint(2^t / m) m in one exact
+// step: with 2^t / m <= 2^49 the quotient is off by at most 0.5 + 1.5 2^-3, so |x| <= 0.69 m;
+// tmax (uniform) = 49 + floor(log2 m) of the smallest m the caller passes.  H's other bits are
+// scalar, so each takes a plain or a doubling square by a scalar branch (fsq_bits), no ldexp; H's last bit
+// also absorbs bit 10 of v (a per-lane shift of at most 2); then W-bit windows take v's low ten
+// bits.  Valid while m 2^(2^W) <= 2^50.
 template <int W>
 __device__ __forceinline__ double fpow2_split(uint64_t H, uint32_t v, int tmax, double m,
                                               double minv) {
@@ -130,8 +177,8 @@ __device__ __forceinline__ double fpow2_split(uint64_t H, uint32_t v, int tmax, 
   const int bt = 32 - __builtin_clz((uint32_t)tmax);
   int s = 0;
   if (bl > bt || (bl == bt && (uint32_t)H >= (uint32_t)tmax)) {  // s = 0 needs H + 1 <= tmax
-    s = bl > bt + W ? W * ((bl - bt + W - 1) / W) : W;           // now H >> s < 2^bt
-    if ((uint32_t)(H >> s) > (uint32_t)tmax) s += W;
+    s = bl > bt ? bl - bt : 1;                                   // now H >> s < 2^bt
+    if ((uint32_t)(H >> s) > (uint32_t)tmax) s += 1;
   }
   const uint32_t t = (uint32_t)(H >> s);
   const int carry = (int)(v >> 10);
@@ -142,14 +189,8 @@ __device__ __forceinline__ double fpow2_split(uint64_t H, uint32_t v, int tmax, 
   } else {
     const double p = __builtin_bit_cast(double, (uint64_t)(1023 + t) << 52);  // 2^t, t uniform
     x = __builtin_fma(-__builtin_rint(p * minv), m, p);
-    for (int pos = s - W; pos > 0; pos -= W) {
-#pragma unroll
-      for (int i = 0; i < W - 1; ++i) x = fsq(x, m, minv);
-      x = fsq_shift(x, (int)((H >> pos) & mask), m, minv);
-    }
-#pragma unroll
-    for (int i = 0; i < W - 1; ++i) x = fsq(x, m, minv);
-    x = fsq_shift(x, (int)(H & mask) + carry, m, minv);
+    if (s > 1) x = fsq_bits(x, H, (uint32_t)(s - 1), m, minv);
+    x = fsq_shift(x, (int)(H & 1) + carry, m, minv);
   }
 #pragma unroll
   for (int pos = 10 - W; pos >= 0; pos -= W) {

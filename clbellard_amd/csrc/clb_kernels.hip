@@ -15,6 +15,17 @@
 
 namespace clb {
 
+// Work-item, work-group and grid sizes straight from the hardware registers and the dispatch
+// packet.  threadIdx and friends go through device-library functions built for the default FP
+// mode, which cannot be inlined into code built with the IEEE mode bit clear, and the calls would
+// cost registers and spills.
+__device__ __forceinline__ uint32_t thread_x() { return __builtin_amdgcn_workitem_id_x(); }
+__device__ __forceinline__ uint32_t block_x() { return __builtin_amdgcn_workgroup_id_x(); }
+__device__ __forceinline__ uint32_t block_dim_x() { return __builtin_amdgcn_workgroup_size_x(); }
+__device__ __forceinline__ uint32_t grid_dim_x() {  // every launch here is whole blocks
+  return __builtin_amdgcn_grid_size_x() / __builtin_amdgcn_workgroup_size_x();
+}
+
 __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int mask) {
   const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, mask, kWave);
   const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), mask, kWave);
@@ -125,12 +136,13 @@ __device__ __forceinline__ void tile_split(const JobDev& jb, uint64_t kbase, int
   } else {
     split_terms<W, TPL, false>(jb, kbase, lane, nt, newton1, E0, tmax, f, corr, nterms);
   }
-  // Digit i has weight 2^(144 - 48 i).  f[0] may be negative (balanced residue); adding 2^52
-  // keeps it non-negative and changes the sum by 2^196 = 0 mod 2^192.
+  // Digit i has weight 2^(144 - 48 i).  f[0] may be negative (the residue is balanced, |x| up to
+  // 1.34 m, so |f[0]| < 16 1.34 2^48 < 2^53): only its low 48 bits matter mod 2^192, and its two's
+  // complement carries them.  Digits 1..3 are in [0, 2^49).
   const uint64_t u3 = f2u64(f[3]) + corr;
   const uint64_t u2 = f2u64(f[2]);
   const uint64_t u1 = f2u64(f[1]);
-  const uint64_t u0 = f2u64(f[0] + 0x1p52);
+  const uint64_t u0 = f2i64(f[0]);
   add192(acc, u3, 0, 0);
   add192(acc, u2 << 48, u2 >> 16, 0);
   add192(acc, 0, u1 << 32, u1 >> 32);
@@ -260,14 +272,14 @@ k_head(LaunchDev* __restrict__ hdr) {
   const int njobs = (int)hdr->njobs;
   const uint64_t total_tiles = hdr->total_tiles;
   const uint64_t chunk = hdr->chunk;
-  const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
-  // readfirstlane: the compiler does not know threadIdx.x / 64 is uniform, and a divergent tile
+  const uint64_t nwaves = (uint64_t)grid_dim_x() * (kBlock / kWave);
+  // readfirstlane: the compiler does not know the work-item index / 64 is uniform, and a divergent tile
   // index would move the job table loads and all per-tile values into vector registers.
-  const uint64_t wave_id = (uint64_t)blockIdx.x * (kBlock / kWave) +
-                           (uint64_t)__builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint64_t wave_id = (uint64_t)block_x() * (kBlock / kWave) +
+                           (uint64_t)__builtin_amdgcn_readfirstlane(thread_x() / kWave);
   const uint64_t static_tiles = hdr->static_tiles;
   const uint64_t dyn0 = static_tiles * nwaves;  // first tile of the dynamic phase
-  const int lane = (int)(threadIdx.x & (kWave - 1));
+  const int lane = (int)(thread_x() & (kWave - 1));
   // Slot s has `replicas` copies; wave w adds into copy w mod replicas, which spreads the atomics
   // of many short flushes over more addresses (atomics on one address serialise).
   const uint32_t replicas = hdr->replicas;
@@ -326,8 +338,8 @@ k_head(LaunchDev* __restrict__ hdr) {
 // ------------------------------------------------------------------ combine
 
 __global__ void k_zero(unsigned long long* p, uint64_t n) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
+  for (uint64_t i = block_x() * (uint64_t)block_dim_x() + thread_x(); i < n;
+       i += (uint64_t)grid_dim_x() * block_dim_x())
     p[i] = 0;
 }
 
@@ -366,9 +378,9 @@ __global__ void __launch_bounds__(kCombineWaves * kWave) k_combine(LaunchDev* __
   uint64_t* dst = reinterpret_cast<uint64_t*>(hdr->dst);
   uint64_t* slot_out = reinterpret_cast<uint64_t*>(hdr->slot_out);
   const uint32_t replicas = hdr->replicas;
-  const int lane = (int)(threadIdx.x & (kWave - 1));
-  const uint32_t wave = threadIdx.x / kWave;
-  for (uint32_t g = blockIdx.x; g < hdr->ngroups; g += gridDim.x) {
+  const int lane = (int)(thread_x() & (kWave - 1));
+  const uint32_t wave = thread_x() / kWave;
+  for (uint32_t g = block_x(); g < hdr->ngroups; g += grid_dim_x()) {
     const GroupDev gr = groups[g];
     U192 total = {0, 0, 0};
     for (uint32_t i = wave; i < gr.count; i += kCombineWaves) {
@@ -398,7 +410,7 @@ __global__ void __launch_bounds__(kCombineWaves * kWave) k_combine(LaunchDev* __
       part[wave][2] = total.w2;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (thread_x() == 0) {
       U192 sum = {0, 0, 0};
       if (hdr->accumulate) sum = {dst[3 * g + 2], dst[3 * g + 1], dst[3 * g + 0]};
       for (int w = 0; w < kCombineWaves; ++w) add192(sum, part[w][0], part[w][1], part[w][2]);
@@ -411,7 +423,7 @@ __global__ void __launch_bounds__(kCombineWaves * kWave) k_combine(LaunchDev* __
 }
 
 __global__ void k_sum7(uint64_t* ans, uint64_t* partial, int shift) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (thread_x() != 0 || block_x() != 0) return;
   const U192 p = shl192({partial[2], partial[1], partial[0]}, shift);
   partial[0] = p.w2;
   partial[1] = p.w1;

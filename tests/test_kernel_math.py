@@ -1,6 +1,7 @@
 """CPU model of the fp64 engine's arithmetic (clb_device.hpp / clb_kernels.hip), with exactly
 rounded fma, checked against exact integers.  It pins the error analysis the kernels rely on:
-the balanced modular square stays exact and within 0.75 m, the lazy 48-bit quotient digits
+the balanced modular squares (plain, windowed, doubled) stay exact and within 1.02-1.34 m, the
+exponent chain gives 2^e mod m, the lazy 48-bit quotient digits and the per-lane digit sums
 reconstruct floor(r 2^192 / m), and the tail's long division of 2^E (whose first digit may
 estimate -1) gives floor(2^E / m).  The GPU tests check the kernels themselves."""
 import math
@@ -29,6 +30,15 @@ def f2u64(f: float) -> int:
     return (((int(h) & 0xFFFFFFFF) << 32) | (int(lo) & 0xFFFFFFFF)) & ((1 << 64) - 1)
 
 
+def f2i64(f: float) -> int:
+    """The kernel's conversion of a possibly negative digit sum: high half through a signed
+    32-bit conversion, low half unsigned."""
+    h = math.floor(f * 2.0**-32)
+    lo = fma(h, -(2.0**32), f)
+    assert -(2**31) <= h < 2**31 and 0 <= lo < 2**32
+    return (((int(h) & 0xFFFFFFFF) << 32) | int(lo)) & ((1 << 64) - 1)
+
+
 def fsq_shift(x: float, c: int, m: float, minv: float) -> float:
     y = math.ldexp(x, c)
     hi = x * y
@@ -38,16 +48,60 @@ def fsq_shift(x: float, c: int, m: float, minv: float) -> float:
     return t + lo
 
 
+def fsq2(x: float, m: float, minv: float) -> float:
+    """The doubling square of fsq_bits: fsq with the output modifier mul:2 on the final add."""
+    hi = x * x
+    lo = fma(x, x, -hi)
+    q = float(round(hi * minv))
+    t = fma(-q, m, hi)
+    return 2.0 * (t + lo)
+
+
+def fpow2_split(H: int, v: int, tmax: int, W: int, m: float, minv: float, ymax: float) -> float:
+    """The kernel's 2^(1024 H + v) mod m: start 2^t, H's bits by plain or doubling squares, H's
+    last bit with v's bit 10, then W-bit windows over v's low ten bits."""
+    bl, bt = H.bit_length(), tmax.bit_length()
+    s = 0
+    if bl > bt or (bl == bt and H >= tmax):
+        s = bl - bt if bl > bt else 1
+        if (H >> s) > tmax:
+            s += 1
+    t, carry = H >> s, v >> 10
+    assert t + (carry if s == 0 else 0) <= tmax
+    if s == 0:
+        p = math.ldexp(1.0, t + carry)
+        x = fma(-float(round(p * minv)), m, p)
+    else:
+        p = math.ldexp(1.0, t)
+        x = fma(-float(round(p * minv)), m, p)
+        for pos in range(s - 1, 0, -1):
+            x = fsq2(x, m, minv) if (H >> pos) & 1 else fsq_shift(x, 0, m, minv)
+            assert abs(x) <= ymax * m
+        x = fsq_shift(x, (H & 1) + carry, m, minv)
+    for pos in range(10 - W, -1, -W):
+        for _ in range(W - 1):
+            x = fsq_shift(x, 0, m, minv)
+        x = fsq_shift(x, (v >> pos) & ((1 << W) - 1), m, minv)
+        assert abs(x) <= ymax * m
+    rest = 10 % W
+    if rest:
+        for _ in range(rest - 1):
+            x = fsq_shift(x, 0, m, minv)
+        x = fsq_shift(x, v & ((1 << rest) - 1), m, minv)
+    return x
+
+
 def fdigit(rem: float, m: float, minv: float):
     n = math.ldexp(rem, 48)
     q = math.floor(fma(n, minv, -0.25))
     return float(q), fma(-q, m, n)
 
 
-def head_term(r: int, m: int, ulps: int) -> int:
-    """Quotient digits of the head terms: floor(r 2^192 / m) from a balanced residue r."""
+def head_term(x: int, m: int, ulps: int) -> int:
+    """Quotient digits of the head terms: floor(r 2^192 / m) mod 2^192, r = x mod m, from the
+    exponentiation's result x (|x| up to 1.34 m)."""
     mf, minv = float(m), recip(float(m), ulps)
-    rem = float(r - m if r > m // 2 else r)
+    rem = float(x)
     total = 0
     for i in range(3, -1, -1):
         q, rem = fdigit(rem, mf, minv)
@@ -74,25 +128,102 @@ def tail_quot(E: int, m: int, ulps: int) -> int:
     return q & M192
 
 
-@pytest.mark.parametrize("W,bits", [(3, 42), (2, 46), (1, 48)])
-def test_balanced_square_is_exact(W, bits):
+# (window width, modulus bits, bound on |x| / m kept by every step, bound after fsq_shift): with
+# |x| <= y m the estimate of x y' / m is off by at most y^2 (m 2^c) 1.5 2^-52, which fixes y.
+BOUNDS = [(3, 42, 1.02, 0.97), (2, 46, 1.06, 1.0), (1, 48, 1.34, 1.3)]
+
+
+@pytest.mark.parametrize("W,bits,ymax,yshift", BOUNDS)
+def test_balanced_square_is_exact(W, bits, ymax, yshift):
+    """Inputs up to ymax m (what a doubling square can leave): fsq_shift stays exact and within
+    yshift m, the doubling square within ymax m."""
     rng = random.Random(W)
-    for _ in range(3000):
+    for trial in range(3000):
         m = rng.randrange(1 << (bits - 8), 1 << bits) | 1
+        if trial % 10 == 0:
+            m = (1 << bits) - 1 - 2 * rng.randrange(1 << 8)  # the largest moduli of the range
         minv = recip(float(m), rng.choice((-1, 0, 1)))
-        x = rng.randrange(-(3 * m) // 4, (3 * m) // 4 + 1)
+        x = rng.randrange(-int(ymax * m), int(ymax * m) + 1)
+        if trial % 7 == 0:
+            x = int(ymax * m) * rng.choice((-1, 1))
         c = rng.randrange(0, (1 << W) + 1)  # window digit, plus the absorbed carry bit
         got = fsq_shift(float(x), c, float(m), minv)
-        assert abs(got) <= 0.75 * m
+        assert abs(got) <= yshift * m
         assert int(got) % m == (x * x << c) % m
+        got2 = fsq2(float(x), float(m), minv)
+        assert abs(got2) <= ymax * m
+        assert int(got2) % m == (2 * x * x) % m
+
+
+@pytest.mark.parametrize("W,bits,ymax,yshift", BOUNDS)
+def test_exponent_chain(W, bits, ymax, yshift):
+    """The whole split-form exponentiation against pow(2, 1024 H + v, m), including exponents
+    whose H is all ones (a doubling square on every bit)."""
+    rng = random.Random(100 + W)
+    for trial in range(400):
+        m = rng.randrange(1 << (bits - 10), 1 << bits) | 1
+        tmax = 49 + m.bit_length() - 1
+        H = rng.randrange(1, 1 << rng.randrange(1, 40))
+        if trial % 5 == 0:
+            H = (1 << rng.randrange(1, 40)) - 1
+        v = rng.randrange(10, 1664)
+        x = fpow2_split(H, v, tmax, W, float(m), recip(float(m), rng.choice((-1, 0, 1))), ymax)
+        assert abs(x) <= ymax * m
+        assert int(x) % m == pow(2, 1024 * H + v, m)
+
+
+def test_start_choice_is_minimal():
+    """s is the smallest shift with H >> s <= tmax (s = 0 only when H + 1 <= tmax, which leaves
+    room for v's bit 10)."""
+    for tmax in (49, 60, 87, 90, 96, 112):
+        for H in list(range(1, 600)) + [(1 << b) + d for b in range(10, 40) for d in (-1, 0, 1)]:
+            bl, bt = H.bit_length(), tmax.bit_length()
+            s = 0
+            if bl > bt or (bl == bt and H >= tmax):
+                s = bl - bt if bl > bt else 1
+                if (H >> s) > tmax:
+                    s += 1
+            want = 0 if H + 1 <= tmax else min(k for k in range(1, 64) if (H >> k) <= tmax)
+            assert s == want, (H, tmax)
+
+
+def tile_sum(xs, ms, ulps) -> int:
+    """One lane's tile: four fp64 digit sums over up to 16 terms, the integer count of lazy last
+    digits, converted (digit 0 as two's complement) and placed at 2^144, 2^96, 2^48, 2^0."""
+    f = [0.0, 0.0, 0.0, 0.0]
+    corr = 0
+    for x, m in zip(xs, ms):
+        mf, minv = float(m), recip(float(m), ulps)
+        rem = float(x)
+        for i in range(4):
+            q, rem = fdigit(rem, mf, minv)
+            f[i] += q
+        corr += rem >= mf
+    u3 = (f2u64(f[3]) + corr) & ((1 << 64) - 1)
+    u0 = f2i64(f[0])
+    return (u0 << 144) + (f2u64(f[1]) << 96) + (f2u64(f[2]) << 48) + u3 & M192
+
+
+def test_tile_digit_sums():
+    """16 terms per lane with residues anywhere in [-1.34 m, 1.34 m] (the widest the exponent
+    chain leaves), all at the extremes too: the converted sums equal the exact floor sum."""
+    rng = random.Random(12)
+    for trial in range(300):
+        ms = [rng.randrange(3, 1 << 48) | 1 for _ in range(16)]
+        if trial % 3 == 0:
+            xs = [int(1.34 * m) * (1 if trial % 2 else -1) for m in ms]
+        else:
+            xs = [rng.randrange(-int(1.34 * m), int(1.34 * m) + 1) for m in ms]
+        want = sum(((x % m) << 192) // m for x, m in zip(xs, ms)) & M192
+        assert tile_sum(xs, ms, rng.choice((-1, 0, 1))) == want, trial
 
 
 def test_quotient_digits_reconstruct_the_term():
     rng = random.Random(11)
     for _ in range(3000):
         m = rng.randrange(3, 1 << 48) | 1
-        r = rng.randrange(1, m)
-        assert head_term(r, m, rng.choice((-1, 0, 1))) == (r << 192) // m
+        x = rng.randrange(-int(1.34 * m), int(1.34 * m) + 1)
+        assert head_term(x, m, rng.choice((-1, 0, 1))) == ((x % m) << 192) // m
 
 
 def test_tail_long_division():
