@@ -8,8 +8,9 @@
 //  * fp64 engine ("split"): every intermediate is an exact integer held in a double.  A modular
 //    square uses the FMA error-free product (hi = x*y, lo = fma(x, y, -hi)), a quotient estimate
 //    from a double reciprocal of m, and an exact FMA residue.  Residues stay in balanced form
-//    |x| <= 0.75 m, so no correction step is needed inside the powering loop.  Valid while
-//    m * 2^c <= 2^50, c = the window's largest shift (checked by the host).
+//    |x| <= 1.02 m (up to 1.34 m for the widest moduli; fsq_bits gives the bounds), so no
+//    correction step is needed inside the powering loop.  Valid while m * 2^c <= 2^50, c = the
+//    window's largest shift (checked by the host).
 //  * int engine ("plain"): 64-bit Montgomery (R = 2^64) built from 32x32->64 products, lazy
 //    residues in [0, 2m), exponent windows folded into the REDC input, and the quotient words
 //    obtained by exact division (Q word = s * (-m^-1) mod 2^64 for s = r 2^(64i) mod m).
@@ -74,8 +75,9 @@ __device__ __forceinline__ double recip(double m) {
   return __builtin_fma(r, e, r);
 }
 
-// x * (x * 2^c) mod m in balanced form; x, the result: |.| <= 0.75 m.  Six FP64 operations plus
-// the scaling: hi + lo is the exact product, q its rounded quotient, t the exact residue of hi.
+// x * (x * 2^c) mod m in balanced form (|x| and the result within the bounds given at fsq_bits).
+// Six FP64 operations plus the scaling: hi + lo is the exact product, q its rounded quotient, t
+// the exact residue of hi.
 __device__ __forceinline__ double fsq(double x, double m, double minv) {
   double hi = x * x;
   double lo = __builtin_fma(x, x, -hi);
@@ -164,9 +166,9 @@ __device__ __forceinline__ double fsq_bits(double x, uint64_t H, uint32_t npos, 
 // smallest count that leaves t <= tmax) give the start value 2^t - rint(2^t / m) m in one exact
 // step: with 2^t / m <= 2^49 the quotient is off by at most 0.5 + 1.5 2^-3, so |x| <= 0.69 m;
 // tmax (uniform) = 49 + floor(log2 m) of the smallest m the caller passes.  H's other bits are
-// scalar, so each takes a plain or a doubling square by a scalar branch (fsq_bits), no ldexp; H's last bit
-// also absorbs bit 10 of v (a per-lane shift of at most 2); then W-bit windows take v's low ten
-// bits.  Valid while m 2^(2^W) <= 2^50.
+// scalar, so each takes a plain or a doubling square by a scalar branch (fsq_bits), no ldexp;
+// H's last bit also absorbs bit 10 of v (a per-lane shift of at most 2); then W-bit windows take
+// v's low ten bits.  Valid while m 2^(2^W) <= 2^50.
 template <int W>
 __device__ __forceinline__ double fpow2_split(uint64_t H, uint32_t v, int tmax, double m,
                                               double minv) {

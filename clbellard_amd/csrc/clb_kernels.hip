@@ -273,8 +273,9 @@ k_head(LaunchDev* __restrict__ hdr) {
   const uint64_t total_tiles = hdr->total_tiles;
   const uint64_t chunk = hdr->chunk;
   const uint64_t nwaves = (uint64_t)grid_dim_x() * (kBlock / kWave);
-  // readfirstlane: the compiler does not know the work-item index / 64 is uniform, and a divergent tile
-  // index would move the job table loads and all per-tile values into vector registers.
+  // readfirstlane: the compiler does not know the work-item index / 64 is uniform, and a
+  // divergent tile index would move the job table loads and all per-tile values into vector
+  // registers.
   const uint64_t wave_id = (uint64_t)block_x() * (kBlock / kWave) +
                            (uint64_t)__builtin_amdgcn_readfirstlane(thread_x() / kWave);
   const uint64_t static_tiles = hdr->static_tiles;
