@@ -108,5 +108,7 @@ hipError_t launch_head(int kind, int tpl_index, LaunchDev* hdr, int grid, hipStr
 hipError_t launch_combine(LaunchDev* hdr, hipStream_t stream);
 hipError_t launch_zero(unsigned long long* p, uint64_t nwords, hipStream_t stream);
 hipError_t launch_sum7(uint64_t* ans, uint64_t* partial, int shift, hipStream_t stream);
+// Writes 18 and 9 to out2[0..1] when the device code applies FP64 output modifiers.
+hipError_t launch_selfcheck(uint64_t* out2, hipStream_t stream);
 
 }  // namespace clb

@@ -426,6 +426,17 @@ __global__ void __launch_bounds__(kCombineWaves * kWave) k_combine(LaunchDev* __
   }
 }
 
+// Start-up check of the FP64 output modifier that the doubling squares rely on: one lane runs
+// fsq_bits on 3 mod 1000003 with the exponent bit set (2 * 3^2 = 18) and clear (3^2 = 9).  A
+// device build with the IEEE mode bit set ignores the modifier and returns 9 for both.
+__global__ void k_selfcheck(uint64_t* out) {
+  if (thread_x() != 0) return;
+  const double m = 1000003.0;
+  const double minv = recip(m);
+  out[0] = (uint64_t)(int64_t)fsq_bits(3.0, 2ull, 1u, m, minv);
+  out[1] = (uint64_t)(int64_t)fsq_bits(3.0, 1ull, 1u, m, minv);
+}
+
 __global__ void k_sum7(uint64_t* ans, uint64_t* partial, int shift) {
   if (thread_x() != 0 || block_x() != 0) return;
   const U192 p = shl192({partial[2], partial[1], partial[0]}, shift);
@@ -514,6 +525,11 @@ hipError_t launch_zero(unsigned long long* p, uint64_t nwords, hipStream_t strea
 
 hipError_t launch_sum7(uint64_t* ans, uint64_t* partial, int shift, hipStream_t stream) {
   hipLaunchKernelGGL(k_sum7, dim3(1), dim3(64), 0, stream, ans, partial, shift);
+  return hipGetLastError();
+}
+
+hipError_t launch_selfcheck(uint64_t* out2, hipStream_t stream) {
+  hipLaunchKernelGGL(k_selfcheck, dim3(1), dim3(64), 0, stream, out2);
   return hipGetLastError();
 }
 
