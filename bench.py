@@ -66,6 +66,18 @@ def algorithmic_fp64_slots(d):
     return 7 * (6 * squares + 16 * d)
 
 
+def cpu_model() -> str:
+    """The host CPU's model name and logical CPU count, as /proc/cpuinfo gives them."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            names = [ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")]
+        if names:
+            return f"{names[0]} ({len(names)} logical CPUs)"
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -330,7 +342,7 @@ def main():
             raise SystemExit("GPU and CPU oracle disagree on the sample window")
         cpu = {"value": sample / cpu_s, "unit": "terms/s", "cores": threads, "kind": "port",
                "sample": f"k in [{windows[0]}, {windows[0] + sample}) of N=10^12, all 7 formulas, "
-                         f"bit-identical to the GPU sum"}
+                         f"bit-identical to the GPU sum; {threads} threads on {cpu_model()}"}
         log(f"cpu oracle: {sample / cpu_s:.3e} terms/s on {threads} threads")
 
     out = {
