@@ -7,6 +7,7 @@
  */
 #include "clb_oracle.h"
 
+#include <math.h>
 #include <pthread.h>
 #include <stdio.h>
 #include <string.h>
@@ -337,6 +338,327 @@ int orc_normal(uint64_t n, int nthreads, uint64_t ans[3]) {
   for (int j = 0; j < 7; ++j) {
     uint64_t part[3];
     orc_batch_calc(d, 0, d, j, nthreads, part);
+    orc_sum7_192(ans, part, kNumer[j]);
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------- reference replay (dd) */
+
+/* The reference's 192-bit kernels of precision 1 and 2 keep, per work-item, two double-double
+ * sums (of r/m and of (r 2^96 mod m)/m) and convert them to 192 bits once; everything after is
+ * exact integer addition.  The functions below restate that double-double path operation by
+ * operation, including the fused multiply-adds that the reference's OpenCL build makes (OpenCL C
+ * contracts a*b + c inside one expression; each is marked), so that the result is the
+ * reference's to the bit.  The residues r are exact integers in the reference (Montgomery
+ * exponentiation, Bellard.cl:395-461), so they are computed here with orc_pow1024_mod. */
+
+typedef struct {
+  double hi, lo;
+} rdd;
+
+/* Bellard.cl:1-7 */
+static rdd rp_twosum(double a, double b) {
+  rdd r;
+  double x = a + b;
+  double tmp = x - a;
+  r.hi = x;
+  r.lo = (a - (x - tmp)) + (b - tmp);
+  return r;
+}
+
+/* Bellard.cl:9-15 (Veltkamp split by 2^27 + 1) */
+static rdd rp_dsplit(double a) {
+  rdd r;
+  double tmp = a * 134217729.0;
+  r.hi = tmp - (tmp - a);
+  r.lo = a - r.hi;
+  return r;
+}
+
+/* Bellard.cl:17-27; the error term's four products are fused (one expression). */
+static rdd rp_twoproduct(double a, double b) {
+  rdd r;
+  r.hi = a * b;
+  rdd ca = rp_dsplit(a), cb = rp_dsplit(b);
+  double e = fma(ca.hi, cb.hi, -r.hi);
+  e = fma(ca.lo, cb.hi, e);
+  e = fma(ca.hi, cb.lo, e);
+  r.lo = fma(ca.lo, cb.lo, e);
+  return r;
+}
+
+/* Bellard.cl:29-35 */
+static rdd rp_addcab(rdd a, rdd b) {
+  rdd c = rp_twosum(a.hi, b.hi);
+  c.lo = (c.lo + a.lo) + b.lo;
+  return rp_twosum(c.hi, c.lo);
+}
+
+static rdd rp_neg(rdd a) {
+  rdd r = {-a.hi, -a.lo};
+  return r;
+}
+
+/* Bellard.cl:99-106 (nn_div) */
+static rdd rp_nn_div(double x, double y) {
+  double z1 = x / y;
+  rdd c = rp_twoproduct(-z1, y);
+  double z2 = ((c.hi + x) + c.lo) / y;
+  return rp_twosum(z1, z2);
+}
+
+/* Bellard.cl:89-96 (nd_div); "- z1 * y.y" is fused with the sum before it. */
+static rdd rp_nd_div(double x, rdd y) {
+  double z1 = x / y.hi;
+  rdd c = rp_twoproduct(-z1, y.hi);
+  double z2 = (fma(-z1, y.lo, c.hi + x) + c.lo) / y.hi;
+  return rp_twosum(z1, z2);
+}
+
+/* Bellard.cl:70-77 (dd_div); fused as in nd_div. */
+static rdd rp_dd_div(rdd x, rdd y) {
+  double z1 = x.hi / y.hi;
+  rdd c = rp_twoproduct(-z1, y.hi);
+  double z2 = ((fma(-z1, y.lo, c.hi + x.hi) + x.lo) + c.lo) / y.hi;
+  return rp_twosum(z1, z2);
+}
+
+/* Bellard.cl:472-489 (sumfix2) */
+static void rp_sumfix2(rdd *s) {
+  double lo = 0.0;
+  if (s->hi >= 1.0) lo = -1.0;
+  if (s->hi <= -1.0) lo = 1.0;
+  rdd t = rp_twosum(s->hi, lo);
+  t.lo += s->lo;
+  *s = rp_twosum(t.hi, t.lo);
+}
+
+/* Bellard.cl:506-578 (dd2_to_ulong3): two double-double sums -> 192 bits, w[0] most significant:
+ * 64 bits from s0 (52 + 12 of the next 52), 40 + 24 more, then s1's 96 bits with the 2-bit
+ * overlap check at bits 30-31 / 46-47. */
+static void rp_dd2_to_ulong3(rdd s0, rdd s1, uint64_t w[3]) {
+  const rdd one = {1.0, 0.0};
+  if (s0.hi < 0.0) s0 = rp_addcab(s0, one);
+  if (s1.hi < 0.0) s1 = rp_addcab(s1, one);
+  uint64_t u0, u1, u2, ut;
+  double t;
+  s0.hi *= 4503599627370496.0; /* 2^52 */
+  s0.lo *= 4503599627370496.0;
+  t = trunc(s0.hi);
+  u0 = (uint64_t)t * 4096u;
+  s0.hi -= t;
+  s0 = rp_twosum(s0.hi, s0.lo);
+  s0.hi *= 4503599627370496.0;
+  if (s0.hi < 0.0) {
+    s0.hi += 4503599627370496.0;
+    u0 -= 4096u;
+  }
+  ut = (uint64_t)s0.hi;
+  u0 |= ut >> 40;
+  u1 = (ut % (1ull << 40)) << 24;
+  s1.hi *= 281474976710656.0; /* 2^48 */
+  s1.lo *= 281474976710656.0;
+  t = trunc(s1.hi);
+  ut = (uint64_t)t;
+  s1.hi -= t;
+  s1 = rp_twosum(s1.hi, s1.lo);
+  s1.hi *= 281474976710656.0;
+  if (s1.hi < 0.0) {
+    s1.hi += 281474976710656.0;
+    ut--;
+  }
+  u2 = (ut % (1ull << 16)) << 48;
+  u2 += (uint64_t)s1.hi;
+  const uint32_t chk0 = (uint32_t)((u1 >> 30) % 4u);
+  const uint32_t chk1 = (uint32_t)(ut >> 46);
+  if (chk0 != chk1) {
+    const uint64_t last = u1;
+    if (((chk0 + 1u) % 4u) == chk1) {
+      u1 += 1ull << 30;
+      if (u1 < last) u0++;
+    }
+    if (((chk0 - 1u) % 4u) == chk1) {
+      u1 -= 1ull << 30;
+      if (u1 > last) u0--;
+    }
+  }
+  u1 = (u1 & 0xFFFFFFFF00000000ull) | (ut >> 16);
+  w[0] = u0;
+  w[1] = u1;
+  w[2] = u2;
+}
+
+/* One term of Bellard.cl:783-805 / :1012-1069: the numerator is steered so that the sum stays
+ * in (-1, 1) (m - r and a negated quotient while the sum is positive). */
+static void rp_ddstep(rdd *s, uint64_t r, uint64_t m) {
+  const int pos = s->hi > 0.0;
+  rdd q = rp_nn_div((double)(pos ? m - r : r), (double)m);
+  if (pos) q = rp_neg(q);
+  *s = rp_addcab(*s, q);
+}
+
+void orc_ref_item192(uint64_t d, uint64_t offset, uint64_t k_max, uint64_t idx, uint64_t gsize,
+                     int j, uint64_t out[3]) {
+  rdd s0 = {0.0, 0.0}, s1 = {0.0, 0.0};
+  uint64_t k = offset + idx;
+  for (; k < k_max; k += gsize) {
+    const uint64_t m = kDen0[j] * k + kDen1[j];
+    const uint64_t r = orc_pow1024_mod(d - k, m);
+    uint64_t b = (uint64_t)(((u128)r << 64) % m); /* r 2^96 mod m, Bellard.cl:798-799, :1052-1053 */
+    b = (uint64_t)(((u128)b << 32) % m);
+    rp_ddstep(&s0, r, m);
+    rp_ddstep(&s1, b, m);
+  }
+  if (((uint64_t)kNumeSign[j] + k % 2) % 2 == 1) { /* Bellard.cl:807-810, :1075-1078 */
+    s0 = rp_neg(s0);
+    s1 = rp_neg(s1);
+  }
+  rp_dd2_to_ulong3(s0, s1, out);
+}
+
+/* 2^e mod m for e <= 96 by doubling (the tail's numerators, exact in the reference). */
+static uint64_t pow2_small(int e, uint64_t m) {
+  uint64_t x = 1 % m;
+  for (int i = 0; i < e; ++i) x = (uint64_t)(((u128)x << 1) % m);
+  return x;
+}
+
+void orc_ref_tail192(uint64_t d, int j, uint64_t out[3]) {
+  const uint64_t den0 = kDen0[j], den1 = kDen1[j];
+  rdd s0 = {0.0, 0.0}, s1 = {0.0, 0.0};
+  double n0 = 1.0, n1 = 79228162514264337593543950336.0; /* 2^96 */
+  for (int i = 0; i < 19; ++i) {
+    n0 /= 1024.0;
+    n1 /= 1024.0;
+  }
+  int64_t i = 19;
+  for (; i >= 0; --i) { /* Bellard.cl:1171-1185 */
+    const uint64_t m = den0 * (d + (uint64_t)i) + den1;
+    const rdd y = {(double)m, (double)((int64_t)m - (int64_t)(double)m)};
+    rdd q = rp_nd_div(n0, y);
+    if ((d + (uint64_t)i) % 2 == 1) q = rp_neg(q);
+    s0 = rp_addcab(s0, q);
+    n0 *= 1024.0;
+  }
+  for (i = 19; i >= 10; --i) { /* :1187-1199 */
+    const uint64_t m = den0 * (d + (uint64_t)i) + den1;
+    const rdd y = {(double)m, (double)((int64_t)m - (int64_t)(double)m)};
+    rdd q = rp_nd_div(n1, y);
+    if ((d + (uint64_t)i) % 2 == 1) q = rp_neg(q);
+    s1 = rp_addcab(s1, q);
+    n1 *= 1024.0;
+  }
+  for (; i >= 0; --i) { /* :1202-1239: numerators 2^(96 - 10 i) mod m, exact */
+    const uint64_t m = den0 * (d + (uint64_t)i) + den1;
+    const uint64_t nmr = pow2_small(96 - 10 * (int)i, m);
+    rdd q;
+    if (m > 4294967296ull) {
+      const rdd x = {(double)nmr, (double)((int64_t)nmr - (int64_t)(double)nmr)};
+      const rdd y = {(double)m, (double)((int64_t)m - (int64_t)(double)m)};
+      q = rp_dd_div(x, y);
+    } else {
+      q = rp_nn_div((double)nmr, (double)m);
+    }
+    if ((d + (uint64_t)i) % 2 == 1) q = rp_neg(q);
+    s1 = rp_addcab(s1, q);
+    rp_sumfix2(&s1);
+  }
+  if (kNumeSign[j] == 1) { /* :1241-1244 */
+    s0 = rp_neg(s0);
+    s1 = rp_neg(s1);
+  }
+  rp_dd2_to_ulong3(s0, s1, out);
+}
+
+struct item_job {
+  uint64_t d, offset, k_max, gsize, i0, i1, step;
+  int j;
+  uint64_t acc[3];
+};
+
+static void *item_worker(void *arg) {
+  struct item_job *jb = (struct item_job *)arg;
+  jb->acc[0] = jb->acc[1] = jb->acc[2] = 0;
+  for (uint64_t idx = jb->i0; idx < jb->i1; idx += jb->step) {
+    uint64_t w[3];
+    orc_ref_item192(jb->d, jb->offset, jb->k_max, idx, jb->gsize, jb->j, w);
+    orc_add192(jb->acc, w);
+  }
+  return NULL;
+}
+
+/* One launch of Sglobal32mtg_192 / Sglobal64mtg_192 with gsize work-items: the sum of its
+ * work-items' 192-bit values (the integer reductions that follow are exact). */
+static void ref_launch192(uint64_t d, uint64_t offset, uint64_t k_max, uint64_t gsize, int j,
+                          int nthreads, uint64_t out[3]) {
+  const uint64_t items = k_max - offset < gsize ? k_max - offset : gsize;
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  if ((uint64_t)nthreads > items) nthreads = (int)items;
+  struct item_job jobs[256];
+  pthread_t th[256];
+  for (int t = 0; t < nthreads; ++t) {
+    jobs[t].d = d;
+    jobs[t].offset = offset;
+    jobs[t].k_max = k_max;
+    jobs[t].gsize = gsize;
+    jobs[t].i0 = (uint64_t)t;
+    jobs[t].i1 = items;
+    jobs[t].step = (uint64_t)nthreads;
+    jobs[t].j = j;
+    if (nthreads == 1 || pthread_create(&th[t], NULL, item_worker, &jobs[t]) != 0) {
+      item_worker(&jobs[t]);
+      th[t] = 0;
+    }
+  }
+  for (int t = 0; t < nthreads; ++t) {
+    if (th[t]) pthread_join(th[t], NULL);
+    orc_add192(out, jobs[t].acc);
+  }
+}
+
+int orc_ref_batch192(uint64_t d, uint64_t k_start, uint64_t k_end, int j, int precision,
+                     uint64_t gsize, int nthreads, uint64_t out[3]) {
+  out[0] = out[1] = out[2] = 0;
+  if (j < 0 || j > 6 || (precision != 1 && precision != 2) || gsize == 0) return -1;
+  if (d * 10 + 9 >= 9007199254740992ull) return -1; /* -D OVERDOUBLE builds differ */
+  uint64_t k_max = k_start, offset = k_start;
+  uint64_t lm32 = ((1ull << 32) - kDen1[j] - 1) / kDen0[j] + 1; /* KernelFuncs.as:31-33 */
+  if (lm32 > d) lm32 = d;
+  if (lm32 > k_end) lm32 = k_end;
+  while (offset < lm32) { /* :34-41, GWS * 16 terms per launch */
+    k_max = k_max + gsize * 16 < lm32 ? k_max + gsize * 16 : lm32;
+    ref_launch192(d, offset, k_max, gsize, j, nthreads, out);
+    offset = k_max;
+  }
+  const uint64_t lm64 = k_end < d ? k_end : d; /* :45-53, GWS * 32 terms per launch */
+  if (precision == 2 && offset < lm64) {
+    uint64_t part[3]; /* Sglobal64mtg_192_Refresh: exact per term, chunking irrelevant */
+    orc_range_sum(d, offset, lm64, j, nthreads, part);
+    orc_add192(out, part);
+    offset = lm64;
+  }
+  while (offset < lm64) {
+    k_max = k_max + gsize * 32 < lm64 ? k_max + gsize * 32 : lm64;
+    ref_launch192(d, offset, k_max, gsize, j, nthreads, out);
+    offset = k_max;
+  }
+  if (offset == d) { /* :56-59 */
+    uint64_t t[3];
+    orc_ref_tail192(d, j, t);
+    orc_add192(out, t);
+  }
+  return 0;
+}
+
+int orc_ref_normal192(uint64_t n, int precision, uint64_t gsize, int nthreads, uint64_t ans[3]) {
+  uint64_t d;
+  ans[0] = ans[1] = ans[2] = 0;
+  if (orc_normalize(n, NULL, &d, NULL)) return -1;
+  for (int j = 0; j < 7; ++j) {
+    uint64_t part[3];
+    if (orc_ref_batch192(d, 0, d, j, precision, gsize, nthreads, part)) return -1;
     orc_sum7_192(ans, part, kNumer[j]);
   }
   return 0;

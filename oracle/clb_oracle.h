@@ -31,6 +31,11 @@
  *   orc_work_sum_add                    big-integer sum mod 16^48
  *   orc_hex_dd                          parity unpinned below 2^-100 (reference formatting of
  *                                       double-double values is only pinned on exact inputs)
+ *   orc_ref_item192, orc_ref_tail192,
+ *   orc_ref_batch192, orc_ref_normal192 the reference kernels' own results (GPU runs of
+ *                                       oracle/_ref, committed as tests/golden/ref_dd192.json):
+ *                                       bit-exact; and within the double-double bound of the
+ *                                       exact sums
  */
 #ifndef CLB_ORACLE_H
 #define CLB_ORACLE_H
@@ -91,6 +96,22 @@ void orc_batch_calc(uint64_t d, uint64_t k_start, uint64_t k_end, int j, int nth
                     uint64_t out[3]);
 /* Bellard_192.hsp:264-296 (Allloop): frac(16^(N'-1) * pi) as 192 bits. Returns orc_normalize's code. */
 int orc_normal(uint64_t n, int nthreads, uint64_t ans[3]);
+
+/* Reference replay of the precision-1 and -2 kernels (bit-exact with the reference's own results,
+ * double-double rounding included).  One work-item of Sglobal32mtg_192 (Bellard.cl:771-823) or
+ * Sglobal64mtg_192 (:992-1084, the build without -D OVERDOUBLE): terms k = offset + idx + i gsize
+ * < k_max, two double-double sums, dd2_to_ulong3 (:506-578). */
+void orc_ref_item192(uint64_t d, uint64_t offset, uint64_t k_max, uint64_t idx, uint64_t gsize,
+                     int j, uint64_t out[3]);
+/* Sglobal_after_192 (Bellard.cl:1153-1253), formula j at loop count d. */
+void orc_ref_tail192(uint64_t d, int j, uint64_t out[3]);
+/* BatchCalc (KernelFuncs.as:25-66) with precision 1 (Sglobal32mtg_192 + Sglobal64mtg_192) or 2
+ * (Sglobal32mtg_192 + the exact Refresh kernel), gsize work-items per launch (the reference
+ * launches 2^24).  Returns -1 for a bad argument or 10 d + 9 >= 2^53 (the OVERDOUBLE build). */
+int orc_ref_batch192(uint64_t d, uint64_t k_start, uint64_t k_end, int j, int precision,
+                     uint64_t gsize, int nthreads, uint64_t out[3]);
+/* Allloop (Bellard_192.hsp:264-296) on those kernels. */
+int orc_ref_normal192(uint64_t n, int precision, uint64_t gsize, int nthreads, uint64_t ans[3]);
 
 /* Bellard_192.hsp:354-359 + hexsademical.as:104-127: 48 lowercase hex digits, w[0] first. */
 void orc_hex48(const uint64_t w[3], char out[49]);

@@ -44,6 +44,16 @@ static char g_err[512];
 static double g_last_ms;
 
 const char *ref_last_error(void) { return g_err; }
+
+/* Work-items per term-kernel launch: the reference's GWS unless a test sets a smaller one (a
+ * multiple of LWS).  The kernels stride by get_global_size(0), so a smaller launch gives each
+ * work-item its 16 or 32 terms on a small range.  Returns -1 for a bad size. */
+static uint64_t g_gws = GWS;
+int ref_set_gws(uint64_t gws) {
+  if (gws == 0 || gws % LWS || gws > GWS) return -1;
+  g_gws = gws;
+  return 0;
+}
 double ref_last_ms(void) { return g_last_ms; }
 
 static int failf(const char *what, cl_int e) {
@@ -173,14 +183,14 @@ static int batch_calc(int p, uint64_t d, uint64_t k_start, uint64_t k_end, int j
   if (loopmax32 > d) loopmax32 = d;
   if (loopmax32 > k_end) loopmax32 = k_end;
   while (offset < loopmax32) {
-    k_max = k_max + (uint64_t)GWS * 16 < loopmax32 ? k_max + (uint64_t)GWS * 16 : loopmax32;
-    if (set_sglobal(g_k32[p], p, offset, d, k_max, j) || launch(g_k32[p], GWS, LWS)) return -1;
+    k_max = k_max + g_gws * 16 < loopmax32 ? k_max + g_gws * 16 : loopmax32;
+    if (set_sglobal(g_k32[p], p, offset, d, k_max, j) || launch(g_k32[p], g_gws, LWS)) return -1;
     offset = k_max;
   }
   const uint64_t loopmax64 = k_end < d ? k_end : d;
   while (offset < loopmax64) {
-    k_max = k_max + (uint64_t)GWS * 32 < loopmax64 ? k_max + (uint64_t)GWS * 32 : loopmax64;
-    if (set_sglobal(g_k64[p], p, offset, d, k_max, j) || launch(g_k64[p], GWS, LWS)) return -1;
+    k_max = k_max + g_gws * 32 < loopmax64 ? k_max + g_gws * 32 : loopmax64;
+    if (set_sglobal(g_k64[p], p, offset, d, k_max, j) || launch(g_k64[p], g_gws, LWS)) return -1;
     offset = k_max;
   }
   if (offset == d) {

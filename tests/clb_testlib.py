@@ -59,6 +59,11 @@ class Oracle:
             "orc_hex48": ([p64, ctypes.c_char_p], None),
             "orc_hex_dd": ([ctypes.POINTER(ctypes.c_double), ctypes.c_char_p], None),
             "orc_work_sum_add": ([ctypes.POINTER(ctypes.c_int), ctypes.c_char_p], None),
+            "orc_ref_item192": ([u64, u64, u64, u64, u64, ctypes.c_int, U3], None),
+            "orc_ref_tail192": ([u64, ctypes.c_int, U3], None),
+            "orc_ref_batch192": ([u64, u64, u64, ctypes.c_int, ctypes.c_int, u64, ctypes.c_int,
+                                  U3], ctypes.c_int),
+            "orc_ref_normal192": ([u64, ctypes.c_int, u64, ctypes.c_int, U3], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -116,6 +121,29 @@ class Oracle:
 
     def pow1024_mod(self, n, m):
         return self.lib.orc_pow1024_mod(n, m)
+
+    # -- replay of the reference's precision-1/2 kernels (double-double rounding included)
+    def ref_item192(self, d, offset, k_max, idx, gsize, j):
+        q = U3()
+        self.lib.orc_ref_item192(d, offset, k_max, idx, gsize, j, q)
+        return w2i(q)
+
+    def ref_tail192(self, d, j):
+        q = U3()
+        self.lib.orc_ref_tail192(d, j, q)
+        return w2i(q)
+
+    def ref_batch192(self, d, k0, k1, j, precision, gsize=1 << 24, threads=None):
+        q = U3()
+        rc = self.lib.orc_ref_batch192(d, k0, k1, j, precision, gsize, threads or self.threads, q)
+        assert rc == 0
+        return w2i(q)
+
+    def ref_normal192(self, n, precision, gsize=1 << 24, threads=None):
+        q = U3()
+        rc = self.lib.orc_ref_normal192(n, precision, gsize, threads or self.threads, q)
+        assert rc == 0
+        return w2i(q)
 
 
 def w2i(w) -> int:
@@ -230,6 +258,38 @@ def loopmax32(j: int) -> int:
     return ((1 << 32) - DEN1[j] - 1) // DEN0[j] + 1
 
 
+# Term ranges for the double-double replay of the reference's precision-1/2 kernels, as
+# (d, k_start, k_end, j, precision, work-items per launch).  The reference launches 2^24
+# work-items; smaller launches (its kernels stride by the global size) give every work-item its
+# full 16 or 32 terms on ranges the CPU restatement finishes in seconds.
+REF_DD_GOLDEN = os.path.join(GOLDEN, "ref_dd192.json")
+
+
+def ref_dd_cases():
+    d6, d7 = py_normalize(10**6)[1], py_normalize(10**7)[1]          # 399_999, 3_999_999
+    d10, d12 = py_normalize(25 * 10**9)[1], py_normalize(10**12)[1]  # 9_999_999_999, 4e11 - 1
+    cases = []
+    for j in range(7):
+        for p in (1, 2):
+            cases.append((d6, 0, d6, j, p, 1 << 24))  # one term per work-item, and the tail
+            cases.append((d6, 0, d6, j, p, 4096))     # 16 terms per work-item
+    for j in (0, 3, 6):
+        cases.append((d7, 0, d7, j, 1, 65536))
+        cases.append((d7, 12345, d7 // 2 + 7, j, 2, 2048))  # odd start, no tail
+    for j in range(7):
+        lo = loopmax32(j)
+        for p in (1, 2):
+            cases.append((d10, lo - 3001, lo + 9000, j, p, 256))  # both head kernels
+    for j in (1, 4):
+        cases.append((d10, d10 - 5000, d10, j, 1, 256))           # 64-bit kernel and the tail
+        cases.append((d10, d10 - 5000, d10, j, 2, 256))           # Refresh and the tail
+        cases.append((d12, d12 - 4000, d12 + 7, j, 1, 512))       # k_end past d
+        cases.append((d12, d12 // 2 | 1, d12 // 2 + 8192, j, 1, 256))
+        cases.append((d12, 0, 6000, j, 1, 256))                   # the smallest moduli
+    cases.append((d6, d6, d6, 2, 1, 1 << 24))                     # the tail alone
+    return cases
+
+
 class RefKernels:
     """ctypes view of oracle/ref_runner.c: the reference's OpenCL kernels (compiled from its
     sources into oracle/_ref/) driven through the OpenCL runtime with the reference's launch
@@ -247,6 +307,7 @@ class RefKernels:
         lib.ref_last_error.restype = ctypes.c_char_p
         lib.ref_last_ms.restype = ctypes.c_double
         lib.ref_shutdown.restype = None
+        lib.ref_set_gws.argtypes, lib.ref_set_gws.restype = [u64], c_int
         self.lib = lib
         self._check(lib.ref_init(co_dir.encode()))
 
@@ -270,6 +331,10 @@ class RefKernels:
         w = U3()
         self._check(self.lib.ref_job(d, k0, k1, j, precision, w))
         return self._value(w, precision)
+
+    def set_gws(self, gws: int = 1 << 24) -> None:
+        """Work-items per term-kernel launch (the reference launches 2^24)."""
+        self._check(self.lib.ref_set_gws(gws))
 
     def last_ms(self) -> float:
         return self.lib.ref_last_ms()

@@ -1,0 +1,50 @@
+"""CPU: the double-double replay of the reference's precision-1/2 kernels (orc_ref_batch192)
+against the reference kernels' own results in tests/golden/ref_dd192.json (written by
+tests/golden/make_ref_dd_fixtures.py on an MI355X with oracle/_ref), bit for bit, and the
+replay's distance from the exact sums within the double-double error the reference carries."""
+import json
+
+import pytest
+
+from clb_testlib import (M192, REF_DD_GOLDEN, SHIFT, load_oracle, py_batch_calc, py_normalize,
+                         ref_dd_cases)
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return load_oracle()
+
+
+def golden():
+    with open(REF_DD_GOLDEN) as f:
+        return json.load(f)["cases"]
+
+
+def test_fixture_covers_the_cases():
+    have = {(c["d"], c["k0"], c["k1"], c["j"], c["precision"], c["gws"]) for c in golden()}
+    assert have == set(ref_dd_cases())
+
+
+def case_id(c):
+    return "d{d}_k{k0}-{k1}_j{j}_p{precision}_g{gws}".format(**c)
+
+
+@pytest.mark.parametrize("case", golden(), ids=case_id)
+def test_replay_equals_reference_kernels(orc, case):
+    d, k0, k1, j, p, gws = (case[k] for k in ("d", "k0", "k1", "j", "precision", "gws"))
+    want = int(case["part"], 16)
+    got = (orc.ref_batch192(d, k0, k1, j, p, gws) << SHIFT[j]) & M192
+    assert got == want, f"{got:048x} != {want:048x}"
+
+
+@pytest.mark.parametrize("gws", [1 << 24, 4096, 256])
+def test_replay_close_to_exact(orc, gws):
+    """Each work-item's conversion truncates at bit 192 and its sums round near bit 202, so the
+    replay lies within 2 units of 2^-192 per term of the exact sum (measured: about 0.5 per term
+    when work-items hold 16 terms, 0 for most single-term work-items)."""
+    d = py_normalize(10**5)[1]
+    for j in range(7):
+        exact = py_batch_calc(d, 0, d, j)
+        diff = (orc.ref_batch192(d, 0, d, j, 1, gws) - exact) & M192
+        diff = diff - (1 << 192) if diff >> 191 else diff
+        assert abs(diff) <= 2 * (d + 20), (j, diff)
