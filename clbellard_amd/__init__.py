@@ -29,12 +29,13 @@ __all__ = [
     "u192_to_int", "int_to_u192",
     "OK", "EINVAL", "EDEVICE", "ERANGE", "EIO", "ENOMEM", "EFORMAT",
     "PREC_DD106", "PREC_U192", "PREC_U192_BETA", "KERNEL_AUTO", "KERNEL_PLAIN", "KERNEL_SPLIT",
-    "NUMER", "NUMESIGN", "DEN0", "DEN1",
+    "NUMER", "NUMESIGN", "DEN0", "DEN1", "REF_GLOBAL_WORK_SIZE",
 ]
 
 OK, EINVAL, EDEVICE, ERANGE, EIO, ENOMEM, EFORMAT = 0, -1, -2, -3, -4, -5, -6
 PREC_DD106, PREC_U192, PREC_U192_BETA = 0, 1, 2
 KERNEL_AUTO, KERNEL_PLAIN, KERNEL_SPLIT = 0, 1, 2
+REF_GLOBAL_WORK_SIZE = 65536 * 256  # the reference's work-items per launch, Bellard_192.hsp:5-7
 
 # KernelFuncs.as:9-12
 NUMER = (32, 1, 256, 64, 4, 4, 1)
@@ -77,6 +78,7 @@ _SIGNATURES = {
     "clb_set_kernel": ([_CTX, ctypes.c_int], ctypes.c_int),
     "clb_set_launch": ([_CTX, ctypes.c_int], ctypes.c_int),
     "clb_set_graph": ([_CTX, ctypes.c_int], ctypes.c_int),
+    "clb_set_replay": ([_CTX, _U64], ctypes.c_int),
     "clb_set_tiling": ([_CTX, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "clb_autotune": ([_CTX, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(ctypes.c_int),
                       ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
@@ -280,6 +282,12 @@ class Context:
     def set_graph(self, enable: bool) -> None:
         """Replay each call's kernel sequence as a captured HIP graph (identical results)."""
         _check(lib.clb_set_graph(self._ctx, 1 if enable else 0))
+
+    def set_replay(self, global_work_size: int = 0) -> None:
+        """Reproduce the reference's precision-1/2 kernels bit for bit, double-double rounding
+        included, as launched with global_work_size work-items (REF_GLOBAL_WORK_SIZE is the
+        reference's); 0 = exact evaluation (the default)."""
+        _check(lib.clb_set_replay(self._ctx, global_work_size))
 
     # --- KernelFuncs.as surface
     def set0mem(self) -> None:

@@ -9,6 +9,7 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -25,13 +26,17 @@ ARCH = os.environ.get("CLB_OFFLOAD_ARCH", "gfx950")
 # output modifiers (the mul:2 of the doubling square) only then.  Neither matters otherwise here:
 # no value in the kernels is a NaN or a denormal (every one is an integer, 0, or 1/m for
 # m < 2^64), and clang accepts the IEEE mode change only with NaNs declared absent.
-HIP_FLAGS = [
+BASE_FLAGS = [
     "-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Xarch_device", "-fdenormal-fp-math=preserve-sign",
-    "-Xarch_device", "-mno-amdgpu-ieee", "-Xarch_device", "-fno-honor-nans",
     "-Wall", "-Wno-unused-result",
 ]
+HIP_FLAGS = BASE_FLAGS + [
+    "-Xarch_device", "-fdenormal-fp-math=preserve-sign",
+    "-Xarch_device", "-mno-amdgpu-ieee", "-Xarch_device", "-fno-honor-nans",
+]
+# clb_replay.hip reproduces the reference's double-double rounding, so it keeps the default FP
+# mode (IEEE mode on, denormals kept) and is compiled on its own with BASE_FLAGS.
 
 
 def _hipcc() -> str:
@@ -52,9 +57,16 @@ def _run(cmd) -> None:
 
 
 def build_library() -> str:
-    srcs = [os.path.join(CSRC, f) for f in ("clb_kernels.hip", "clb_host.cpp")]
+    units = [("clb_kernels.hip", HIP_FLAGS), ("clb_host.cpp", HIP_FLAGS),
+             ("clb_replay.hip", BASE_FLAGS)]
     tmp = LIB + ".tmp"
-    _run([_hipcc(), *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
+    with tempfile.TemporaryDirectory() as work:
+        objs = []
+        for src, flags in units:
+            obj = os.path.join(work, os.path.splitext(src)[0] + ".o")
+            _run([_hipcc(), *flags, "-c", "-o", obj, os.path.join(CSRC, src)])
+            objs.append(obj)
+        _run([_hipcc(), f"--offload-arch={ARCH}", "-fPIC", "-shared", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     return LIB
 

@@ -137,6 +137,9 @@ struct clb_ctx {
   // Graph mode: the kernel sequence of a call, captured once per buffer layout and kernel choice
   // (every argument the kernels see), replayed afterwards.  Counts travel in hdr.
   bool use_graph = false;
+  // Reference replay: 0, or the work-items per replayed launch of the reference's kernels.
+  uint64_t replay_gws = 0;
+  int replay_resident = -1;  // resident k_replay workgroups per CU (-1: not queried yet)
   struct Graph {
     uintptr_t key[4];
     hipGraphExec_t exec;
@@ -218,6 +221,65 @@ struct Mirrors {
   uint32_t index = 0;
 };
 
+// Reference replay of the BatchCalc jobs `in` (KernelFuncs.as:25-66 with ctx->replay_gws
+// work-items per launch): the launches of the reference's double-double kernels (GWS * 16 terms
+// each below loopmax32, GWS * 32 above for precision 1), precision 2's range past loopmax32 as an
+// exact job (its Refresh kernel is exact per term, so the chunking does not matter), and the tail
+// kernel where the range reaches d.  Replay tiles are 64 work-items; *rtiles receives their count.
+int expand_replay(const clb_ctx* ctx, const std::vector<HostJob>& in, std::vector<HostJob>* exact,
+                  std::vector<ReplayJobDev>* rjobs, std::vector<TailDev>* rtails,
+                  uint64_t* rtiles) {
+  const uint64_t g = ctx->replay_gws;
+  uint64_t flat = 0;
+  for (const HostJob& h : in) {
+    if (h.d > ((1ull << 53) - 10) / 10)  // Bellard_192.hsp:472: 10 d + 9 >= 2^53
+      return fail(CLB_ERANGE, "replay: d = %llu needs the reference's OVERDOUBLE build, which "
+                  "is not replayed", (unsigned long long)h.d);
+    uint64_t offset = h.k0, k_max = h.k0;
+    auto launch = [&](uint64_t end) {
+      ReplayJobDev r{};
+      r.d = h.d;
+      r.offset = offset;
+      r.k_max = end;
+      r.gsize = g;
+      r.flat_begin = flat;
+      r.den0 = h.den0;
+      r.den1 = h.den1;
+      r.numesign = h.numesign;
+      r.slot = h.slot;
+      const uint64_t items = end - offset < g ? end - offset : g;
+      flat += (items + kWave - 1) / kWave;
+      rjobs->push_back(r);
+    };
+    if (h.k0 < h.k1) {
+      const uint64_t x = ((1ull << 32) - h.den1 - 1) / h.den0 + 1;  // KernelFuncs.as:31-33
+      const uint64_t lm32 = x < h.k1 ? x : h.k1;
+      while (offset < lm32) {  // :34-41
+        k_max = k_max + g * 16 < lm32 ? k_max + g * 16 : lm32;
+        launch(k_max);
+        offset = k_max;
+      }
+      if (ctx->precision == CLB_PREC_U192_BETA) {
+        if (offset < h.k1) {
+          HostJob e = h;
+          e.k0 = offset;
+          e.tail = false;
+          exact->push_back(e);
+        }
+      } else {
+        while (offset < h.k1) {  // :45-53
+          k_max = k_max + g * 32 < h.k1 ? k_max + g * 32 : h.k1;
+          launch(k_max);
+          offset = k_max;
+        }
+      }
+    }
+    if (h.tail) rtails->push_back({h.d, h.den0, h.den1, h.numesign, h.slot});  // :56-59
+  }
+  *rtiles = flat;
+  return CLB_OK;
+}
+
 // Runs head jobs + tails into slots [0, nslots), then combines the slots by groups.
 // out_host receives 3 words per group unless accumulate_dev is given, in which case the single
 // group's value is added into that device buffer.  slot_host (optional) receives each slot's
@@ -231,9 +293,17 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
   const size_t nslots = shift.size();
   std::vector<JobDev> jobs;
   std::vector<TailDev> tails;
+  // Reference replay: the head jobs and tails below are then only the exact part.
+  std::vector<HostJob> exact_jobs;
+  std::vector<ReplayJobDev> rjobs;
+  std::vector<TailDev> rtails;
+  uint64_t rtiles = 0;
+  const bool replay = ctx->replay_gws != 0;
+  if (replay && (rc = expand_replay(ctx, hjobs, &exact_jobs, &rjobs, &rtails, &rtiles))) return rc;
+  const std::vector<HostJob>& head_jobs = replay ? exact_jobs : hjobs;
   uint64_t max_m = 1;
   size_t nhead = 0;
-  for (const HostJob& h : hjobs) {
+  for (const HostJob& h : head_jobs) {
     // exponents 10 (d - k) must fit 64 bits; tail moduli den0 (d + 19) + den1 below 2^63 (the
     // bit-serial division's bound)
     if (h.d > UINT64_MAX / 10) return fail(CLB_ERANGE, "loop count %llu too large",
@@ -261,7 +331,7 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
     grid = (uint64_t)ctx->num_cus * (uint64_t)blocks_per_cu(ctx, kind, t);
     const uint64_t tk = tile_k(kTplChoices[t]);
     flat = 0;
-    for (const HostJob& h : hjobs)
+    for (const HostJob& h : head_jobs)
       if (h.k0 < h.k1) flat += (h.k1 + tk - 1) / tk - h.k0 / tk;
     tpl = t;
     const uint64_t waves = grid * (uint64_t)(kBlock / kWave);
@@ -271,7 +341,7 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
   }
   const uint64_t tk = tile_k(kTplChoices[tpl]);
   flat = 0;
-  for (const HostJob& h : hjobs) {
+  for (const HostJob& h : head_jobs) {
     if (h.k0 >= h.k1) continue;
     JobDev j{};
     j.d = h.d;
@@ -296,7 +366,9 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
   auto align64 = [](size_t x) { return (x + 63) & ~(size_t)63; };
   const size_t off_jobs = align64(sizeof(LaunchDev));
   const size_t off_tails = off_jobs + align64(jobs.size() * sizeof(JobDev));
-  const size_t off_groups = off_tails + align64(tails.size() * sizeof(TailDev));
+  const size_t off_rjobs = off_tails + align64(tails.size() * sizeof(TailDev));
+  const size_t off_rtails = off_rjobs + align64(rjobs.size() * sizeof(ReplayJobDev));
+  const size_t off_groups = off_rtails + align64(rtails.size() * sizeof(TailDev));
   const size_t off_shifts = off_groups + align64(groups.size() * sizeof(GroupDev));
   const size_t off_acc = off_shifts + align64(nslots);
   const size_t off_out = off_acc + nslots * replicas * kAccWords * sizeof(uint64_t);
@@ -339,10 +411,31 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
   h.off_shifts = (uint32_t)off_shifts;
   h.off_acc = (uint32_t)off_acc;
   h.off_out = (uint32_t)off_out;
+  int rgrid = 0;
+  if (replay) {
+    if (ctx->replay_resident < 0) ctx->replay_resident = replay_blocks_per_cu();
+    if (ctx->replay_resident <= 0) return fail(CLB_EDEVICE, "replay kernel: no occupancy");
+    rgrid = ctx->num_cus * ctx->replay_resident;
+    const uint64_t rwaves = (uint64_t)rgrid * (kBlock / kWave);
+    const uint64_t per = rtiles / (8 * rwaves);
+    if (rjobs.size() > 0x7fffffff || rtails.size() > 0x7fffffff)
+      return fail(CLB_EINVAL, "too many replayed launches");
+    h.rcounter = 0;
+    h.rtotal_tiles = rtiles;
+    h.nrjobs = (uint32_t)rjobs.size();
+    h.nrtails = (uint32_t)rtails.size();
+    h.rchunk = (uint32_t)(per < 1 ? 1 : per > 256 ? 256 : per);
+  }
+  h.off_rjobs = (uint32_t)off_rjobs;
+  h.off_rtails = (uint32_t)off_rtails;
   uint8_t* stage = ctx->block_host.p;
   memcpy(stage, &h, sizeof h);
   if (!jobs.empty()) memcpy(stage + off_jobs, jobs.data(), jobs.size() * sizeof(JobDev));
   if (!tails.empty()) memcpy(stage + off_tails, tails.data(), tails.size() * sizeof(TailDev));
+  if (!rjobs.empty())
+    memcpy(stage + off_rjobs, rjobs.data(), rjobs.size() * sizeof(ReplayJobDev));
+  if (!rtails.empty())
+    memcpy(stage + off_rtails, rtails.data(), rtails.size() * sizeof(TailDev));
   if (!groups.empty()) memcpy(stage + off_groups, groups.data(), groups.size() * sizeof(GroupDev));
   if (nslots) memcpy(stage + off_shifts, shift.data(), nslots);
   memset(stage + off_acc, 0, off_out - off_acc);
@@ -350,13 +443,14 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
   hipStream_t s = ctx->stream;
   HIP_TRY(hipMemcpyAsync(dev, stage, off_out, hipMemcpyHostToDevice, s));
   HIP_TRY(hipEventRecord(ctx->ev[0], s));
-  if (ctx->use_graph) {
+  if (ctx->use_graph && !replay) {
     hipGraphExec_t exec = nullptr;
     if ((rc = graph_for(ctx, kind, tpl, (int)grid, &exec))) return rc;
     HIP_TRY(hipGraphLaunch(exec, s));
     HIP_TRY(hipEventRecord(ctx->ev[1], s));
   } else {
-    HIP_TRY(launch_head(kind, tpl, hdr, (int)grid, s));
+    if (!replay || !jobs.empty()) HIP_TRY(launch_head(kind, tpl, hdr, (int)grid, s));
+    if (replay && (!rjobs.empty() || !rtails.empty())) HIP_TRY(launch_replay(hdr, rgrid, s));
     HIP_TRY(hipEventRecord(ctx->ev[1], s));
     HIP_TRY(launch_combine(hdr, s));
   }
@@ -845,6 +939,21 @@ int clb_autotune(clb_ctx* ctx, uint64_t n, double budget_ms, int* terms_per_lane
   ctx->blocks_per_cu = best_bpc == resident ? 0 : best_bpc;
   if (terms_per_lane) *terms_per_lane = kTplChoices[best_tpl];
   if (blocks_per_cu_out) *blocks_per_cu_out = best_bpc;
+  return CLB_OK;
+}
+
+int clb_set_replay(clb_ctx* ctx, uint64_t global_work_size) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (global_work_size == 0) {
+    ctx->replay_gws = 0;
+    return CLB_OK;
+  }
+  if (ctx->precision != CLB_PREC_U192 && ctx->precision != CLB_PREC_U192_BETA)
+    return fail(CLB_EINVAL, "replay needs precision 1 or 2 (the 106-bit mode is not replayed)");
+  if (global_work_size % 256 || global_work_size > kRefGlobalWorkSize)
+    return fail(CLB_EINVAL, "replay: %llu work-items (a multiple of 256 up to 2^24)",
+                (unsigned long long)global_work_size);
+  ctx->replay_gws = global_work_size;
   return CLB_OK;
 }
 

@@ -47,6 +47,20 @@ struct TailDev {
   uint32_t slot;
 };
 
+// Reference replay: one launch of the reference's Sglobal32mtg_192 / Sglobal64mtg_192
+// (Bellard.cl:771-823, :992-1084) with `gsize` work-items.  Work-item idx < min(gsize, k_max -
+// offset) sums the terms k = offset + idx + i gsize < k_max in double-double and converts once;
+// its tiles are 64 consecutive work-items, in the flat replay tile space from flat_begin.
+struct ReplayJobDev {
+  uint64_t d;
+  uint64_t offset, k_max;
+  uint64_t gsize;
+  uint64_t flat_begin;
+  uint64_t den0, den1;
+  uint32_t numesign;
+  uint32_t slot;
+};
+
 // One result: the sum over slots [first, first + count) of (slot value << shift[slot]) mod 2^192.
 struct GroupDev {
   uint32_t first;
@@ -76,6 +90,12 @@ struct LaunchDev {
   uint32_t replicas;           // copies per accumulator slot (a power of 2, at most 64)
   uint32_t nslots;             // accumulator slots
   uint32_t off_jobs, off_tails, off_groups, off_shifts, off_acc, off_out;  // byte offsets
+  // Reference replay (0 entries unless the call replays the reference's kernels): its launches'
+  // work-item tiles from their own queue, and its tails (Sglobal_after_192, one lane each).
+  unsigned long long rcounter;  // replay tiles handed out
+  uint64_t rtotal_tiles;
+  uint32_t nrjobs, nrtails, rchunk;
+  uint32_t off_rjobs, off_rtails;
 };
 
 __host__ __device__ inline const uint8_t* hdr_at(const LaunchDev* h, uint32_t off) {
@@ -92,6 +112,12 @@ __host__ __device__ inline const GroupDev* hdr_groups(const LaunchDev* h) {
 }
 __host__ __device__ inline const uint8_t* hdr_shifts(const LaunchDev* h) {
   return hdr_at(h, h->off_shifts);
+}
+__host__ __device__ inline const ReplayJobDev* hdr_rjobs(const LaunchDev* h) {
+  return reinterpret_cast<const ReplayJobDev*>(hdr_at(h, h->off_rjobs));
+}
+__host__ __device__ inline const TailDev* hdr_rtails(const LaunchDev* h) {
+  return reinterpret_cast<const TailDev*>(hdr_at(h, h->off_rtails));
 }
 __host__ __device__ inline unsigned long long* hdr_acc(LaunchDev* h) {
   return reinterpret_cast<unsigned long long*>(reinterpret_cast<uint8_t*>(h) + h->off_acc);
@@ -110,5 +136,10 @@ hipError_t launch_zero(unsigned long long* p, uint64_t nwords, hipStream_t strea
 hipError_t launch_sum7(uint64_t* ans, uint64_t* partial, int shift, hipStream_t stream);
 // Writes 18 and 9 to out2[0..1] when the device code applies FP64 output modifiers.
 hipError_t launch_selfcheck(uint64_t* out2, hipStream_t stream);
+
+// Reference replay (clb_replay.hip): persistent grid over the replay tiles and tails of `hdr`.
+int replay_blocks_per_cu();
+hipError_t launch_replay(LaunchDev* hdr, int grid, hipStream_t stream);
+constexpr uint64_t kRefGlobalWorkSize = 65536ull * 256ull;  // Bellard_192.hsp:5-7
 
 }  // namespace clb

@@ -12,6 +12,7 @@
 // atomics.  Integer sums are exact, so the result does not depend on the schedule.
 #include "clb_device.hpp"
 #include "clb_internal.hpp"
+#include "clb_wave.hpp"
 
 namespace clb {
 
@@ -27,32 +28,6 @@ __device__ __forceinline__ uint32_t block_x() { return __builtin_amdgcn_workgrou
 __device__ __forceinline__ uint32_t block_dim_x() { return __builtin_amdgcn_workgroup_size_x(); }
 __device__ __forceinline__ uint32_t grid_dim_x() {
   return *(const uint32_t*)__builtin_amdgcn_implicitarg_ptr();  // C cast: address space 4
-}
-
-__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int mask) {
-  const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, mask, kWave);
-  const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), mask, kWave);
-  return ((uint64_t)hi << 32) | lo;
-}
-
-// Sum of a 192-bit value over the 64 lanes of the wave (every lane receives the sum).
-__device__ __forceinline__ void wave_sum192(U192& v) {
-#pragma unroll
-  for (int mask = 32; mask >= 1; mask >>= 1) {
-    const uint64_t o0 = shfl_xor64(v.w0, mask);
-    const uint64_t o1 = shfl_xor64(v.w1, mask);
-    const uint64_t o2 = shfl_xor64(v.w2, mask);
-    add192(v, o0, o1, o2);
-  }
-}
-
-__device__ __forceinline__ void atomic_add192(unsigned long long* slot, const U192& v) {
-  atomicAdd(slot + 0, (unsigned long long)(v.w0 & 0xffffffffull));
-  atomicAdd(slot + 1, (unsigned long long)(v.w0 >> 32));
-  atomicAdd(slot + 2, (unsigned long long)(v.w1 & 0xffffffffull));
-  atomicAdd(slot + 3, (unsigned long long)(v.w1 >> 32));
-  atomicAdd(slot + 4, (unsigned long long)(v.w2 & 0xffffffffull));
-  atomicAdd(slot + 5, (unsigned long long)(v.w2 >> 32));
 }
 
 // Sign, wave sum and atomic add of the lanes' job sums.

@@ -1,0 +1,314 @@
+// clb_replay.hip -- reference replay: the rounding of the reference's precision-1 and -2 kernels
+// reproduced to the bit, for callers that need the reference's exact output bits rather than the
+// exact sum the other kernels compute.
+//
+// The reference's 192-bit kernels keep, per work-item, two double-double sums over the
+// work-item's strided terms (of r/m and of (r 2^96 mod m)/m, r = 1024^(d-k) mod m) and convert
+// them to 192 bits once (Bellard.cl:771-823, :992-1084, :506-578); its tail kernel is
+// double-double throughout (:1153-1253).  Everything after that is exact integer addition, so a
+// replay must reproduce each work-item's double-double arithmetic operation by operation, and may
+// add the work-items' 192-bit values in any order.  Here one lane is one work-item: a tile is 64
+// consecutive work-items of one reference launch, tiles come from a queue in chunks, and each
+// wave adds its lanes' values into the job's accumulator slot as the exact kernels do.
+//
+// This file is built in the default FP mode (IEEE mode on, denormals kept) and with contraction
+// off: each fused multiply-add below is one that the reference's OpenCL build makes (OpenCL C
+// contracts a*b + c within one expression) and is written out; every other operation is rounded
+// on its own.  The residues r are exact integers in the reference (its Montgomery
+// exponentiations, Bellard.cl:395-461), so they come from this library's exact integer engine.
+#include "clb_device.hpp"
+#include "clb_internal.hpp"
+#include "clb_wave.hpp"
+
+namespace clb {
+namespace {
+
+struct DD {
+  double hi, lo;
+};
+
+__device__ __forceinline__ DD two_sum(double a, double b) {  // twosum, Bellard.cl:1-7
+  const double x = a + b;
+  const double t = x - a;
+  return {x, (a - (x - t)) + (b - t)};
+}
+
+__device__ __forceinline__ DD split27(double a) {  // dsplit, :9-15
+  const double t = a * 134217729.0;
+  const double hi = t - (t - a);
+  return {hi, a - hi};
+}
+
+// twoproduct, :17-27: the error term is one expression, so its four products are fused.
+__device__ __forceinline__ DD two_prod(double a, double b) {
+  const double p = a * b;
+  const DD x = split27(a), y = split27(b);
+  double e = __builtin_fma(x.hi, y.hi, -p);
+  e = __builtin_fma(x.lo, y.hi, e);
+  e = __builtin_fma(x.hi, y.lo, e);
+  e = __builtin_fma(x.lo, y.lo, e);
+  return {p, e};
+}
+
+__device__ __forceinline__ DD dd_add(DD a, DD b) {  // addcab, :29-35
+  const DD c = two_sum(a.hi, b.hi);
+  return two_sum(c.hi, (c.lo + a.lo) + b.lo);
+}
+
+__device__ __forceinline__ DD dd_neg(DD a) { return {-a.hi, -a.lo}; }
+
+__device__ __forceinline__ DD div_nn(double x, double y) {  // nn_div, :99-106
+  const double z1 = x / y;
+  const DD c = two_prod(-z1, y);
+  return two_sum(z1, ((c.hi + x) + c.lo) / y);
+}
+
+// nd_div, :89-96, and dd_div, :70-77: "- z1 * y.y" is fused with the sum before it.
+__device__ __forceinline__ DD div_nd(double x, DD y) {
+  const double z1 = x / y.hi;
+  const DD c = two_prod(-z1, y.hi);
+  return two_sum(z1, (__builtin_fma(-z1, y.lo, c.hi + x) + c.lo) / y.hi);
+}
+
+__device__ __forceinline__ DD div_dd(DD x, DD y) {
+  const double z1 = x.hi / y.hi;
+  const DD c = two_prod(-z1, y.hi);
+  return two_sum(z1, ((__builtin_fma(-z1, y.lo, c.hi + x.hi) + x.lo) + c.lo) / y.hi);
+}
+
+__device__ __forceinline__ void fix_unit(DD& s) {  // sumfix2, :472-489
+  double lo = 0.0;
+  if (s.hi >= 1.0) lo = -1.0;
+  if (s.hi <= -1.0) lo = 1.0;
+  const DD t = two_sum(s.hi, lo);
+  s = two_sum(t.hi, t.lo + s.lo);
+}
+
+// dd2_to_ulong3, :506-578: 52 + 12 bits of s0, 40 more, then 96 bits of s1 (48 + 48) with the
+// two overlapping bits (30-31 of the middle word, 46-47 of s1's first 48) reconciled.
+__device__ U192 to192(DD s0, DD s1) {
+  const DD one = {1.0, 0.0};
+  if (s0.hi < 0.0) s0 = dd_add(s0, one);
+  if (s1.hi < 0.0) s1 = dd_add(s1, one);
+  s0.hi *= 0x1p52;
+  s0.lo *= 0x1p52;
+  double t = __builtin_trunc(s0.hi);
+  uint64_t u0 = (uint64_t)t * 4096u;
+  s0 = two_sum(s0.hi - t, s0.lo);
+  s0.hi *= 0x1p52;
+  if (s0.hi < 0.0) {
+    s0.hi += 0x1p52;
+    u0 -= 4096u;
+  }
+  uint64_t ut = (uint64_t)s0.hi;
+  u0 |= ut >> 40;
+  uint64_t u1 = (ut & ((1ull << 40) - 1)) << 24;
+  s1.hi *= 0x1p48;
+  s1.lo *= 0x1p48;
+  t = __builtin_trunc(s1.hi);
+  ut = (uint64_t)t;
+  s1 = two_sum(s1.hi - t, s1.lo);
+  s1.hi *= 0x1p48;
+  if (s1.hi < 0.0) {
+    s1.hi += 0x1p48;
+    ut -= 1;
+  }
+  const uint64_t u2 = ((ut & 0xffffull) << 48) + (uint64_t)s1.hi;
+  const uint32_t chk0 = (uint32_t)((u1 >> 30) & 3u);
+  const uint32_t chk1 = (uint32_t)(ut >> 46);
+  if (chk0 != chk1) {
+    const uint64_t last = u1;
+    if (((chk0 + 1u) & 3u) == chk1) {
+      u1 += 1ull << 30;
+      if (u1 < last) u0++;
+    }
+    if (((chk0 - 1u) % 4u) == chk1) {
+      u1 -= 1ull << 30;
+      if (u1 > last) u0--;
+    }
+  }
+  u1 = (u1 & 0xFFFFFFFF00000000ull) | (ut >> 16);
+  return {u2, u1, u0};
+}
+
+// One term's update of a sum (:786-796, :1037-1048): while the sum is positive the numerator is
+// m - r and the quotient is negated, which keeps the sum in (-1, 1).
+__device__ __forceinline__ void dd_step(DD& s, uint64_t r, uint64_t m) {
+  const bool pos = s.hi > 0.0;
+  DD q = div_nn((double)(pos ? m - r : r), (double)m);
+  if (pos) q = dd_neg(q);
+  s = dd_add(s, q);
+}
+
+// r = 2^e mod m and b = 2^(e + 96) mod m = r 2^96 mod m (:798-799, :1052-1053), exact, for odd
+// m < 2^55 with e < 2^bl (bl uniform): Montgomery form h = 2^(e + 64) mod m, r = REDC(h), and
+// b = REDC(h * (2^96 mod m)) with 2^96 mod m the Montgomery form of 2^32.
+__device__ __forceinline__ void residues(uint64_t e, int bl, uint64_t m, uint64_t& r,
+                                         uint64_t& b) {
+  const uint64_t mp = neg_inv64(m);
+  const uint64_t one = (0ull - m) % m;
+  const uint64_t h = ipow2_mont<3>(e, bl, m, mp, one);
+  const uint64_t c = ipow2_mont<3>(32, 6, m, mp, one);
+  r = redc(0, h, m, mp);
+  if (r >= m) r -= m;
+  b = redc(mulhi64(h, c), h * c, m, mp);
+  if (b >= m) b -= m;
+}
+
+// Work-item idx = 64 tile + lane of a replayed launch (:783-810, :1012-1078); idx < the launch's
+// work-items.  Its terms are k = offset + idx + i gsize < k_max; lane 0 of the tile has the
+// largest exponent of each round, which fixes the uniform window count.
+__device__ U192 replay_item(const ReplayJobDev& jb, uint64_t tile, int lane) {
+  DD s0 = {0.0, 0.0}, s1 = {0.0, 0.0};
+  const uint64_t k_lane0 = jb.offset + tile * kWave;
+  for (uint64_t kw = k_lane0; kw < jb.k_max; kw += jb.gsize) {
+    const uint64_t k = kw + (uint64_t)lane;
+    if (k < jb.k_max) {
+      const uint64_t m = jb.den0 * k + jb.den1;
+      const int bl = 64 - __builtin_clzll(10ull * (jb.d - kw));
+      uint64_t r, b;
+      residues(10ull * (jb.d - k), bl, m, r, b);
+      dd_step(s0, r, m);
+      dd_step(s1, b, m);
+    }
+  }
+  // The reference's k after its loop has the parity of offset + idx (gsize is even).
+  if ((jb.numesign + (k_lane0 + (uint64_t)lane) % 2) % 2 == 1) {
+    s0 = dd_neg(s0);
+    s1 = dd_neg(s1);
+  }
+  return to192(s0, s1);
+}
+
+__device__ uint64_t pow2_mod_small(int e, uint64_t m) {  // 2^e mod m, m < 2^63, by doubling
+  uint64_t x = 1 % m;
+  for (int i = 0; i < e; ++i) {
+    x <<= 1;
+    if (x >= m) x -= m;
+  }
+  return x;
+}
+
+__device__ __forceinline__ DD dd_of(uint64_t v) {  // (double)v and (double)((long)v - (long)x)
+  const double x = (double)v;
+  return {x, (double)((int64_t)v - (int64_t)x)};
+}
+
+// Sglobal_after_192, :1153-1253: k = d + i, i = 19..0, into s0 with numerators 2^(-10 i);
+// i = 19..10 into s1 with 2^(96 - 10 i); i = 9..0 into s1 with the residues 2^(96 - 10 i) mod m
+// (exact in the reference) and sumfix2 after each.
+__device__ U192 replay_tail(const TailDev& t) {
+  DD s0 = {0.0, 0.0}, s1 = {0.0, 0.0};
+  double n0 = 1.0, n1 = 0x1p96;
+  for (int i = 0; i < 19; ++i) {
+    n0 /= 1024.0;
+    n1 /= 1024.0;
+  }
+  int64_t i = 19;
+  for (; i >= 0; --i) {
+    const uint64_t k = t.d + (uint64_t)i;
+    DD q = div_nd(n0, dd_of(t.den0 * k + t.den1));
+    if (k % 2 == 1) q = dd_neg(q);
+    s0 = dd_add(s0, q);
+    n0 *= 1024.0;
+  }
+  for (i = 19; i >= 10; --i) {
+    const uint64_t k = t.d + (uint64_t)i;
+    DD q = div_nd(n1, dd_of(t.den0 * k + t.den1));
+    if (k % 2 == 1) q = dd_neg(q);
+    s1 = dd_add(s1, q);
+    n1 *= 1024.0;
+  }
+  for (; i >= 0; --i) {
+    const uint64_t k = t.d + (uint64_t)i;
+    const uint64_t m = t.den0 * k + t.den1;
+    const uint64_t nm = pow2_mod_small(96 - 10 * (int)i, m);
+    DD q = m > 4294967296ull ? div_dd(dd_of(nm), dd_of(m)) : div_nn((double)nm, (double)m);
+    if (k % 2 == 1) q = dd_neg(q);
+    s1 = dd_add(s1, q);
+    fix_unit(s1);
+  }
+  if (t.numesign == 1) {
+    s0 = dd_neg(s0);
+    s1 = dd_neg(s1);
+  }
+  return to192(s0, s1);
+}
+
+__device__ __forceinline__ void flush(U192& v, int lane, unsigned long long* slot) {
+  wave_sum192(v);
+  if (lane == 0) atomic_add192(slot, v);
+  v = {0, 0, 0};
+}
+
+}  // namespace
+
+// Persistent replay kernel: tails first (wave w: tails w, w + nwaves, ..., one lane each), then
+// chunks of consecutive replay tiles from the queue.  A wave's tiles only move forward, so its
+// job index does too; it adds its lanes' values into the job's slot when the job changes.
+__global__ void __launch_bounds__(kBlock) k_replay(LaunchDev* __restrict__ hdr) {
+  unsigned long long* __restrict__ acc = hdr_acc(hdr);
+  const ReplayJobDev* __restrict__ jobs = hdr_rjobs(hdr);
+  const int njobs = (int)hdr->nrjobs;
+  const uint64_t total = hdr->rtotal_tiles;
+  const uint64_t chunk = hdr->rchunk;
+  const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
+  const uint64_t wave_id = (uint64_t)blockIdx.x * (kBlock / kWave) +
+                           (uint64_t)__builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  const uint32_t replicas = hdr->replicas;
+  const uint32_t rep = (uint32_t)wave_id & (replicas - 1);
+  const TailDev* __restrict__ tails = hdr_rtails(hdr);
+  for (uint64_t t = wave_id; t < hdr->nrtails; t += nwaves) {
+    if (lane == 0)
+      atomic_add192(acc + ((size_t)tails[t].slot * replicas + rep) * kAccWords,
+                    replay_tail(tails[t]));
+  }
+  int cur = -1;
+  ReplayJobDev jb{};
+  U192 sum = {0, 0, 0};
+  for (;;) {
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&hdr->rcounter, (unsigned long long)chunk);
+    const uint64_t f0 = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32) |
+                        (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)base);
+    if (f0 >= total) break;
+    const uint64_t f1 = f0 + chunk < total ? f0 + chunk : total;
+    for (uint64_t f = f0; f < f1; ++f) {
+      int lo = cur < 0 ? 0 : cur, hi = njobs - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].flat_begin <= f)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      if (lo != cur) {
+        if (cur >= 0) flush(sum, lane, acc + ((size_t)jb.slot * replicas + rep) * kAccWords);
+        cur = lo;
+        jb = jobs[lo];
+      }
+      const uint64_t tile = f - jb.flat_begin;
+      const uint64_t span = jb.k_max - jb.offset;
+      const uint64_t items = span < jb.gsize ? span : jb.gsize;
+      if (tile * kWave + (uint64_t)lane < items) {
+        const U192 v = replay_item(jb, tile, lane);
+        add192(sum, v.w0, v.w1, v.w2);
+      }
+    }
+  }
+  if (cur >= 0) flush(sum, lane, acc + ((size_t)jb.slot * replicas + rep) * kAccWords);
+}
+
+int replay_blocks_per_cu() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_replay, kBlock, 0) != hipSuccess) n = 0;
+  return n;
+}
+
+hipError_t launch_replay(LaunchDev* hdr, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(k_replay, dim3(grid), dim3(kBlock), 0, stream, hdr);
+  return hipGetLastError();
+}
+
+}  // namespace clb

@@ -102,6 +102,18 @@ int clb_autotune(clb_ctx *ctx, uint64_t n, double budget_ms, int *terms_per_lane
  * as a HIP graph, captured on the first call with a given buffer layout and kernel choice and
  * replayed by later calls; 0 (default) = launch the kernels one by one.  Results are identical. */
 int clb_set_graph(clb_ctx *ctx, int enable);
+/* Reference replay.  By default every term is evaluated exactly (floor(r' 2^192 / m), the rule of
+ * the reference's Sglobal64mtg_192_Refresh, Bellard_div2by1.cl:313-359), so results do not depend
+ * on launch shapes.  global_work_size > 0 instead reproduces, bit for bit, the results of the
+ * reference's own precision-1 and -2 kernels launched with that many work-items, their
+ * double-double rounding included: Sglobal32mtg_192 (Bellard.cl:771-823), Sglobal64mtg_192
+ * (:992-1084, precision 1), Sglobal_after_192 (:1153-1253), replacing Refresh only where the
+ * reference does not use it.  The reference launches CLB_REF_GLOBAL_WORK_SIZE work-items
+ * (Bellard_192.hsp:5-7).  0 turns replay off.  CLB_EINVAL for precision 0 (not replayed) or a
+ * size that is not a multiple of 256 up to 2^24; calls with 10 d + 9 >= 2^53 (the reference's
+ * OVERDOUBLE build) then fail with CLB_ERANGE.  Replayed calls never use graph mode. */
+#define CLB_REF_GLOBAL_WORK_SIZE 16777216ull
+int clb_set_replay(clb_ctx *ctx, uint64_t global_work_size);
 
 /* BuildOptions core (Bellard_192.hsp:449-450, :468-474): N -> odd loop count d and N' (a multiple
  * of 5, the 1-based hex position of the first output digit); overdouble = (10d + 9 >= 2^53).

@@ -3,15 +3,18 @@
 The reference's 192-bit kernels round: each work-item keeps two double-double sums over its
 strided terms (Bellard.cl:771-823, :992-1084) and converts them once (dd2_to_ulong3,
 :506-578); the tail kernel is double-double throughout (:1153-1253).  orc_ref_batch192 restates
-that arithmetic operation by operation.  Here it is held, bit for bit, to the reference's own
-kernels (oracle/_ref, GPU) on every case of ref_dd_cases().
+that arithmetic operation by operation, and the library's replay mode (clb_set_replay,
+clb_replay.hip) reproduces it on the GPU.  Both are held, bit for bit, to the reference's own
+kernels (oracle/_ref, GPU) on every case of ref_dd_cases() and on whole positions.
 """
 import os
 import sys
 
 import pytest
 
-from clb_testlib import (M192, REF_LIB, SHIFT, RefKernels, load_oracle, ref_dd_cases, w2i)
+import clbellard_amd as clb
+from clb_testlib import (M192, REF_LIB, SHIFT, RefKernels, load_oracle, py_normalize,
+                         ref_dd_cases, w2i)
 
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not os.path.exists(REF_LIB),
@@ -38,6 +41,14 @@ def orc():
     return load_oracle()
 
 
+@pytest.fixture(scope="module")
+def ctxs():
+    c = {p: clb.Context(0, p) for p in (clb.PREC_U192, clb.PREC_U192_BETA)}
+    yield c
+    for x in c.values():
+        x.close()
+
+
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "d{}_k{}-{}_j{}_p{}_g{}".format(*c))
 def test_oracle_replay_equals_reference_kernels(ref, orc, case):
     d, k0, k1, j, p, gws = case
@@ -46,3 +57,52 @@ def test_oracle_replay_equals_reference_kernels(ref, orc, case):
     got = w2i(ref.job(d, k0, k1, j, p))
     want = (orc.ref_batch192(d, k0, k1, j, p, gws) << SHIFT[j]) & M192
     assert got == want, f"{got:048x} != {want:048x}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "d{}_k{}-{}_j{}_p{}_g{}".format(*c))
+def test_hip_replay_equals_reference_kernels(ref, ctxs, case):
+    d, k0, k1, j, p, gws = case
+    note(f"replay d={d} k=[{k0},{k1}) j={j} precision={p} gws={gws}")
+    ctx = ctxs[p]
+    ctx.set_replay(gws)
+    ref.set_gws(gws)
+    got = w2i(ctx.run_jobs([(d, k0, k1, j)])[0])
+    want = w2i(ref.job(d, k0, k1, j, p))
+    assert got == want, f"{got:048x} != {want:048x}"
+
+
+@pytest.mark.parametrize("n", [10**6, 10**7, 10**8, 10**9, 2 * 10**9])
+@pytest.mark.parametrize("p", [1, 2])
+def test_hip_replay_positions_equal_reference_kernels(ref, ctxs, n, p):
+    """Normal mode at the reference's own launch size: all 48 hex digits equal."""
+    nn, d = py_normalize(n)
+    note(f"replay position N={nn} precision {p}")
+    ctx = ctxs[p]
+    ctx.set_replay(clb.REF_GLOBAL_WORK_SIZE)
+    ref.set_gws()
+    frac, hexs, _ = ctx.normal(n)
+    want = ref.normal(d, p)
+    note(f"  ours {hexs}\n  ref  {clb.hex48(want)}")
+    assert w2i(frac) == w2i(want)
+    ctx.set_replay(0)
+    assert ctx.normal(n)[1][:24] == hexs[:24]  # the exact mode agrees on the leading digits
+
+
+def test_replay_arguments(ctxs):
+    ctx = ctxs[clb.PREC_U192]
+    for bad in (100, clb.REF_GLOBAL_WORK_SIZE * 2):
+        with pytest.raises(clb.ClbError) as e:
+            ctx.set_replay(bad)
+        assert e.value.code == clb.EINVAL
+    with clb.Context(0, clb.PREC_DD106) as c0:
+        with pytest.raises(clb.ClbError) as e:
+            c0.set_replay(clb.REF_GLOBAL_WORK_SIZE)
+        assert e.value.code == clb.EINVAL
+    ctx.set_replay(clb.REF_GLOBAL_WORK_SIZE)
+    try:
+        d_over = (1 << 53) // 10  # 10 d + 9 >= 2^53: the reference's OVERDOUBLE build
+        with pytest.raises(clb.ClbError) as e:
+            ctx.run_jobs([(d_over, d_over - 100, d_over, 0)])
+        assert e.value.code == clb.ERANGE
+    finally:
+        ctx.set_replay(0)
