@@ -217,6 +217,40 @@ def bench_worklist(clb, args):
     return 0
 
 
+def bench_replay(clb, args):
+    """Normal mode at N (default 10^12) replaying the reference's precision-1 and -2 kernels
+    (clb_set_replay with the reference's 2^24 work-items per launch): the reference's own output
+    bits, its double-double rounding included.  The leading 24 hex digits are compared with the
+    exact mode's."""
+    n_pos, d, _ = clb.normalize(args.position)
+    exact = None
+    rows = {}
+    for prec in (clb.PREC_U192_BETA, clb.PREC_U192):
+        ctx = clb.Context(0, prec)
+        if exact is None:
+            exact = ctx.normal(n_pos)[1]
+            log(f"exact mode: {exact}")
+        ctx.set_replay(clb.REF_GLOBAL_WORK_SIZE)
+        t0 = time.perf_counter()
+        _, hexs, _ = ctx.normal(n_pos)
+        dt = time.perf_counter() - t0
+        ctx.close()
+        if hexs[:24] != exact[:24]:
+            raise SystemExit(f"precision {prec} replay {hexs} disagrees with exact {exact}")
+        rows[str(prec)] = {"seconds": round(dt, 3), "digits": hexs}
+        log(f"replay precision {prec}: {dt:.2f}s {hexs}")
+    print(json.dumps({
+        "metric": "Bellard terms/sec, reference replay (precision 1)",
+        "value": d / rows["1"]["seconds"],
+        "unit": "terms/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": 1e3 * rows["1"]["seconds"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"normal mode at N={n_pos} replaying the reference's precision-1 "
+                               "and -2 kernels (2^24 work-items per launch)", "position": n_pos,
+                   "d": d, "precision": rows}}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,11 +261,12 @@ def main():
     ap.add_argument("--window-log2", type=int, default=26)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["normal", "batched", "worklist", "small"],
+    ap.add_argument("--workload", choices=["normal", "batched", "worklist", "small", "replay"],
                     default="normal",
                     help="normal: the metric (N = 10^12); batched: many positions in one launch; "
                          "worklist: uneven term-range jobs written as work files and summed; "
-                         "small: call latency of small positions, eager vs captured graph")
+                         "small: call latency of small positions, eager vs captured graph; "
+                         "replay: the reference's precision-1/2 output bits at N")
     ap.add_argument("--batches", type=int, default=8,
                     help="batched: split the positions over this many launches (each launch "
                          "prints when it ends; 4096 positions in one launch run ~10 minutes "
@@ -269,6 +304,8 @@ def main():
         return bench_worklist(clb, args)
     if args.workload == "small":
         return bench_small(clb, args)
+    if args.workload == "replay":
+        return bench_replay(clb, args)
 
     n_pos, d, _ = clb.normalize(args.position)
     ctx = clb.Context(0, clb.PREC_U192, kernel=clb.KERNEL_SPLIT)
