@@ -139,7 +139,7 @@ struct clb_ctx {
   bool use_graph = false;
   // Reference replay: 0, or the work-items per replayed launch of the reference's kernels.
   uint64_t replay_gws = 0;
-  int replay_resident = -1;  // resident k_replay workgroups per CU (-1: not queried yet)
+  int replay_resident[5] = {-1, -1, -1, -1, -1};  // resident k_replay workgroups per CU, per kind
   struct Graph {
     uintptr_t key[4];
     hipGraphExec_t exec;
@@ -230,6 +230,8 @@ int expand_replay(const clb_ctx* ctx, const std::vector<HostJob>& in, std::vecto
                   std::vector<ReplayJobDev>* rjobs, std::vector<TailDev>* rtails,
                   uint64_t* rtiles) {
   const uint64_t g = ctx->replay_gws;
+  if (ctx->precision != CLB_PREC_U192 && ctx->precision != CLB_PREC_U192_BETA)
+    return fail(CLB_EINVAL, "replay: the 106-bit mode (precision 0) is not replayed");
   uint64_t flat = 0;
   for (const HostJob& h : in) {
     if (h.d > ((1ull << 53) - 10) / 10)  // Bellard_192.hsp:472: 10 d + 9 >= 2^53
@@ -411,11 +413,19 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
   h.off_shifts = (uint32_t)off_shifts;
   h.off_acc = (uint32_t)off_acc;
   h.off_out = (uint32_t)off_out;
-  int rgrid = 0;
+  int rgrid = 0, rkind = kPlainW3;
   if (replay) {
-    if (ctx->replay_resident < 0) ctx->replay_resident = replay_blocks_per_cu();
-    if (ctx->replay_resident <= 0) return fail(CLB_EDEVICE, "replay kernel: no occupancy");
-    rgrid = ctx->num_cus * ctx->replay_resident;
+    uint64_t rmax_m = 1;
+    for (const ReplayJobDev& r : rjobs) {
+      const uint64_t m = r.den0 * (r.k_max - 1) + r.den1;
+      if (m > rmax_m) rmax_m = m;
+    }
+    if ((rc = choose_kind(ctx->kernel, rmax_m, &rkind))) return rc;
+    if (rkind == kPlainWide) rkind = kPlainW3;  // replayed moduli are below 2^53
+    int& res = ctx->replay_resident[rkind];
+    if (res < 0) res = replay_blocks_per_cu(rkind);
+    if (res <= 0) return fail(CLB_EDEVICE, "replay kernel: no occupancy");
+    rgrid = ctx->num_cus * res;
     const uint64_t rwaves = (uint64_t)rgrid * (kBlock / kWave);
     const uint64_t per = rtiles / (8 * rwaves);
     if (rjobs.size() > 0x7fffffff || rtails.size() > 0x7fffffff)
@@ -450,7 +460,8 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
     HIP_TRY(hipEventRecord(ctx->ev[1], s));
   } else {
     if (!replay || !jobs.empty()) HIP_TRY(launch_head(kind, tpl, hdr, (int)grid, s));
-    if (replay && (!rjobs.empty() || !rtails.empty())) HIP_TRY(launch_replay(hdr, rgrid, s));
+    if (replay && (!rjobs.empty() || !rtails.empty()))
+      HIP_TRY(launch_replay(rkind, hdr, rgrid, s));
     HIP_TRY(hipEventRecord(ctx->ev[1], s));
     HIP_TRY(launch_combine(hdr, s));
   }

@@ -138,8 +138,9 @@ hipError_t launch_sum7(uint64_t* ans, uint64_t* partial, int shift, hipStream_t 
 hipError_t launch_selfcheck(uint64_t* out2, hipStream_t stream);
 
 // Reference replay (clb_replay.hip): persistent grid over the replay tiles and tails of `hdr`.
-int replay_blocks_per_cu();
-hipError_t launch_replay(LaunchDev* hdr, int grid, hipStream_t stream);
+// kind: kSplitW3/W2/W1 (fp64 residues) or kPlainW3 (integer residues), by the largest modulus.
+int replay_blocks_per_cu(int kind);
+hipError_t launch_replay(int kind, LaunchDev* hdr, int grid, hipStream_t stream);
 constexpr uint64_t kRefGlobalWorkSize = 65536ull * 256ull;  // Bellard_192.hsp:5-7
 
 }  // namespace clb

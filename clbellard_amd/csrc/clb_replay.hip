@@ -12,10 +12,12 @@
 // wave adds its lanes' values into the job's accumulator slot as the exact kernels do.
 //
 // This file is built in the default FP mode (IEEE mode on, denormals kept) and with contraction
-// off: each fused multiply-add below is one that the reference's OpenCL build makes (OpenCL C
-// contracts a*b + c within one expression) and is written out; every other operation is rounded
-// on its own.  The residues r are exact integers in the reference (its Montgomery
-// exponentiations, Bellard.cl:395-461), so they come from this library's exact integer engine.
+// off: each fused multiply-add in the double-double code below is one that the reference's OpenCL
+// build makes (OpenCL C contracts a*b + c within one expression) and is written out; every other
+// operation is rounded on its own.  The residues r are exact integers in the reference (its
+// Montgomery exponentiations, Bellard.cl:395-461), so they come from this library's exact
+// engines: the fp64 engine's windowed squares (not its doubling squares, whose output modifier
+// needs the IEEE mode bit clear) for m < 2^48, the integer engine above.
 #include "clb_device.hpp"
 #include "clb_internal.hpp"
 #include "clb_wave.hpp"
@@ -140,24 +142,48 @@ __device__ __forceinline__ void dd_step(DD& s, uint64_t r, uint64_t m) {
   s = dd_add(s, q);
 }
 
-// r = 2^e mod m and b = 2^(e + 96) mod m = r 2^96 mod m (:798-799, :1052-1053), exact, for odd
-// m < 2^55 with e < 2^bl (bl uniform): Montgomery form h = 2^(e + 64) mod m, r = REDC(h), and
-// b = REDC(h * (2^96 mod m)) with 2^96 mod m the Montgomery form of 2^32.
+// r = 2^e mod m and b = 2^(e + 96) mod m = r 2^96 mod m (:798-799, :1052-1053), exact, with
+// e < 2^bl (bl uniform per wave).
+//  * fp64 engine (KIND kSplitW3/W2/W1, m < 2^42 / 2^46 / 2^48): per-lane W-bit windows of plain
+//    and windowed balanced squares (fpow2, |x| <= 0.75 m), then two exact shift-and-reduce
+//    steps of 48 bits (|x 2^48 / m - q| <= 0.5 + 0.75 2^48 1.5 2^-52 < 0.6, so the residue stays
+//    below 0.6 m and every intermediate is an exact integer below 2^53);
+//  * int engine (kPlainW3, odd m < 2^55): Montgomery form h = 2^(e + 64) mod m, r = REDC(h), and
+//    b = REDC(h (2^96 mod m)) with 2^96 mod m the Montgomery form of 2^32.
+template <int KIND>
 __device__ __forceinline__ void residues(uint64_t e, int bl, uint64_t m, uint64_t& r,
                                          uint64_t& b) {
-  const uint64_t mp = neg_inv64(m);
-  const uint64_t one = (0ull - m) % m;
-  const uint64_t h = ipow2_mont<3>(e, bl, m, mp, one);
-  const uint64_t c = ipow2_mont<3>(32, 6, m, mp, one);
-  r = redc(0, h, m, mp);
-  if (r >= m) r -= m;
-  b = redc(mulhi64(h, c), h * c, m, mp);
-  if (b >= m) b -= m;
+  if constexpr (KIND == kPlainW3) {
+    const uint64_t mp = neg_inv64(m);
+    const uint64_t one = (0ull - m) % m;
+    const uint64_t h = ipow2_mont<3>(e, bl, m, mp, one);
+    const uint64_t c = ipow2_mont<3>(32, 6, m, mp, one);
+    r = redc(0, h, m, mp);
+    if (r >= m) r -= m;
+    b = redc(mulhi64(h, c), h * c, m, mp);
+    if (b >= m) b -= m;
+  } else {
+    constexpr int W = KIND == kSplitW3 ? 3 : KIND == kSplitW2 ? 2 : 1;
+    const double mf = (double)m;
+    const double minv = recip(mf);
+    const int s = bl > 3 ? W * ((bl - 3) / W) : 0;  // top bits t = e >> s < 2^(3 + W - 1)
+    double x = fpow2<W>(e, s, mf, minv);
+    double y = x < 0.0 ? x + mf : x;
+    r = (uint64_t)y;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const double z = __builtin_amdgcn_ldexp(x, 48);
+      x = __builtin_fma(-__builtin_rint(z * minv), mf, z);
+    }
+    y = x < 0.0 ? x + mf : x;
+    b = (uint64_t)y;
+  }
 }
 
 // Work-item idx = 64 tile + lane of a replayed launch (:783-810, :1012-1078); idx < the launch's
 // work-items.  Its terms are k = offset + idx + i gsize < k_max; lane 0 of the tile has the
 // largest exponent of each round, which fixes the uniform window count.
+template <int KIND>
 __device__ U192 replay_item(const ReplayJobDev& jb, uint64_t tile, int lane) {
   DD s0 = {0.0, 0.0}, s1 = {0.0, 0.0};
   const uint64_t k_lane0 = jb.offset + tile * kWave;
@@ -167,7 +193,7 @@ __device__ U192 replay_item(const ReplayJobDev& jb, uint64_t tile, int lane) {
       const uint64_t m = jb.den0 * k + jb.den1;
       const int bl = 64 - __builtin_clzll(10ull * (jb.d - kw));
       uint64_t r, b;
-      residues(10ull * (jb.d - k), bl, m, r, b);
+      residues<KIND>(10ull * (jb.d - k), bl, m, r, b);
       dd_step(s0, r, m);
       dd_step(s1, b, m);
     }
@@ -246,6 +272,7 @@ __device__ __forceinline__ void flush(U192& v, int lane, unsigned long long* slo
 // Persistent replay kernel: tails first (wave w: tails w, w + nwaves, ..., one lane each), then
 // chunks of consecutive replay tiles from the queue.  A wave's tiles only move forward, so its
 // job index does too; it adds its lanes' values into the job's slot when the job changes.
+template <int KIND>
 __global__ void __launch_bounds__(kBlock) k_replay(LaunchDev* __restrict__ hdr) {
   unsigned long long* __restrict__ acc = hdr_acc(hdr);
   const ReplayJobDev* __restrict__ jobs = hdr_rjobs(hdr);
@@ -292,7 +319,7 @@ __global__ void __launch_bounds__(kBlock) k_replay(LaunchDev* __restrict__ hdr) 
       const uint64_t span = jb.k_max - jb.offset;
       const uint64_t items = span < jb.gsize ? span : jb.gsize;
       if (tile * kWave + (uint64_t)lane < items) {
-        const U192 v = replay_item(jb, tile, lane);
+        const U192 v = replay_item<KIND>(jb, tile, lane);
         add192(sum, v.w0, v.w1, v.w2);
       }
     }
@@ -300,14 +327,33 @@ __global__ void __launch_bounds__(kBlock) k_replay(LaunchDev* __restrict__ hdr) 
   if (cur >= 0) flush(sum, lane, acc + ((size_t)jb.slot * replicas + rep) * kAccWords);
 }
 
-int replay_blocks_per_cu() {
+template <int KIND>
+static int replay_blocks() {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_replay, kBlock, 0) != hipSuccess) n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_replay<KIND>, kBlock, 0) != hipSuccess)
+    n = 0;
   return n;
 }
 
-hipError_t launch_replay(LaunchDev* hdr, int grid, hipStream_t stream) {
-  hipLaunchKernelGGL(k_replay, dim3(grid), dim3(kBlock), 0, stream, hdr);
+int replay_blocks_per_cu(int kind) {
+  switch (kind) {
+    case kSplitW3: return replay_blocks<kSplitW3>();
+    case kSplitW2: return replay_blocks<kSplitW2>();
+    case kSplitW1: return replay_blocks<kSplitW1>();
+    case kPlainW3: return replay_blocks<kPlainW3>();
+    default: return 0;
+  }
+}
+
+hipError_t launch_replay(int kind, LaunchDev* hdr, int grid, hipStream_t stream) {
+  const dim3 g(grid), b(kBlock);
+  switch (kind) {
+    case kSplitW3: hipLaunchKernelGGL(k_replay<kSplitW3>, g, b, 0, stream, hdr); break;
+    case kSplitW2: hipLaunchKernelGGL(k_replay<kSplitW2>, g, b, 0, stream, hdr); break;
+    case kSplitW1: hipLaunchKernelGGL(k_replay<kSplitW1>, g, b, 0, stream, hdr); break;
+    case kPlainW3: hipLaunchKernelGGL(k_replay<kPlainW3>, g, b, 0, stream, hdr); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -44,6 +44,8 @@ def orc():
 @pytest.fixture(scope="module")
 def ctxs():
     c = {p: clb.Context(0, p) for p in (clb.PREC_U192, clb.PREC_U192_BETA)}
+    c.update({(p, "plain"): clb.Context(0, p, kernel=clb.KERNEL_PLAIN)
+              for p in (clb.PREC_U192, clb.PREC_U192_BETA)})
     yield c
     for x in c.values():
         x.close()
@@ -59,11 +61,13 @@ def test_oracle_replay_equals_reference_kernels(ref, orc, case):
     assert got == want, f"{got:048x} != {want:048x}"
 
 
+@pytest.mark.parametrize("engine", ["auto", "plain"])
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "d{}_k{}-{}_j{}_p{}_g{}".format(*c))
-def test_hip_replay_equals_reference_kernels(ref, ctxs, case):
+def test_hip_replay_equals_reference_kernels(ref, ctxs, case, engine):
+    """Residues from the fp64 engine (auto: moduli below 2^48) and from the integer engine."""
     d, k0, k1, j, p, gws = case
-    note(f"replay d={d} k=[{k0},{k1}) j={j} precision={p} gws={gws}")
-    ctx = ctxs[p]
+    note(f"replay ({engine}) d={d} k=[{k0},{k1}) j={j} precision={p} gws={gws}")
+    ctx = ctxs[p] if engine == "auto" else ctxs[(p, "plain")]
     ctx.set_replay(gws)
     ref.set_gws(gws)
     got = w2i(ctx.run_jobs([(d, k0, k1, j)])[0])
@@ -71,7 +75,7 @@ def test_hip_replay_equals_reference_kernels(ref, ctxs, case):
     assert got == want, f"{got:048x} != {want:048x}"
 
 
-@pytest.mark.parametrize("n", [10**6, 10**7, 10**8, 10**9, 2 * 10**9])
+@pytest.mark.parametrize("n", [10**6, 10**7, 10**8, 10**9, 2 * 10**9, 10**10])
 @pytest.mark.parametrize("p", [1, 2])
 def test_hip_replay_positions_equal_reference_kernels(ref, ctxs, n, p):
     """Normal mode at the reference's own launch size: all 48 hex digits equal."""
@@ -106,3 +110,42 @@ def test_replay_arguments(ctxs):
         assert e.value.code == clb.ERANGE
     finally:
         ctx.set_replay(0)
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_replay_work_files_equal_reference_jobs(ref, ctxs, tmp_path, p):
+    """Work mode (Batchloop + fsave, Bellard_192.hsp:298-333) replayed: every work file equals the
+    reference kernels' job (Set0mem, BatchCalc, AddAns) bit for bit, double-double ranges and
+    tails included, so the files sum (workmodeSum) to what the reference's own work files sum to.
+    That is not the normal-mode result: each job restarts the launch chunking at its k_start, so
+    above loopmax32 its double-double grouping differs from one run from 0 (the replayed normal
+    mode equals the reference kernels' normal mode, checked separately)."""
+    n = 25_000_000_000
+    _, d = py_normalize(n)
+    batch = 1 << 30
+    nb = d // batch + 1
+    wl = tmp_path / "worklist.txt"
+    lines = [f"digits={n}", f"precision={p}"] + [f"{j}_{b}" for j in range(7) for b in range(nb)]
+    wl.write_bytes(("\r\n".join(lines) + "\r\n").encode())
+    work = str(tmp_path / "work")
+    ctx = ctxs[p]
+    ctx.set_replay(clb.REF_GLOBAL_WORK_SIZE)
+    ref.set_gws()
+    try:
+        assert ctx.run_worklist(str(wl), work, batch_size=batch) == 7 * nb
+        frac, digits, _ = ctx.normal(n)
+    finally:
+        ctx.set_replay(0)
+    theirs_total = 0
+    for j in range(7):
+        for b in range(nb):
+            k0, k1 = b * batch, (b + 1) * batch
+            line0 = open(os.path.join(work, str(j), f"{j}_{b}.txt"), "rb").read().split(b"\r\n")[0]
+            theirs = w2i(ref.job(d, k0, k1, j, p))
+            assert int(line0, 16) == theirs, (j, b)
+            theirs_total = (theirs_total + theirs) % (1 << 192)
+    assert w2i(frac) == w2i(ref.normal(d, p))
+    assert clb.work_sum(work) == (f"{theirs_total:048x}", [nb] * 7)
+    if p == 2:  # the double-double range (below loopmax32) lies in batch 0 either way
+        assert f"{theirs_total:048x}" == digits
+    note(f"precision {p}: {7 * nb} replayed work files equal the reference kernels' jobs")

@@ -1,0 +1,33 @@
+"""GPU: the reference replay at the headline position N = 10^12 against the reference kernels' own
+output there (tests/golden/ref_positions.json): all 48 hex digits, double-double rounding
+included.  Precision 2 takes about 19 s; precision 1 (about 100 s) runs when CLB_REPLAY_SLOW is
+set."""
+import json
+import os
+import sys
+import time
+
+import pytest
+
+import clbellard_amd as clb
+from clb_testlib import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+with open(os.path.join(GOLDEN, "ref_positions.json")) as f:
+    ROWS = json.load(f)["positions"]
+
+
+@pytest.mark.parametrize("row", ROWS, ids=lambda r: f"n{r['n']}_p{r['precision']}")
+def test_replay_equals_reference_output(row):
+    if row["precision"] == 1 and not os.environ.get("CLB_REPLAY_SLOW"):
+        pytest.skip("precision 1 replay at N = 10^12 takes ~100 s; set CLB_REPLAY_SLOW=1")
+    print(f"replay N={row['n']} precision {row['precision']}", file=sys.stderr, flush=True)
+    with clb.Context(0, row["precision"]) as ctx:
+        ctx.set_replay(clb.REF_GLOBAL_WORK_SIZE)
+        t0 = time.perf_counter()
+        _, hexs, _ = ctx.normal(row["n"])
+        dt = time.perf_counter() - t0
+    print(f"  {hexs} in {dt:.2f} s (reference kernels: {row['reference_seconds']} s)",
+          file=sys.stderr, flush=True)
+    assert hexs == row["hex"]
