@@ -10,6 +10,7 @@
 #include <math.h>
 #include <pthread.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 typedef unsigned __int128 u128;
@@ -661,6 +662,196 @@ int orc_ref_normal192(uint64_t n, int precision, uint64_t gsize, int nthreads, u
     if (orc_ref_batch192(d, 0, d, j, precision, gsize, nthreads, part)) return -1;
     orc_sum7_192(ans, part, kNumer[j]);
   }
+  return 0;
+}
+
+/* ---------------------------------------------------------------- reference replay, 106-bit mode */
+
+/* The 106-bit mode (precision 0) keeps double-doubles all the way: per work-item sums, a 256-lane
+ * tree per work-group (shared_reduction_dd), 65536 per-group accumulators updated launch by launch
+ * (bigSum), the tail added into bigSum[0], two Ssum trees and Sum7.  Restated literally. */
+
+#define REF_GROUPS 65536 /* BIGSUM_SIZE, Bellard_192.hsp:5 */
+
+/* Bellard.cl:37-43 (subcab) */
+static rdd rp_subcab(rdd a, rdd b) {
+  rdd c = rp_twosum(a.hi, -b.hi);
+  c.lo = (c.lo + a.lo) - b.lo;
+  return rp_twosum(c.hi, c.lo);
+}
+
+/* Bellard.cl:491-501 (sumfix) */
+static void rp_sumfix(rdd *s) {
+  if (s->hi >= 1.0) {
+    rdd t = rp_twosum(s->hi, -1.0);
+    t.lo += s->lo;
+    *s = rp_twosum(t.hi, t.lo);
+  }
+}
+
+/* One work-item of Sglobal32mtg (Bellard.cl:729-752: sumfix after each term) or Sglobal64mtg
+ * (:914-975: the steered numerator of the 192-bit kernels), with the sign of its terms. */
+static rdd rp_item106(uint64_t d, uint64_t offset, uint64_t k_max, uint64_t idx, uint64_t gsize,
+                      int j, int is64) {
+  rdd s = {0.0, 0.0};
+  uint64_t k = offset + idx;
+  for (; k < k_max; k += gsize) {
+    const uint64_t m = kDen0[j] * k + kDen1[j];
+    const uint64_t r = orc_pow1024_mod(d - k, m);
+    if (is64) {
+      rp_ddstep(&s, r, m);
+    } else {
+      s = rp_addcab(s, rp_nn_div((double)r, (double)m));
+      rp_sumfix(&s);
+    }
+  }
+  if (((uint64_t)kNumeSign[j] + k % 2) % 2 == 1) s = rp_neg(s);
+  return s;
+}
+
+/* shared_reduction_dd (Bellard.cl:583-597) and Ssum's tree (:1274-1297) over p[0..255]. */
+static rdd rp_tree256(rdd *p) {
+  for (int i = 128; i > 0; i /= 2)
+    for (int l = 0; l < i; ++l) {
+      rdd s = rp_addcab(p[l], p[l + i]);
+      rp_sumfix2(&s);
+      p[l] = s;
+    }
+  return p[0];
+}
+
+struct group_job {
+  uint64_t d, offset, k_max, gsize;
+  int j, is64;
+  uint64_t g0, g1, step;
+  rdd *bigsum;
+};
+
+static void *group_worker(void *arg) {
+  struct group_job *jb = (struct group_job *)arg;
+  rdd p[256];
+  for (uint64_t g = jb->g0; g < jb->g1; g += jb->step) {
+    for (int l = 0; l < 256; ++l)
+      p[l] = rp_item106(jb->d, jb->offset, jb->k_max, g * 256 + (uint64_t)l, jb->gsize, jb->j,
+                        jb->is64);
+    rdd s = rp_addcab(jb->bigsum[g], rp_tree256(p)); /* :760-764, :979-983 */
+    rp_sumfix2(&s);
+    jb->bigsum[g] = s;
+  }
+  return NULL;
+}
+
+static void ref_launch106(uint64_t d, uint64_t offset, uint64_t k_max, uint64_t gsize, int j,
+                          int is64, int nthreads, rdd *bigsum) {
+  const uint64_t groups = gsize / 256;
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  if ((uint64_t)nthreads > groups) nthreads = (int)groups;
+  struct group_job jobs[256];
+  pthread_t th[256];
+  for (int t = 0; t < nthreads; ++t) {
+    jobs[t] = (struct group_job){d, offset, k_max, gsize, j, is64, (uint64_t)t, groups,
+                                 (uint64_t)nthreads, bigsum};
+    if (nthreads == 1 || pthread_create(&th[t], NULL, group_worker, &jobs[t]) != 0) {
+      group_worker(&jobs[t]);
+      th[t] = 0;
+    }
+  }
+  for (int t = 0; t < nthreads; ++t)
+    if (th[t]) pthread_join(th[t], NULL);
+}
+
+/* Sglobal_after (Bellard.cl:1118-1150): k = d .. d+9, numerators 2^(-10 i). */
+static rdd rp_tail106(uint64_t d, int j) {
+  rdd s = {0.0, 0.0};
+  double n = 1.0;
+  for (uint64_t i = 0; i < 10; ++i) {
+    const uint64_t m = kDen0[j] * (d + i) + kDen1[j];
+    const rdd y = {(double)m, (double)((int64_t)m - (int64_t)(double)m)};
+    rdd q = rp_nd_div(n, y);
+    if ((d + i) % 2 == 1) q = rp_neg(q);
+    s = rp_addcab(s, q);
+    n /= 1024.0;
+  }
+  if (((uint64_t)kNumeSign[j] + 2) % 2 == 1) s = rp_neg(s);
+  return s;
+}
+
+/* BatchCalc in the 106-bit mode on bigsum[REF_GROUPS] (updated as the reference's bigSumMem). */
+static void ref_batch106(uint64_t d, uint64_t k_start, uint64_t k_end, int j, uint64_t gsize,
+                         int nthreads, rdd *bigsum) {
+  uint64_t k_max = k_start, offset = k_start;
+  uint64_t lm32 = ((1ull << 32) - kDen1[j] - 1) / kDen0[j] + 1; /* KernelFuncs.as:31-33 */
+  if (lm32 > d) lm32 = d;
+  if (lm32 > k_end) lm32 = k_end;
+  while (offset < lm32) { /* :34-41 */
+    k_max = k_max + gsize * 16 < lm32 ? k_max + gsize * 16 : lm32;
+    ref_launch106(d, offset, k_max, gsize, j, 0, nthreads, bigsum);
+    offset = k_max;
+  }
+  const uint64_t lm64 = k_end < d ? k_end : d; /* :45-53 */
+  while (offset < lm64) {
+    k_max = k_max + gsize * 32 < lm64 ? k_max + gsize * 32 : lm64;
+    ref_launch106(d, offset, k_max, gsize, j, 1, nthreads, bigsum);
+    offset = k_max;
+  }
+  if (offset == d) bigsum[0] = rp_addcab(bigsum[0], rp_tail106(d, j)); /* :1149, no sumfix2 */
+  rdd p[256], small[256];
+  for (int b = 0; b < 256; ++b) { /* Ssum, 65536 -> 256 -> 1 (KernelFuncs.as:62-65) */
+    memcpy(p, bigsum + 256 * b, sizeof p);
+    small[b] = rp_tree256(p);
+  }
+  bigsum[0] = rp_tree256(small);
+}
+
+/* Sum7 (Bellard.cl:1324-1356): part <- frac(numer * part), ans <- sumfix(ans + part). */
+static void rp_sum7_106(rdd *part, rdd *ans, uint64_t numer) {
+  rdd a = *part;
+  if (a.hi < 0.0) {
+    const rdd one = {1.0, 0.0};
+    a = rp_addcab(a, one);
+  }
+  a.hi *= (double)numer;
+  a.lo *= (double)numer;
+  const rdd t = {trunc(a.hi), 0.0};
+  a = rp_subcab(a, t);
+  *part = a;
+  rdd s = rp_addcab(*ans, a);
+  rp_sumfix(&s);
+  *ans = s;
+}
+
+int orc_ref_job106(uint64_t d, uint64_t k_start, uint64_t k_end, int j, uint64_t gsize,
+                   int nthreads, double out[2]) {
+  if (j < 0 || j > 6 || gsize == 0 || gsize % 256 || gsize > 256ull * REF_GROUPS) return -1;
+  if (d * 10 + 9 >= 9007199254740992ull) return -1;
+  rdd *bigsum = calloc(REF_GROUPS, sizeof(rdd));
+  if (!bigsum) return -1;
+  rdd ans = {0.0, 0.0};
+  ref_batch106(d, k_start, k_end, j, gsize, nthreads, bigsum);
+  rp_sum7_106(&bigsum[0], &ans, kNumer[j]);
+  out[0] = bigsum[0].hi;
+  out[1] = bigsum[0].lo;
+  free(bigsum);
+  return 0;
+}
+
+int orc_ref_normal106(uint64_t n, uint64_t gsize, int nthreads, double ans_out[2]) {
+  uint64_t d;
+  if (orc_normalize(n, NULL, &d, NULL)) return -1;
+  if (gsize == 0 || gsize % 256 || gsize > 256ull * REF_GROUPS) return -1;
+  if (d * 10 + 9 >= 9007199254740992ull) return -1;
+  rdd *bigsum = malloc(REF_GROUPS * sizeof(rdd));
+  if (!bigsum) return -1;
+  rdd ans = {0.0, 0.0};
+  for (int j = 0; j < 7; ++j) { /* Bellard_192.hsp:264-296 */
+    memset(bigsum, 0, REF_GROUPS * sizeof(rdd));
+    ref_batch106(d, 0, d, j, gsize, nthreads, bigsum);
+    rp_sum7_106(&bigsum[0], &ans, kNumer[j]);
+  }
+  ans_out[0] = ans.hi;
+  ans_out[1] = ans.lo;
+  free(bigsum);
   return 0;
 }
 

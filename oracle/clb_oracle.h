@@ -32,10 +32,11 @@
  *   orc_hex_dd                          parity unpinned below 2^-100 (reference formatting of
  *                                       double-double values is only pinned on exact inputs)
  *   orc_ref_item192, orc_ref_tail192,
- *   orc_ref_batch192, orc_ref_normal192 the reference kernels' own results (GPU runs of
- *                                       oracle/_ref, committed as tests/golden/ref_dd192.json):
- *                                       bit-exact; and within the double-double bound of the
- *                                       exact sums
+ *   orc_ref_batch192, orc_ref_normal192,
+ *   orc_ref_job106, orc_ref_normal106   the reference kernels' own results (GPU runs of
+ *                                       oracle/_ref, committed as tests/golden/ref_dd192.json and
+ *                                       ref_dd106.json): bit-exact; and within the double-double
+ *                                       bound of the exact sums
  */
 #ifndef CLB_ORACLE_H
 #define CLB_ORACLE_H
@@ -112,6 +113,16 @@ int orc_ref_batch192(uint64_t d, uint64_t k_start, uint64_t k_end, int j, int pr
                      uint64_t gsize, int nthreads, uint64_t out[3]);
 /* Allloop (Bellard_192.hsp:264-296) on those kernels. */
 int orc_ref_normal192(uint64_t n, int precision, uint64_t gsize, int nthreads, uint64_t ans[3]);
+
+/* The 106-bit mode (precision 0) replayed: Set0memdd, BatchCalc with Sglobal32mtg / Sglobal64mtg
+ * (Bellard.cl:729-766, :914-985), their 256-lane trees and the 65536 per-group accumulators,
+ * Sglobal_after (:1118-1150), two Ssum trees (:1274-1297), then Sum7 (:1324-1356); out = bigSum[0]
+ * after Sum7's update (what the reference's work mode saves), as (hi, lo).  gsize: work-items per
+ * launch (a multiple of 256, at most 2^24). */
+int orc_ref_job106(uint64_t d, uint64_t k_start, uint64_t k_end, int j, uint64_t gsize,
+                   int nthreads, double out[2]);
+/* Allloop in the 106-bit mode: dev_ans as (hi, lo). */
+int orc_ref_normal106(uint64_t n, uint64_t gsize, int nthreads, double ans[2]);
 
 /* Bellard_192.hsp:354-359 + hexsademical.as:104-127: 48 lowercase hex digits, w[0] first. */
 void orc_hex48(const uint64_t w[3], char out[49]);

@@ -64,6 +64,10 @@ class Oracle:
             "orc_ref_batch192": ([u64, u64, u64, ctypes.c_int, ctypes.c_int, u64, ctypes.c_int,
                                   U3], ctypes.c_int),
             "orc_ref_normal192": ([u64, ctypes.c_int, u64, ctypes.c_int, U3], ctypes.c_int),
+            "orc_ref_job106": ([u64, u64, u64, ctypes.c_int, u64, ctypes.c_int,
+                                ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+            "orc_ref_normal106": ([u64, u64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
+                                  ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -138,6 +142,19 @@ class Oracle:
         rc = self.lib.orc_ref_batch192(d, k0, k1, j, precision, gsize, threads or self.threads, q)
         assert rc == 0
         return w2i(q)
+
+    def ref_job106(self, d, k0, k1, j, gsize=1 << 24, threads=None):
+        """(hi, lo) of bigSum[0] after BatchCalc and Sum7 in the 106-bit mode."""
+        out = (ctypes.c_double * 2)()
+        rc = self.lib.orc_ref_job106(d, k0, k1, j, gsize, threads or self.threads, out)
+        assert rc == 0
+        return out[0], out[1]
+
+    def ref_normal106(self, n, gsize=1 << 24, threads=None):
+        out = (ctypes.c_double * 2)()
+        rc = self.lib.orc_ref_normal106(n, gsize, threads or self.threads, out)
+        assert rc == 0
+        return out[0], out[1]
 
     def ref_normal192(self, n, precision, gsize=1 << 24, threads=None):
         q = U3()
@@ -263,6 +280,7 @@ def loopmax32(j: int) -> int:
 # work-items; smaller launches (its kernels stride by the global size) give every work-item its
 # full 16 or 32 terms on ranges the CPU restatement finishes in seconds.
 REF_DD_GOLDEN = os.path.join(GOLDEN, "ref_dd192.json")
+REF_DD106_GOLDEN = os.path.join(GOLDEN, "ref_dd106.json")
 
 
 def ref_dd_cases():
@@ -287,6 +305,24 @@ def ref_dd_cases():
         cases.append((d12, d12 // 2 | 1, d12 // 2 + 8192, j, 1, 256))
         cases.append((d12, 0, 6000, j, 1, 256))                   # the smallest moduli
     cases.append((d6, d6, d6, 2, 1, 1 << 24))                     # the tail alone
+    return cases
+
+
+def ref_dd106_cases():
+    """106-bit mode jobs (d, k_start, k_end, j, work-items per launch), small enough for the CPU
+    restatement's literal 65536-group trees."""
+    d6, d7 = py_normalize(10**6)[1], py_normalize(10**7)[1]
+    d10, d12 = py_normalize(25 * 10**9)[1], py_normalize(10**12)[1]
+    cases = []
+    for j in range(7):
+        cases.append((d6, 0, d6, j, 1 << 24))   # one launch, the tail
+        cases.append((d6, 0, d6, j, 4096))      # 16 terms per work-item, 7 launches
+    for j in (0, 5):
+        cases.append((d7, 777, d7 // 3, j, 2048))
+        lo = loopmax32(j)
+        cases.append((d10, lo - 3001, lo + 9000, j, 256))     # Sglobal32mtg, then Sglobal64mtg
+        cases.append((d10, d10 - 5000, d10 + 3, j, 512))      # Sglobal64mtg and the tail
+        cases.append((d12, d12 // 2 | 1, d12 // 2 + 8192, j, 256))
     return cases
 
 

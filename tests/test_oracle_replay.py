@@ -1,13 +1,14 @@
-"""CPU: the double-double replay of the reference's precision-1/2 kernels (orc_ref_batch192)
-against the reference kernels' own results in tests/golden/ref_dd192.json (written by
+"""CPU: the double-double replay of the reference's kernels (orc_ref_batch192 for precision 1/2,
+orc_ref_job106 / orc_ref_normal106 for the 106-bit mode) against the reference kernels' own
+results in tests/golden/ref_dd192.json and ref_dd106.json (written by
 tests/golden/make_ref_dd_fixtures.py on an MI355X with oracle/_ref), bit for bit, and the
 replay's distance from the exact sums within the double-double error the reference carries."""
 import json
 
 import pytest
 
-from clb_testlib import (M192, REF_DD_GOLDEN, SHIFT, load_oracle, py_batch_calc, py_normalize,
-                         ref_dd_cases)
+from clb_testlib import (M192, REF_DD106_GOLDEN, REF_DD_GOLDEN, SHIFT, load_oracle, py_batch_calc,
+                         py_normalize, ref_dd106_cases, ref_dd_cases)
 
 
 @pytest.fixture(scope="module")
@@ -48,3 +49,25 @@ def test_replay_close_to_exact(orc, gws):
         diff = (orc.ref_batch192(d, 0, d, j, 1, gws) - exact) & M192
         diff = diff - (1 << 192) if diff >> 191 else diff
         assert abs(diff) <= 2 * (d + 20), (j, diff)
+
+
+def golden106():
+    with open(REF_DD106_GOLDEN) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("case", golden106()["cases"],
+                         ids=lambda c: "d{d}_k{k0}-{k1}_j{j}_g{gws}".format(**c))
+def test_106bit_replay_equals_reference_kernels(orc, case):
+    want = (float.fromhex(case["hi"]), float.fromhex(case["lo"]))
+    assert orc.ref_job106(case["d"], case["k0"], case["k1"], case["j"], case["gws"]) == want
+
+
+@pytest.mark.parametrize("row", golden106()["positions"], ids=lambda r: f"n{r['n']}")
+def test_106bit_positions_equal_reference_kernels(orc, row):
+    assert orc.ref_normal106(row["n"]) == (float.fromhex(row["hi"]), float.fromhex(row["lo"]))
+
+
+def test_106bit_fixture_covers_the_cases():
+    have = {(c["d"], c["k0"], c["k1"], c["j"], c["gws"]) for c in golden106()["cases"]}
+    assert have == set(ref_dd106_cases())

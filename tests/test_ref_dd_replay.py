@@ -14,7 +14,7 @@ import pytest
 
 import clbellard_amd as clb
 from clb_testlib import (M192, REF_LIB, SHIFT, RefKernels, load_oracle, py_normalize,
-                         ref_dd_cases, w2i)
+                         ref_dd106_cases, ref_dd_cases, w2i)
 
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not os.path.exists(REF_LIB),
@@ -59,6 +59,20 @@ def test_oracle_replay_equals_reference_kernels(ref, orc, case):
     got = w2i(ref.job(d, k0, k1, j, p))
     want = (orc.ref_batch192(d, k0, k1, j, p, gws) << SHIFT[j]) & M192
     assert got == want, f"{got:048x} != {want:048x}"
+
+
+@pytest.mark.parametrize("case", ref_dd106_cases(), ids=lambda c: "d{}_k{}-{}_j{}_g{}".format(*c))
+def test_oracle_106bit_replay_equals_reference_kernels(ref, orc, case):
+    d, k0, k1, j, gws = case
+    note(f"reference kernels, 106-bit mode, d={d} k=[{k0},{k1}) j={j} gws={gws}")
+    ref.set_gws(gws)
+    assert ref.job(d, k0, k1, j, 0) == orc.ref_job106(d, k0, k1, j, gws)
+
+
+@pytest.mark.parametrize("n", [10**6, 10**7])
+def test_oracle_106bit_positions_equal_reference_kernels(ref, orc, n):
+    ref.set_gws()
+    assert ref.normal(py_normalize(n)[1], 0) == orc.ref_normal106(n)
 
 
 @pytest.mark.parametrize("engine", ["auto", "plain"])
