@@ -63,6 +63,7 @@ struct HostJob {
   uint32_t numesign;
   uint32_t slot;
   bool tail;
+  bool single = false;  // one kernel launch over [k0, k1) (clb_sglobal), not BatchCalc's chunks
 };
 
 // Device buffer that grows geometrically, so that its address (which a captured graph records)
@@ -140,6 +141,12 @@ struct clb_ctx {
   // Reference replay: 0, or the work-items per replayed launch of the reference's kernels.
   uint64_t replay_gws = 0;
   int replay_resident[5] = {-1, -1, -1, -1, -1};  // resident k_replay workgroups per CU, per kind
+  // 106-bit-mode replay: the call's tables and results, the jobs' group accumulators, and the
+  // context's own bigSumMem (65536 double-doubles) for BatchCalc / AddAns at the operator level.
+  DevBuf<uint8_t> r106_block;
+  HostBuf r106_host;
+  DevBuf<uint8_t> r106_bigsum;
+  DevBuf<uint8_t> r106_state;
   struct Graph {
     uintptr_t key[4];
     hipGraphExec_t exec;
@@ -253,7 +260,15 @@ int expand_replay(const clb_ctx* ctx, const std::vector<HostJob>& in, std::vecto
       flat += (items + kWave - 1) / kWave;
       rjobs->push_back(r);
     };
-    if (h.k0 < h.k1) {
+    if (h.k0 < h.k1 && h.single) {  // one kernel launch, the precision's kernel for m(k0)
+      if (ctx->precision == CLB_PREC_U192_BETA && h.den0 * h.k0 + h.den1 >= (1ull << 32)) {
+        HostJob e = h;
+        e.tail = false;
+        exact->push_back(e);
+      } else {
+        launch(h.k1);
+      }
+    } else if (h.k0 < h.k1) {
       const uint64_t x = ((1ull << 32) - h.den1 - 1) / h.den0 + 1;  // KernelFuncs.as:31-33
       const uint64_t lm32 = x < h.k1 ? x : h.k1;
       while (offset < lm32) {  // :34-41
@@ -282,6 +297,153 @@ int expand_replay(const clb_ctx* ctx, const std::vector<HostJob>& in, std::vecto
   return CLB_OK;
 }
 
+constexpr size_t kBigSumBytes = (size_t)kRefGroups * 2 * sizeof(double);
+
+// The context's bigSumMem for the 106-bit-mode replay, allocated and cleared on first use.
+int r106_state(clb_ctx* ctx, double** out) {
+  if (!ctx->r106_state.p) {
+    if (int rc = ctx->r106_state.reserve(kBigSumBytes)) return rc;
+    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long*>(ctx->r106_state.p),
+                        kBigSumBytes / 8, ctx->stream));
+  }
+  *out = reinterpret_cast<double*>(ctx->r106_state.p);
+  return CLB_OK;
+}
+
+// 3-word result from a double-double: the bit patterns of (hi, lo) and a zero word.
+void dd_words(const double* dd, uint64_t w[3]) {
+  memcpy(&w[0], &dd[0], 8);
+  memcpy(&w[1], &dd[1], 8);
+  w[2] = 0;
+}
+
+// Reference replay of the 106-bit mode (precision 0): each BatchCalc job on its own bigSumMem
+// (cleared, as Set0memdd before every BatchCalc of Allloop and work mode, Bellard_192.hsp:270,
+// :311), or, with accumulate_dev (clb_batch_calc), the one job on the context's bigSumMem with
+// bigSum[0] copied to accumulate_dev; then AddAns per group (Sum7 in formula order).  Results are
+// double-doubles, in the 3-word outputs as dd_words.
+int run_replay106(clb_ctx* ctx, const std::vector<HostJob>& hjobs,
+                  const std::vector<uint8_t>& shift, const std::vector<GroupDev>& groups,
+                  uint64_t* out_host, uint64_t* accumulate_dev, uint64_t* slot_host,
+                  const Mirrors& mirrors) {
+  const auto t_call = std::chrono::steady_clock::now();
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  const uint64_t gws = ctx->replay_gws;
+  const bool op_level = accumulate_dev != nullptr;
+  const size_t nslots = hjobs.size();
+  uint64_t max_m = 1;
+  for (const HostJob& h : hjobs) {
+    if (h.d > ((1ull << 53) - 10) / 10)
+      return fail(CLB_ERANGE, "replay: d = %llu needs the reference's OVERDOUBLE build, which "
+                  "is not replayed", (unsigned long long)h.d);
+    if (h.single)
+      return fail(CLB_EINVAL, "replay of the 106-bit mode: kernel-level operators are not "
+                  "replayed (use clb_batch_calc)");
+    if (h.k0 < h.k1 && h.den0 * (h.k1 - 1) + h.den1 > max_m) max_m = h.den0 * (h.k1 - 1) + h.den1;
+  }
+  if (op_level && nslots != 1) return fail(CLB_EINVAL, "replay: one job per operator call");
+  int kind = kPlainW3;
+  if ((rc = choose_kind(ctx->kernel, max_m, &kind))) return rc;
+  if (kind == kPlainWide) kind = kPlainW3;  // replayed moduli are below 2^53
+  const size_t ngr = groups.size();
+  auto align64 = [](size_t x) { return (x + 63) & ~(size_t)63; };
+  const size_t off_jobs = 0;
+  const size_t off_parts = align64(nslots * sizeof(R106JobDev));
+  const size_t off_first = off_parts + align64(nslots * 16);
+  const size_t off_count = off_first + align64(ngr * 4);
+  const size_t off_shift = off_count + align64(ngr * 4);
+  const size_t off_ans = off_shift + align64(nslots);
+  const size_t off_slot = off_ans + align64(ngr * 16);
+  const size_t bytes = off_slot + nslots * 16;
+  if ((rc = ctx->r106_block.reserve(bytes)) || (rc = ctx->r106_host.reserve(bytes))) return rc;
+  constexpr size_t kChunk = 64;  // jobs whose 1 MB bigSumMem are live at once
+  double* state = nullptr;
+  if (op_level && (rc = r106_state(ctx, &state))) return rc;
+  if (!op_level && (rc = ctx->r106_bigsum.reserve(kChunk * kBigSumBytes))) return rc;
+  uint8_t* dev = ctx->r106_block.p;
+  uint8_t* stage = ctx->r106_host.p;
+  std::vector<R106JobDev> jobs(nslots);
+  for (size_t i = 0; i < nslots; ++i) {
+    const HostJob& h = hjobs[i];
+    R106JobDev& j = jobs[i];
+    j.d = h.d;
+    j.k0 = h.k0;
+    j.k1 = h.k1;
+    j.den0 = h.den0;
+    j.den1 = h.den1;
+    j.bigsum = (uint64_t)(uintptr_t)(op_level ? reinterpret_cast<uint8_t*>(state)
+                                              : ctx->r106_bigsum.p + (i % kChunk) * kBigSumBytes);
+    j.part = (uint64_t)(uintptr_t)(dev + off_parts + 16 * i);
+    j.numesign = h.numesign;
+    j.tail = h.tail;
+  }
+  memcpy(stage + off_jobs, jobs.data(), nslots * sizeof(R106JobDev));
+  for (size_t g = 0; g < ngr; ++g) {
+    memcpy(stage + off_first + 4 * g, &groups[g].first, 4);
+    memcpy(stage + off_count + 4 * g, &groups[g].count, 4);
+  }
+  if (nslots) memcpy(stage + off_shift, shift.data(), nslots);
+  memset(stage + off_ans, 0, ngr * 16);
+  hipStream_t s = ctx->stream;
+  HIP_TRY(hipMemcpyAsync(dev, stage, off_slot, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipEventRecord(ctx->ev[0], s));
+  const R106JobDev* djobs = reinterpret_cast<const R106JobDev*>(dev + off_jobs);
+  for (size_t c = 0; c < nslots; c += kChunk) {
+    const size_t n = nslots - c < kChunk ? nslots - c : kChunk;
+    if (!op_level)
+      HIP_TRY(launch_zero(reinterpret_cast<unsigned long long*>(ctx->r106_bigsum.p),
+                          n * kBigSumBytes / 8, s));
+    uint64_t most = 0;  // work-items of the chunk's largest launch
+    for (size_t i = c; i < c + n; ++i) {
+      const uint64_t span = hjobs[i].k1 > hjobs[i].k0 ? hjobs[i].k1 - hjobs[i].k0 : 0;
+      const uint64_t items = span < gws ? span : gws;
+      if (items > most) most = items;
+    }
+    HIP_TRY(launch_replay106(kind, djobs + c, (uint32_t)n, gws, (uint32_t)((most + 255) / 256), s));
+  }
+  HIP_TRY(hipEventRecord(ctx->ev[1], s));
+  const double* dparts = reinterpret_cast<const double*>(dev + off_parts);
+  double* dans = reinterpret_cast<double*>(dev + off_ans);
+  double* dslot = reinterpret_cast<double*>(dev + off_slot);
+  if (op_level) {
+    HIP_TRY(hipMemcpyAsync(accumulate_dev, dparts, 16, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long*>(accumulate_dev + 2), 1, s));
+  } else {
+    HIP_TRY(launch_replay106_sum7(dparts, reinterpret_cast<const uint32_t*>(dev + off_first),
+                                  reinterpret_cast<const uint32_t*>(dev + off_count),
+                                  dev + off_shift, dans, dslot, (uint32_t)ngr, s));
+    if (mirrors.total) {
+      HIP_TRY(hipMemcpyAsync(mirrors.total, dans, 16, hipMemcpyDeviceToDevice, s));
+      HIP_TRY(launch_zero(reinterpret_cast<unsigned long long*>(mirrors.total + 2), 1, s));
+    }
+    if (mirrors.slot) {
+      HIP_TRY(hipMemcpyAsync(mirrors.slot, dslot + 2 * mirrors.index, 16,
+                             hipMemcpyDeviceToDevice, s));
+      HIP_TRY(launch_zero(reinterpret_cast<unsigned long long*>(mirrors.slot + 2), 1, s));
+    }
+    HIP_TRY(hipMemcpyAsync(stage + off_ans, dev + off_ans, bytes - off_ans,
+                           hipMemcpyDeviceToHost, s));
+  }
+  hipError_t e2 = hipStreamSynchronize(s);
+  float head = 0.f;
+  if (e2 == hipSuccess) e2 = hipEventElapsedTime(&head, ctx->ev[0], ctx->ev[1]);
+  if (e2 != hipSuccess) return fail(CLB_EDEVICE, "replay failed: %s", hipGetErrorString(e2));
+  if (!op_level) {
+    const double* ans = reinterpret_cast<const double*>(stage + off_ans);
+    const double* sv = reinterpret_cast<const double*>(stage + off_slot);
+    if (out_host)
+      for (size_t g = 0; g < ngr; ++g) dd_words(ans + 2 * g, out_host + 3 * g);
+    if (slot_host)
+      for (size_t i = 0; i < nslots; ++i) dd_words(sv + 2 * i, slot_host + 3 * i);
+  }
+  ctx->head_ms = head;
+  ctx->total_ms = (float)std::chrono::duration<double, std::milli>(
+                      std::chrono::steady_clock::now() - t_call)
+                      .count();
+  return CLB_OK;
+}
+
 // Runs head jobs + tails into slots [0, nslots), then combines the slots by groups.
 // out_host receives 3 words per group unless accumulate_dev is given, in which case the single
 // group's value is added into that device buffer.  slot_host (optional) receives each slot's
@@ -289,6 +451,8 @@ int expand_replay(const clb_ctx* ctx, const std::vector<HostJob>& in, std::vecto
 int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8_t>& shift,
         const std::vector<GroupDev>& groups, uint64_t* out_host, uint64_t* accumulate_dev,
         uint64_t* slot_host, const Mirrors& mirrors = Mirrors()) {
+  if (ctx->replay_gws && ctx->precision == CLB_PREC_DD106)
+    return run_replay106(ctx, hjobs, shift, groups, out_host, accumulate_dev, slot_host, mirrors);
   const auto t_call = std::chrono::steady_clock::now();
   int rc = set_device(ctx);
   if (rc) return rc;
@@ -514,6 +678,12 @@ int check_ctx(clb_ctx* ctx) {
   return CLB_OK;
 }
 
+// How a result computed in `precision` on ctx is printed: the 106-bit-mode replay returns
+// double-double bits.
+int out_format(const clb_ctx* ctx, int precision) {
+  return ctx->replay_gws && precision == CLB_PREC_DD106 ? 3 : precision;
+}
+
 int normalize(uint64_t n_in, uint64_t* n_out, uint64_t* d_out, int* overdouble) {
   if (n_in <= 4 || n_in > kMaxDigits)
     return fail(CLB_EINVAL, "invalid digits %llu (need 5 <= N <= %llu)",
@@ -724,8 +894,16 @@ int make_dir(const std::string& p) {
   return fail(CLB_EIO, "cannot create %s: %s", p.c_str(), strerror(errno));
 }
 
+// Internal format code: the 3 words hold a double-double's bit patterns (dd_words), as the
+// 106-bit-mode replay returns its results.
+constexpr int kDDWords = 3;
+
 void format_result(int precision, const uint64_t w[3], char hex[49]) {
-  if (precision == CLB_PREC_DD106) {
+  if (precision == kDDWords) {
+    double dd[2];
+    memcpy(dd, w, sizeof dd);
+    hex_dd(dd, hex);
+  } else if (precision == CLB_PREC_DD106) {
     double dd[2];
     u192_to_dd(w, dd);
     hex_dd(dd, hex);
@@ -841,6 +1019,10 @@ int clb_ctx_destroy(clb_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   ctx->block.release();
   ctx->block_host.release();
+  ctx->r106_block.release();
+  ctx->r106_host.release();
+  ctx->r106_bigsum.release();
+  ctx->r106_state.release();
   for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
   ctx->graphs.clear();
   if (ctx->partial) (void)hipFree(ctx->partial);
@@ -959,8 +1141,6 @@ int clb_set_replay(clb_ctx* ctx, uint64_t global_work_size) {
     ctx->replay_gws = 0;
     return CLB_OK;
   }
-  if (ctx->precision != CLB_PREC_U192 && ctx->precision != CLB_PREC_U192_BETA)
-    return fail(CLB_EINVAL, "replay needs precision 1 or 2 (the 106-bit mode is not replayed)");
   if (global_work_size % 256 || global_work_size > kRefGlobalWorkSize)
     return fail(CLB_EINVAL, "replay: %llu work-items (a multiple of 256 up to 2^24)",
                 (unsigned long long)global_work_size);
@@ -999,6 +1179,9 @@ int clb_set0mem(clb_ctx* ctx) {
   if (int rc = check_ctx(ctx)) return rc;
   if (int rc = set_device(ctx)) return rc;
   HIP_TRY(launch_zero(reinterpret_cast<unsigned long long*>(ctx->partial), 3, ctx->stream));
+  if (ctx->r106_state.p)  // the 106-bit-mode replay's bigSumMem (Set0memdd, KernelFuncs.as:20-23)
+    HIP_TRY(launch_zero(reinterpret_cast<unsigned long long*>(ctx->r106_state.p),
+                        kBigSumBytes / 8, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   return CLB_OK;
 }
@@ -1049,6 +1232,7 @@ int clb_sglobal(clb_ctx* ctx, uint64_t offset, uint64_t d, uint64_t k_max, int64
   h.k0 = offset < k_max ? offset : 0;
   h.k1 = offset < k_max ? k_max : 0;
   h.tail = false;
+  h.single = true;
   return run_into_partial(ctx, h);
 }
 
@@ -1061,6 +1245,7 @@ int clb_sglobal_after(clb_ctx* ctx, uint64_t d, int64_t numesign, uint64_t den0,
   h.den1 = den1;
   h.numesign = (uint32_t)numesign;
   h.tail = true;
+  h.single = true;
   return run_into_partial(ctx, h);
 }
 
@@ -1068,6 +1253,24 @@ int clb_add_ans(clb_ctx* ctx, int j) {
   if (int rc = check_ctx(ctx)) return rc;
   if (j < 0 || j > 6) return fail(CLB_EINVAL, "formula %d (need 0..6)", j);
   if (int rc = set_device(ctx)) return rc;
+  if (ctx->replay_gws && ctx->precision == CLB_PREC_DD106) {
+    // Sum7 (Bellard.cl:1324-1356) on the context's bigSum[0] and dev_ans, double-doubles.
+    double* state = nullptr;
+    if (int rc = r106_state(ctx, &state)) return rc;
+    if (int rc = ctx->r106_block.reserve(64)) return rc;
+    if (int rc = ctx->r106_host.reserve(64)) return rc;
+    const uint32_t tab[2] = {0, 1};
+    memcpy(ctx->r106_host.p, tab, 8);
+    ctx->r106_host.p[8] = kShift[j];
+    uint8_t* dev = ctx->r106_block.p;
+    HIP_TRY(hipMemcpyAsync(dev, ctx->r106_host.p, 9, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_replay106_sum7(state, reinterpret_cast<const uint32_t*>(dev),
+                                  reinterpret_cast<const uint32_t*>(dev + 4), dev + 8,
+                                  reinterpret_cast<double*>(ctx->ans), state, 1, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->partial, state, 16, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return CLB_OK;
+  }
   HIP_TRY(launch_sum7(ctx->ans, ctx->partial, kShift[j], ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   return CLB_OK;
@@ -1105,7 +1308,7 @@ int clb_normal(clb_ctx* ctx, uint64_t n, uint64_t frac[3], char hex[49], uint64_
   mirrors.index = 6;
   if (int rc = run(ctx, jobs, shift, groups, res, nullptr, nullptr, mirrors)) return rc;
   if (frac) memcpy(frac, res, sizeof res);
-  if (hex) format_result(ctx->precision, res, hex);
+  if (hex) format_result(out_format(ctx, ctx->precision), res, hex);
   if (n_out) *n_out = nn;
   return CLB_OK;
 }
@@ -1278,7 +1481,8 @@ int clb_run_worklist(clb_ctx* ctx, const char* worklist_path, const char* work_d
                    : std::to_string(ctx->device) + "\t" + ctx->device_name;
   for (size_t i = 0; i < todo.size(); ++i) {
     const int64_t b = (int64_t)(todo[i].k_begin / batch);
-    if ((rc = write_work_file(work_dir, formulas[i], b, &part[3 * i], precision, label, ms)))
+    if ((rc = write_work_file(work_dir, formulas[i], b, &part[3 * i], out_format(ctx, precision),
+                              label, ms)))
       return rc;
     if (jobs_run) ++*jobs_run;
   }

@@ -138,9 +138,28 @@ hipError_t launch_sum7(uint64_t* ans, uint64_t* partial, int shift, hipStream_t 
 hipError_t launch_selfcheck(uint64_t* out2, hipStream_t stream);
 
 // Reference replay (clb_replay.hip): persistent grid over the replay tiles and tails of `hdr`.
+// Reference replay of the 106-bit mode (precision 0): per BatchCalc job, the reference's 65536
+// double-double group accumulators (bigSumMem, Bellard_192.hsp:409-411) in device memory.
+constexpr uint32_t kRefGroups = 65536;  // BIGSUM_SIZE, Bellard_192.hsp:5
+struct R106JobDev {
+  uint64_t d, k0, k1;    // BatchCalc head range after the clamp to d (k0 >= k1: no head terms)
+  uint64_t den0, den1;
+  uint64_t bigsum;       // device address of the job's kRefGroups accumulators (hi, lo doubles)
+  uint64_t part;         // device address receiving bigSum[0] after Ssum (hi, lo)
+  uint32_t numesign, tail;
+};
 // kind: kSplitW3/W2/W1 (fp64 residues) or kPlainW3 (integer residues), by the largest modulus.
 int replay_blocks_per_cu(int kind);
 hipError_t launch_replay(int kind, LaunchDev* hdr, int grid, hipStream_t stream);
+// 106-bit mode: the group accumulators of jobs[0..njobs) (kind as above), then per job the tail
+// and the two Ssum trees (writing each job's part), then AddAns per result group: ans[g] (in: the
+// starting answer, out: the answer) and slot_val[s] (out: part after Sum7's update), where group g
+// holds slots [first[g], first[g] + count[g]) with numerators 2^shift[s].  Doubles in (hi, lo).
+hipError_t launch_replay106(int kind, const R106JobDev* jobs, uint32_t njobs, uint64_t gsize,
+                            uint32_t ngroups_max, hipStream_t stream);
+hipError_t launch_replay106_sum7(const double* parts, const uint32_t* first, const uint32_t* count,
+                                 const uint8_t* shift, double* ans, double* slot_val,
+                                 uint32_t ngroups, hipStream_t stream);
 constexpr uint64_t kRefGlobalWorkSize = 65536ull * 256ull;  // Bellard_192.hsp:5-7
 
 }  // namespace clb

@@ -267,7 +267,192 @@ __device__ __forceinline__ void flush(U192& v, int lane, unsigned long long* slo
   v = {0, 0, 0};
 }
 
+// ------------------------------------------------------------------ 106-bit mode
+
+__device__ __forceinline__ DD dd_sub(DD a, DD b) {  // subcab, :37-43
+  const DD c = two_sum(a.hi, -b.hi);
+  return two_sum(c.hi, (c.lo + a.lo) - b.lo);
+}
+
+__device__ __forceinline__ void fix_one(DD& s) {  // sumfix, :491-501
+  if (s.hi >= 1.0) {
+    const DD t = two_sum(s.hi, -1.0);
+    s = two_sum(t.hi, t.lo + s.lo);
+  }
+}
+
+// r = 2^e mod m alone, exact (see residues).
+template <int KIND>
+__device__ __forceinline__ uint64_t residue(uint64_t e, int bl, uint64_t m) {
+  if constexpr (KIND == kPlainW3) {
+    const uint64_t mp = neg_inv64(m);
+    const uint64_t one = (0ull - m) % m;
+    uint64_t r = redc(0, ipow2_mont<3>(e, bl, m, mp, one), m, mp);
+    return r >= m ? r - m : r;
+  } else {
+    constexpr int W = KIND == kSplitW3 ? 3 : KIND == kSplitW2 ? 2 : 1;
+    const double mf = (double)m;
+    const int s = bl > 3 ? W * ((bl - 3) / W) : 0;
+    const double x = fpow2<W>(e, s, mf, recip(mf));
+    return (uint64_t)(x < 0.0 ? x + mf : x);
+  }
+}
+
+// The 256 work-items of one work-group reduced in LDS the reference's way (shared_reduction_dd,
+// :583-597, and Ssum's tree, :1274-1297): p[l] = sumfix2(p[l] + p[l + i]), i = 128 .. 1.
+__device__ __forceinline__ void tree256(DD* p, uint32_t tid) {
+  for (uint32_t i = 128; i > 0; i >>= 1) {
+    __syncthreads();
+    if (tid < i) {
+      DD t = dd_add(p[tid], p[tid + i]);
+      fix_unit(t);
+      p[tid] = t;
+    }
+  }
+  __syncthreads();
+}
+
+// Work-item 256 g + tid of one 106-bit-mode launch: Sglobal32mtg (:729-752, sumfix after each
+// term) or Sglobal64mtg (:914-975, the steered numerator), and the sign of its terms.
+template <int KIND>
+__device__ DD item106(const R106JobDev& jb, uint64_t offset, uint64_t k_max, uint64_t gsize,
+                      uint32_t g, uint32_t tid, bool is64) {
+  DD s = {0.0, 0.0};
+  const uint64_t idx = (uint64_t)g * 256 + tid;
+  const uint64_t lane = tid & (kWave - 1);
+  for (uint64_t kw = offset + (idx - lane); kw < k_max; kw += gsize) {  // kw: the wave's lane 0
+    const uint64_t k = kw + lane;
+    if (k < k_max) {
+      const uint64_t m = jb.den0 * k + jb.den1;
+      const int bl = 64 - __builtin_clzll(10ull * (jb.d - kw));
+      const uint64_t r = residue<KIND>(10ull * (jb.d - k), bl, m);
+      if (is64) {
+        dd_step(s, r, m);
+      } else {
+        s = dd_add(s, div_nn((double)r, (double)m));
+        fix_one(s);
+      }
+    }
+  }
+  if ((jb.numesign + (offset + idx) % 2) % 2 == 1) s = dd_neg(s);
+  return s;
+}
+
+// Sglobal_after (:1118-1150): k = d .. d+9, numerators 2^(-10 i), the formula's sign.
+__device__ DD tail106(const R106JobDev& t) {
+  DD s = {0.0, 0.0};
+  double n = 1.0;
+  for (uint64_t i = 0; i < 10; ++i) {
+    const uint64_t k = t.d + i;
+    DD q = div_nd(n, dd_of(t.den0 * k + t.den1));
+    if (k % 2 == 1) q = dd_neg(q);
+    s = dd_add(s, q);
+    n /= 1024.0;
+  }
+  if ((t.numesign + 2) % 2 == 1) s = dd_neg(s);
+  return s;
+}
+
 }  // namespace
+
+// Group g (blockIdx.x) of job blockIdx.y: the job's launches in the reference's order (GWS * 16
+// terms below loopmax32, GWS * 32 above, KernelFuncs.as:31-53), each reduced over the group's 256
+// work-items and added into the group's accumulator (:760-764, :979-983).  A group beyond every
+// launch's work-items keeps its accumulator (adding the zero tree of its idle work-items leaves
+// a double-double unchanged).
+template <int KIND>
+__global__ void __launch_bounds__(256) k_r106_groups(const R106JobDev* __restrict__ jobs,
+                                                    uint64_t gsize) {
+  __shared__ DD p[256];
+  const R106JobDev jb = jobs[blockIdx.y];
+  const uint32_t g = blockIdx.x, tid = threadIdx.x;
+  const uint64_t span = jb.k1 > jb.k0 ? jb.k1 - jb.k0 : 0;
+  if ((uint64_t)g * 256 >= (span < gsize ? span : gsize)) return;
+  double* acc_ptr = reinterpret_cast<double*>(jb.bigsum) + 2 * (size_t)g;
+  DD acc = {acc_ptr[0], acc_ptr[1]};
+  uint64_t offset = jb.k0, k_max = jb.k0;
+  const uint64_t x32 = ((1ull << 32) - jb.den1 - 1) / jb.den0 + 1;
+  const uint64_t lm32 = x32 < jb.k1 ? x32 : jb.k1;
+  for (int phase = 0; phase < 2; ++phase) {
+    const uint64_t lim = phase == 0 ? lm32 : jb.k1;
+    const uint64_t step = gsize * (phase == 0 ? 16 : 32);
+    while (offset < lim) {
+      k_max = k_max + step < lim ? k_max + step : lim;
+      const uint64_t items = k_max - offset < gsize ? k_max - offset : gsize;
+      if ((uint64_t)g * 256 < items) {
+        p[tid] = item106<KIND>(jb, offset, k_max, gsize, g, tid, phase == 1);
+        tree256(p, tid);
+        if (tid == 0) {
+          DD t = dd_add(acc, p[0]);
+          fix_unit(t);
+          acc = t;
+        }
+        __syncthreads();
+      }
+      offset = k_max;
+    }
+  }
+  if (tid == 0) {
+    acc_ptr[0] = acc.hi;
+    acc_ptr[1] = acc.lo;
+  }
+}
+
+// Job blockIdx.x: the tail into bigSum[0] without sumfix2 (:1149), then Ssum 65536 -> 256 -> 1
+// (KernelFuncs.as:62-65); bigSum[0] and the job's part receive the total.
+__global__ void __launch_bounds__(256) k_r106_finish(const R106JobDev* __restrict__ jobs) {
+  __shared__ DD p[256];
+  __shared__ DD small[256];
+  const R106JobDev jb = jobs[blockIdx.x];
+  const uint32_t tid = threadIdx.x;
+  double* bs = reinterpret_cast<double*>(jb.bigsum);
+  if (tid == 0 && jb.tail) {
+    const DD b0 = dd_add({bs[0], bs[1]}, tail106(jb));
+    bs[0] = b0.hi;
+    bs[1] = b0.lo;
+  }
+  __syncthreads();
+  for (uint32_t b = 0; b < 256; ++b) {
+    const size_t i = 2 * ((size_t)b * 256 + tid);
+    p[tid] = {bs[i], bs[i + 1]};
+    tree256(p, tid);
+    if (tid == 0) small[b] = p[0];
+    __syncthreads();
+  }
+  p[tid] = small[tid];
+  tree256(p, tid);
+  if (tid == 0) {
+    double* part = reinterpret_cast<double*>(jb.part);
+    bs[0] = part[0] = p[0].hi;
+    bs[1] = part[1] = p[0].lo;
+  }
+}
+
+// AddAns (Sum7, :1324-1356) per result group g: for each of its slots in order, part <- frac(numer
+// part) (written to slot_val) and ans <- sumfix(ans + part).
+__global__ void k_r106_sum7(const double* parts, const uint32_t* first, const uint32_t* count,
+                            const uint8_t* shift, double* ans, double* slot_val,
+                            uint32_t ngroups) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngroups) return;
+  DD a = {ans[2 * g], ans[2 * g + 1]};
+  for (uint32_t i = 0; i < count[g]; ++i) {
+    const size_t s = first[g] + i;
+    DD v = {parts[2 * s], parts[2 * s + 1]};
+    if (v.hi < 0.0) v = dd_add(v, {1.0, 0.0});
+    const double numer = (double)(1u << shift[s]);
+    v.hi *= numer;
+    v.lo *= numer;
+    v = dd_sub(v, {__builtin_trunc(v.hi), 0.0});
+    slot_val[2 * s] = v.hi;
+    slot_val[2 * s + 1] = v.lo;
+    DD t = dd_add(a, v);
+    fix_one(t);
+    a = t;
+  }
+  ans[2 * g] = a.hi;
+  ans[2 * g + 1] = a.lo;
+}
 
 // Persistent replay kernel: tails first (wave w: tails w, w + nwaves, ..., one lane each), then
 // chunks of consecutive replay tiles from the queue.  A wave's tiles only move forward, so its
@@ -343,6 +528,35 @@ int replay_blocks_per_cu(int kind) {
     case kPlainW3: return replay_blocks<kPlainW3>();
     default: return 0;
   }
+}
+
+hipError_t launch_replay106(int kind, const R106JobDev* jobs, uint32_t njobs, uint64_t gsize,
+                            uint32_t ngroups_max, hipStream_t stream) {
+  if (njobs == 0) return hipSuccess;
+  if (ngroups_max > 0) {
+    const dim3 g(ngroups_max, njobs), b(256);
+    const uint64_t gs = gsize;
+    switch (kind) {
+      case kSplitW3: hipLaunchKernelGGL(k_r106_groups<kSplitW3>, g, b, 0, stream, jobs, gs); break;
+      case kSplitW2: hipLaunchKernelGGL(k_r106_groups<kSplitW2>, g, b, 0, stream, jobs, gs); break;
+      case kSplitW1: hipLaunchKernelGGL(k_r106_groups<kSplitW1>, g, b, 0, stream, jobs, gs); break;
+      case kPlainW3: hipLaunchKernelGGL(k_r106_groups<kPlainW3>, g, b, 0, stream, jobs, gs); break;
+      default: return hipErrorInvalidValue;
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_r106_finish, dim3(njobs), dim3(256), 0, stream, jobs);
+  return hipGetLastError();
+}
+
+hipError_t launch_replay106_sum7(const double* parts, const uint32_t* first, const uint32_t* count,
+                                 const uint8_t* shift, double* ans, double* slot_val,
+                                 uint32_t ngroups, hipStream_t stream) {
+  if (ngroups == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_r106_sum7, dim3((ngroups + 63) / 64), dim3(64), 0, stream, parts, first,
+                     count, shift, ans, slot_val, ngroups);
+  return hipGetLastError();
 }
 
 hipError_t launch_replay(int kind, LaunchDev* hdr, int grid, hipStream_t stream) {

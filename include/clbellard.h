@@ -42,7 +42,8 @@ extern "C" {
  * All three evaluate every term exactly as floor(2^192 frac(+-2^(10(d-k))/m)), the per-term rule
  * of the reference's all-integer kernel (Bellard_div2by1.cl:327-347); they differ only in how the
  * result is written: CLB_PREC_DD106 emits the 28-hex-digit double-double layout of
- * hexadecimal_dd, the two 192-bit modes emit all 48 digits. */
+ * hexadecimal_dd, the two 192-bit modes emit all 48 digits.  With clb_set_replay they compute what
+ * the reference's kernels of that precision compute instead, rounding included. */
 #define CLB_PREC_DD106 0
 #define CLB_PREC_U192 1
 #define CLB_PREC_U192_BETA 2
@@ -105,13 +106,20 @@ int clb_set_graph(clb_ctx *ctx, int enable);
 /* Reference replay.  By default every term is evaluated exactly (floor(r' 2^192 / m), the rule of
  * the reference's Sglobal64mtg_192_Refresh, Bellard_div2by1.cl:313-359), so results do not depend
  * on launch shapes.  global_work_size > 0 instead reproduces, bit for bit, the results of the
- * reference's own precision-1 and -2 kernels launched with that many work-items, their
- * double-double rounding included: Sglobal32mtg_192 (Bellard.cl:771-823), Sglobal64mtg_192
- * (:992-1084, precision 1), Sglobal_after_192 (:1153-1253), replacing Refresh only where the
- * reference does not use it.  The reference launches CLB_REF_GLOBAL_WORK_SIZE work-items
- * (Bellard_192.hsp:5-7).  0 turns replay off.  CLB_EINVAL for precision 0 (not replayed) or a
- * size that is not a multiple of 256 up to 2^24; calls with 10 d + 9 >= 2^53 (the reference's
- * OVERDOUBLE build) then fail with CLB_ERANGE.  Replayed calls never use graph mode. */
+ * reference's own kernels launched with that many work-items, their double-double rounding
+ * included.  The reference launches CLB_REF_GLOBAL_WORK_SIZE work-items (Bellard_192.hsp:5-7);
+ * 0 turns replay off.
+ *   Precision 1 and 2: Sglobal32mtg_192 (Bellard.cl:771-823), Sglobal64mtg_192 (:992-1084,
+ *   precision 1) and Sglobal_after_192 (:1153-1253), Refresh kept where the reference uses it.
+ *   Precision 0 (the 106-bit mode): Sglobal32mtg / Sglobal64mtg (:729-766, :914-985) with their
+ *   256-lane trees and the 65536 per-group accumulators, Sglobal_after (:1118-1150), Ssum and
+ *   Sum7 (:1274-1356).  Every result (frac, partials, dev_ans, work files) is then a
+ *   double-double, returned in the 3-word outputs as the bit patterns of hi (word 0) and lo
+ *   (word 1) with word 2 = 0, and printed with hexadecimal_dd; clb_batch_calc and clb_add_ans act
+ *   on the context's own bigSumMem (cleared by clb_set0mem), and the kernel-level operators
+ *   clb_sglobal / clb_sglobal_after return CLB_EINVAL.
+ * CLB_EINVAL for a size that is not a multiple of 256 up to 2^24; calls with 10 d + 9 >= 2^53 (the
+ * reference's OVERDOUBLE build) then fail with CLB_ERANGE.  Replayed calls never use graph mode. */
 #define CLB_REF_GLOBAL_WORK_SIZE 16777216ull
 int clb_set_replay(clb_ctx *ctx, uint64_t global_work_size);
 

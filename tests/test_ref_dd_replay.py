@@ -8,6 +8,7 @@ clb_replay.hip) reproduces it on the GPU.  Both are held, bit for bit, to the re
 kernels (oracle/_ref, GPU) on every case of ref_dd_cases() and on whole positions.
 """
 import os
+import struct
 import sys
 
 import pytest
@@ -112,10 +113,6 @@ def test_replay_arguments(ctxs):
         with pytest.raises(clb.ClbError) as e:
             ctx.set_replay(bad)
         assert e.value.code == clb.EINVAL
-    with clb.Context(0, clb.PREC_DD106) as c0:
-        with pytest.raises(clb.ClbError) as e:
-            c0.set_replay(clb.REF_GLOBAL_WORK_SIZE)
-        assert e.value.code == clb.EINVAL
     ctx.set_replay(clb.REF_GLOBAL_WORK_SIZE)
     try:
         d_over = (1 << 53) // 10  # 10 d + 9 >= 2^53: the reference's OVERDOUBLE build
@@ -163,3 +160,81 @@ def test_replay_work_files_equal_reference_jobs(ref, ctxs, tmp_path, p):
     if p == 2:  # the double-double range (below loopmax32) lies in batch 0 either way
         assert f"{theirs_total:048x}" == digits
     note(f"precision {p}: {7 * nb} replayed work files equal the reference kernels' jobs")
+
+
+# ----------------------------------------------------------------- the 106-bit mode
+
+
+@pytest.fixture(scope="module")
+def ctx106():
+    c = clb.Context(0, clb.PREC_DD106)
+    yield c
+    c.close()
+
+
+def dd_of_words(w):
+    """(hi, lo) from the 3-word result of the 106-bit-mode replay."""
+    return tuple(struct.unpack("<d", struct.pack("<Q", x))[0] for x in w[:2])
+
+
+@pytest.mark.parametrize("case", ref_dd106_cases(), ids=lambda c: "d{}_k{}-{}_j{}_g{}".format(*c))
+def test_hip_106bit_replay_equals_reference_kernels(ref, ctx106, case):
+    d, k0, k1, j, gws = case
+    note(f"106-bit replay d={d} k=[{k0},{k1}) j={j} gws={gws}")
+    ctx106.set_replay(gws)
+    ref.set_gws(gws)
+    assert dd_of_words(ctx106.run_jobs([(d, k0, k1, j)])[0]) == ref.job(d, k0, k1, j, 0)
+
+
+@pytest.mark.parametrize("n", [10**6, 10**7, 10**8, 10**9])
+def test_hip_106bit_replay_positions(ref, ctx106, n):
+    """Allloop in the 106-bit mode: dev_ans and the printed 28 hex digits equal the reference's."""
+    nn, d = py_normalize(n)
+    note(f"106-bit replay position N={nn}")
+    ctx106.set_replay(clb.REF_GLOBAL_WORK_SIZE)
+    ref.set_gws()
+    frac, hexs, _ = ctx106.normal(n)
+    want = ref.normal(d, 0)
+    note(f"  ours {hexs}\n  ref  {clb.hex_dd(*want)}")
+    assert dd_of_words(frac) == want
+    assert hexs == clb.hex_dd(*want)
+
+
+def test_hip_106bit_replay_operators(ref, ctx106):
+    """Set0mem / BatchCalc / AddAns by hand (Allloop, Bellard_192.hsp:264-296) on the context's
+    own bigSumMem, with a BatchCalc split in two calls (the group accumulators carry over, as the
+    reference's do), against the reference kernels' normal mode."""
+    n = 10**7
+    _, d = py_normalize(n)
+    ctx106.set_replay(clb.REF_GLOBAL_WORK_SIZE)
+    ref.set_gws()
+    ctx106.clear_ans()
+    for j in range(7):
+        ctx106.set0mem()
+        ctx106.batch_calc(d, 0, d, j)
+        ctx106.add_ans(j)
+    assert dd_of_words(ctx106.read_ans()) == ref.normal(d, 0)
+    with pytest.raises(clb.ClbError) as e:
+        ctx106.sglobal(0, d, 100, 1, 4, 1)
+    assert e.value.code == clb.EINVAL
+
+
+def test_hip_106bit_replay_work_files(ref, ctx106, tmp_path):
+    """Work mode in the 106-bit mode replayed: each file's digits are hexadecimal_dd of the
+    reference kernels' job (Set0memdd, BatchCalc, AddAns, bigSum[0] as fsave writes it)."""
+    n = 2_500_000_000
+    _, d = py_normalize(n)
+    batch = 1 << 28
+    nb = d // batch + 1
+    wl = tmp_path / "worklist.txt"
+    lines = [f"digits={n}", "precision=0"] + [f"{j}_{b}" for j in range(7) for b in range(nb)]
+    wl.write_bytes(("\r\n".join(lines) + "\r\n").encode())
+    work = str(tmp_path / "work")
+    ctx106.set_replay(clb.REF_GLOBAL_WORK_SIZE)
+    ref.set_gws()
+    assert ctx106.run_worklist(str(wl), work, batch_size=batch) == 7 * nb
+    for j in range(7):
+        for b in range(nb):
+            line0 = open(os.path.join(work, str(j), f"{j}_{b}.txt"), "rb").read().split(b"\r\n")[0]
+            want = clb.hex_dd(*ref.job(d, b * batch, (b + 1) * batch, j, 0))
+            assert line0.decode() == want, (j, b)
