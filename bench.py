@@ -218,14 +218,14 @@ def bench_worklist(clb, args):
 
 
 def bench_replay(clb, args):
-    """Normal mode at N (default 10^12) replaying the reference's precision-1 and -2 kernels
+    """Normal mode at N (default 10^12) replaying the reference's kernels of precision 2, 1 and 0
     (clb_set_replay with the reference's 2^24 work-items per launch): the reference's own output
-    bits, its double-double rounding included.  The leading 24 hex digits are compared with the
-    exact mode's."""
+    bits, its double-double rounding included.  The leading 24 hex digits (16 in the 106-bit mode)
+    are compared with the exact mode's."""
     n_pos, d, _ = clb.normalize(args.position)
     exact = None
     rows = {}
-    for prec in (clb.PREC_U192_BETA, clb.PREC_U192):
+    for prec in (clb.PREC_U192_BETA, clb.PREC_U192, clb.PREC_DD106):
         ctx = clb.Context(0, prec)
         if exact is None:
             exact = ctx.normal(n_pos)[1]
@@ -235,7 +235,8 @@ def bench_replay(clb, args):
         _, hexs, _ = ctx.normal(n_pos)
         dt = time.perf_counter() - t0
         ctx.close()
-        if hexs[:24] != exact[:24]:
+        agree = 16 if prec == clb.PREC_DD106 else 24
+        if hexs[:agree] != exact[:agree]:
             raise SystemExit(f"precision {prec} replay {hexs} disagrees with exact {exact}")
         rows[str(prec)] = {"seconds": round(dt, 3), "digits": hexs}
         log(f"replay precision {prec}: {dt:.2f}s {hexs}")
@@ -245,8 +246,9 @@ def bench_replay(clb, args):
         "unit": "terms/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": 1e3 * rows["1"]["seconds"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"normal mode at N={n_pos} replaying the reference's precision-1 "
-                               "and -2 kernels (2^24 work-items per launch)", "position": n_pos,
+        "config": {"workload": f"normal mode at N={n_pos} replaying the reference's kernels of "
+                               "precision 2, 1 and 0 (2^24 work-items per launch)",
+                   "position": n_pos,
                    "d": d, "precision": rows}}), flush=True)
     return 0
 

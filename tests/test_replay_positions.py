@@ -1,9 +1,10 @@
 """GPU: the reference replay at the headline position N = 10^12 against the reference kernels' own
-output there (tests/golden/ref_positions.json): all 48 hex digits, double-double rounding
-included.  Precision 2 takes about 19 s; precision 1 (about 100 s) runs when CLB_REPLAY_SLOW is
-set."""
+output there (tests/golden/ref_positions.json): all 48 hex digits of the 192-bit modes, and the
+106-bit mode's dev_ans to the bit, double-double rounding included.  Precision 2 takes about
+19 s; precisions 1 and 0 (about 34 and 31 s) run when CLB_REPLAY_SLOW is set."""
 import json
 import os
+import struct
 import sys
 import time
 
@@ -20,14 +21,17 @@ with open(os.path.join(GOLDEN, "ref_positions.json")) as f:
 
 @pytest.mark.parametrize("row", ROWS, ids=lambda r: f"n{r['n']}_p{r['precision']}")
 def test_replay_equals_reference_output(row):
-    if row["precision"] == 1 and not os.environ.get("CLB_REPLAY_SLOW"):
-        pytest.skip("precision 1 replay at N = 10^12 takes ~100 s; set CLB_REPLAY_SLOW=1")
+    if row["precision"] != 2 and not os.environ.get("CLB_REPLAY_SLOW"):
+        pytest.skip("precision 0 and 1 replays at N = 10^12 take ~30 s each; set CLB_REPLAY_SLOW=1")
     print(f"replay N={row['n']} precision {row['precision']}", file=sys.stderr, flush=True)
     with clb.Context(0, row["precision"]) as ctx:
         ctx.set_replay(clb.REF_GLOBAL_WORK_SIZE)
         t0 = time.perf_counter()
-        _, hexs, _ = ctx.normal(row["n"])
+        frac, hexs, _ = ctx.normal(row["n"])
         dt = time.perf_counter() - t0
     print(f"  {hexs} in {dt:.2f} s (reference kernels: {row['reference_seconds']} s)",
           file=sys.stderr, flush=True)
     assert hexs == row["hex"]
+    if row["precision"] == 0:  # dev_ans itself: the bit patterns of (hi, lo)
+        hi, lo = (struct.unpack("<d", struct.pack("<Q", w))[0] for w in frac[:2])
+        assert (hi, lo) == (float.fromhex(row["hi"]), float.fromhex(row["lo"]))
