@@ -2,8 +2,12 @@
 // (Bellard_192.hsp normal mode and work mode) and workmodeSum0_6.exe.
 //
 //   clbellard normal <digits> [--precision 0|1|2] [--device i] [--kernel auto|plain|split]
+//                    [--replay | --replay-gws GWS]
 //   clbellard work <worklist.txt> [--workdir work] [--batch-size B] [--device i]
+//                  [--replay | --replay-gws GWS]
 //   clbellard sum [--workdir work]
+// --replay: the reference kernels' own output bits (clb_set_replay) with the reference's
+// 16777216 work-items per launch; --replay-gws: with GWS work-items per launch.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -15,7 +19,9 @@ static int usage() {
   fprintf(stderr,
           "usage:\n"
           "  clbellard normal <digits> [--precision 0|1|2] [--device i] [--kernel auto|plain|split]\n"
+          "                   [--replay | --replay-gws GWS]\n"
           "  clbellard work <worklist.txt> [--workdir DIR] [--batch-size B] [--device i]\n"
+          "                 [--replay | --replay-gws GWS]\n"
           "  clbellard sum [--workdir DIR]\n");
   return 2;
 }
@@ -30,7 +36,7 @@ int main(int argc, char** argv) {
   const std::string cmd = argv[1];
   int precision = 1, device = 0, kernel = CLB_KERNEL_AUTO;
   std::string workdir = "work", positional;
-  unsigned long long batch = 0;
+  unsigned long long batch = 0, replay = 0;
   for (int i = 2; i < argc; ++i) {
     const std::string a = argv[i];
     auto next = [&]() -> const char* { return i + 1 < argc ? argv[++i] : nullptr; };
@@ -46,6 +52,10 @@ int main(int argc, char** argv) {
     } else if (a == "--kernel" && (v = next())) {
       kernel = !strcmp(v, "plain") ? CLB_KERNEL_PLAIN : !strcmp(v, "split") ? CLB_KERNEL_SPLIT
                                                                             : CLB_KERNEL_AUTO;
+    } else if (a == "--replay") {
+      replay = CLB_REF_GLOBAL_WORK_SIZE;
+    } else if (a == "--replay-gws" && (v = next())) {
+      replay = strtoull(v, nullptr, 10);
     } else if (positional.empty() && a[0] != '-') {
       positional = a;
     } else {
@@ -65,6 +75,7 @@ int main(int argc, char** argv) {
   clb_ctx* ctx = nullptr;
   if (int rc = clb_ctx_create(device, precision, &ctx)) return die(rc);
   int rc = clb_set_kernel(ctx, kernel);
+  if (!rc && replay) rc = clb_set_replay(ctx, replay);
   if (!rc && cmd == "normal") {
     uint64_t w[3], n_out = 0;
     char hex[49];

@@ -1065,7 +1065,9 @@ int clb_autotune(clb_ctx* ctx, uint64_t n, double budget_ms, int* terms_per_lane
   const int saved_tpl = ctx->tpl_index, saved_bpc = ctx->blocks_per_cu;
   const int saved_min = ctx->min_tiles_per_wave;
   const bool saved_graph = ctx->use_graph;
+  const uint64_t saved_replay = ctx->replay_gws;
   ctx->use_graph = false;
+  ctx->replay_gws = 0;  // the knobs belong to the exact kernels, so those are timed
   // The sample: the seven formulas' head terms from k0, s terms each (all of [0, d) when that
   // fits the budget), no tails.
   auto jobs_for = [&](uint64_t k0, uint64_t s) {
@@ -1086,6 +1088,7 @@ int clb_autotune(clb_ctx* ctx, uint64_t n, double budget_ms, int* terms_per_lane
     ctx->blocks_per_cu = saved_bpc;
     ctx->min_tiles_per_wave = saved_min;
     ctx->use_graph = saved_graph;
+    ctx->replay_gws = saved_replay;
     return rc;
   };
   // probe at the default settings to size the sample

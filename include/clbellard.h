@@ -95,7 +95,8 @@ int clb_set_tiling(clb_ctx *ctx, int terms_per_lane, int min_tiles_per_wave);
  * each), keeps the fastest on the context (as clb_set_tiling / clb_set_launch would; a candidate
  * must beat the default, 16 terms and full residency, by 2%) and returns it (outputs may be
  * NULL).
- * Every candidate's sum is compared; a difference returns CLB_EDEVICE. */
+ * Every candidate's sum is compared; a difference returns CLB_EDEVICE.  The exact kernels are
+ * timed even on a context in replay mode (clb_set_replay), whose setting is kept. */
 int clb_autotune(clb_ctx *ctx, uint64_t n, double budget_ms, int *terms_per_lane,
                  int *blocks_per_cu);
 /* Launch mode (no reference counterpart; the reference enqueues every kernel from the HSP host,
