@@ -35,3 +35,23 @@ def test_replay_equals_reference_output(row):
     if row["precision"] == 0:  # dev_ans itself: the bit patterns of (hi, lo)
         hi, lo = (struct.unpack("<d", struct.pack("<Q", w))[0] for w in frac[:2])
         assert (hi, lo) == (float.fromhex(row["hi"]), float.fromhex(row["lo"]))
+
+
+@pytest.mark.parametrize("p", [0, 1, 2])
+def test_cli_replay_prints_the_reference_digits(p):
+    """`clbellard normal N --precision p --replay` at N = 10^6 prints the reference kernels'
+    digits, as the restatement pinned against them (orc_ref_*) computes them."""
+    import subprocess
+    from clb_testlib import ROOT, load_oracle
+    cli = os.path.join(ROOT, "clbellard_amd", "clbellard")
+    if not os.path.exists(cli):
+        pytest.skip("CLI not built")
+    orc = load_oracle()
+    if p == 0:
+        want = clb.hex_dd(*orc.ref_normal106(10**6))
+    else:
+        want = f"{orc.ref_normal192(10**6, p):048x}"
+    out = subprocess.run([cli, "normal", "1000000", "--precision", str(p), "--replay"],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.splitlines()[1] == want
