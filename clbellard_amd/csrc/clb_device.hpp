@@ -160,7 +160,7 @@ __device__ __forceinline__ double fsq_bits(double x, uint64_t H, uint32_t npos, 
   return x;
 }
 
-// 2^(1024 H + v) mod m (balanced) for a wave-uniform H >= 1 and a per-lane v in [0, 2048): the
+// 2^(1024 H + v) mod m (balanced) for a wave-uniform H >= 1 and a per-lane even v in [0, 2048): the
 // lanes of a wave have exponents E - 10 L, which this splits into a common high part H and a
 // per-lane low part v.  The leading bits t of H (everything above the lowest s bits, s the
 // smallest count that leaves t <= tmax) give the start value 2^t - rint(2^t / m) m in one exact
@@ -194,18 +194,16 @@ __device__ __forceinline__ double fpow2_split(uint64_t H, uint32_t v, int tmax, 
     if (s > 1) x = fsq_bits(x, H, (uint32_t)(s - 1), m, minv);
     x = fsq_shift(x, (int)(H & 1) + carry, m, minv);
   }
+  // v is even (every exponent 10 (d - k) is), so where bit 0 forms a window of its own (W = 3:
+  // windows 7-9, 4-6, 1-3, then bit 0; W = 1) the last square needs no shift.
+  constexpr int last = W == 2 ? 0 : 1;  // lowest position of a shifted window
 #pragma unroll
-  for (int pos = 10 - W; pos >= 0; pos -= W) {
+  for (int pos = 10 - W; pos >= last; pos -= W) {
 #pragma unroll
     for (int i = 0; i < W - 1; ++i) x = fsq(x, m, minv);
     x = fsq_shift(x, (int)((v >> pos) & mask), m, minv);
   }
-  constexpr int rest = 10 % W;
-  if constexpr (rest != 0) {
-#pragma unroll
-    for (int i = 0; i < rest - 1; ++i) x = fsq(x, m, minv);
-    x = fsq_shift(x, (int)(v & ((1u << rest) - 1)), m, minv);
-  }
+  if constexpr (last == 1) x = fsq(x, m, minv);
   return x;
 }
 

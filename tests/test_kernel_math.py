@@ -59,7 +59,8 @@ def fsq2(x: float, m: float, minv: float) -> float:
 
 def fpow2_split(H: int, v: int, tmax: int, W: int, m: float, minv: float, ymax: float) -> float:
     """The kernel's 2^(1024 H + v) mod m: start 2^t, H's bits by plain or doubling squares, H's
-    last bit with v's bit 10, then W-bit windows over v's low ten bits."""
+    last bit with v's bit 10, then W-bit windows over v's low ten bits (v even, so for W = 3 and
+    1 the last square takes bit 0 without a shift)."""
     bl, bt = H.bit_length(), tmax.bit_length()
     s = 0
     if bl > bt or (bl == bt and H >= tmax):
@@ -78,16 +79,15 @@ def fpow2_split(H: int, v: int, tmax: int, W: int, m: float, minv: float, ymax: 
             x = fsq2(x, m, minv) if (H >> pos) & 1 else fsq_shift(x, 0, m, minv)
             assert abs(x) <= ymax * m
         x = fsq_shift(x, (H & 1) + carry, m, minv)
-    for pos in range(10 - W, -1, -W):
+    last = 0 if W == 2 else 1  # v is even: a window of bit 0 alone is a plain square
+    for pos in range(10 - W, last - 1, -W):
         for _ in range(W - 1):
             x = fsq_shift(x, 0, m, minv)
         x = fsq_shift(x, (v >> pos) & ((1 << W) - 1), m, minv)
         assert abs(x) <= ymax * m
-    rest = 10 % W
-    if rest:
-        for _ in range(rest - 1):
-            x = fsq_shift(x, 0, m, minv)
-        x = fsq_shift(x, v & ((1 << rest) - 1), m, minv)
+    if last == 1:
+        assert v % 2 == 0
+        x = fsq_shift(x, 0, m, minv)
     return x
 
 
@@ -166,7 +166,7 @@ def test_exponent_chain(W, bits, ymax, yshift):
         H = rng.randrange(1, 1 << rng.randrange(1, 40))
         if trial % 5 == 0:
             H = (1 << rng.randrange(1, 40)) - 1
-        v = rng.randrange(10, 1664)
+        v = 2 * rng.randrange(5, 832)  # v = ((E - 640) & 1023) + 640 - 10 L, E even
         x = fpow2_split(H, v, tmax, W, float(m), recip(float(m), rng.choice((-1, 0, 1))), ymax)
         assert abs(x) <= ymax * m
         assert int(x) % m == pow(2, 1024 * H + v, m)
