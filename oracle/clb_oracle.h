@@ -29,8 +29,10 @@
  *   orc_normal                          hex digits of pi from mpmath (tests/golden/)
  *   orc_hex48                           printf("%016llx") layout
  *   orc_work_sum_add                    big-integer sum mod 16^48
- *   orc_hex_dd                          parity unpinned below 2^-100 (reference formatting of
- *                                       double-double values is only pinned on exact inputs)
+ *   orc_hex_dd                          exact 112-bit values, hand-traced borrow vectors, and
+ *                                       floor(v 2^112) for random non-negative inputs; negative
+ *                                       inputs pinned only to within 2^-107 (one dd-add rounding),
+ *                                       parity unpinned below that
  *   orc_ref_item192, orc_ref_tail192,
  *   orc_ref_batch192, orc_ref_normal192,
  *   orc_ref_job106, orc_ref_normal106   the reference kernels' own results (GPU runs of

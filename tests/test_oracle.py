@@ -5,6 +5,7 @@ exact integers (pow, //), Bellard's series for pi as an exact rational, and the 
 from mpmath (tests/golden/pi_hex.json).  The reference has no tests or vectors of its own.
 """
 import ctypes
+import math
 import random
 from fractions import Fraction
 
@@ -250,6 +251,24 @@ def test_hex_dd_exact_inputs(oracle):
         cases.append((hi, abs(lo) / 2.0**7))
     for hi, lo in cases:
         assert oracle.hex_dd(hi, lo) == _dd_exact_hex(hi, lo), (hi, lo)
+
+
+def test_hex_dd_general_inputs(oracle):
+    """Double-doubles with bits far below 2^-112.  hexsademical.as:38-71 keeps hh + ll exact
+    (x256 is exact, TwoSum is error-free, hh - digit is exact), so a non-negative value prints
+    as floor(v 2^112) unless some step's remainder lies within half an ulp below an integer (46
+    one-bits in a row; random mantissas never do).  A negative value first gets 1 added by a
+    double-double add (:25-31) that rounds once, |error| <= 2^-107, so it prints within 2^5 + 1
+    units of 2^-112 of floor((v + 1) 2^112)."""
+    rng = random.Random(2024)
+    for _ in range(3000):
+        hi = rng.randrange(2**52, 2**53) / 2.0**53 * 2.0 ** -rng.randrange(0, 20)
+        lo = math.ldexp(rng.random() - 0.5, math.frexp(hi)[1] - 53)  # |lo| <= ulp(hi) / 2
+        want = int((Fraction(hi) + Fraction(lo)) * 2**112)
+        assert oracle.hex_dd(hi, lo) == f"{want:028x}" + "0" * 20, (hi, lo)
+        got = int(oracle.hex_dd(-hi, -lo)[:28], 16)
+        want_neg = math.floor((1 - Fraction(hi) - Fraction(lo)) * 2**112)
+        assert abs(got - want_neg) <= 33, (hi, lo, got - want_neg)
 
 
 def test_hex_dd_reference_borrow_quirk(oracle):
