@@ -65,6 +65,25 @@ __device__ __forceinline__ DD div_nn(double x, double y) {  // nn_div, :99-106
   return two_sum(z1, ((c.hi + x) + c.lo) / y);
 }
 
+// x / y as the compiler lowers it on gfx950: div_scale of both operands, rcp and two Newton
+// steps (recip), q = x r, its residual, div_fmas, div_fixup.  For y an integer in [1, 2^53) and
+// x = +0 or |x| >= 2^-900, both div_scale steps return their input with VCC clear, div_fmas is
+// then a plain fma and div_fixup returns its input (sign(x) ^ sign(y) is the sign of q, and a
+// +0 numerator gives +0 either way), so with r = recip(y) the remaining three instructions give
+// the same bits.  recip(y) is the exponent engine's reciprocal too, so one serves both.
+__device__ __forceinline__ double div_r(double x, double y, double r) {
+  const double q = x * r;
+  return __builtin_fma(__builtin_fma(-y, q, x), r, q);
+}
+
+// nn_div for a numerator in [0, y] and an integer modulus y in [1, 2^53), r = recip(y): its
+// second numerator is +0 or an exact residual of magnitude >= ulp(x / y) >= 2^-106.
+__device__ __forceinline__ DD div_nn_m(double x, double y, double r) {
+  const double z1 = div_r(x, y, r);
+  const DD c = two_prod(-z1, y);
+  return two_sum(z1, div_r((c.hi + x) + c.lo, y, r));
+}
+
 // nd_div, :89-96, and dd_div, :70-77: "- z1 * y.y" is fused with the sum before it.
 __device__ __forceinline__ DD div_nd(double x, DD y) {
   const double z1 = x / y.hi;
@@ -137,7 +156,8 @@ __device__ U192 to192(DD s0, DD s1) {
 // m - r and the quotient is negated, which keeps the sum in (-1, 1).
 __device__ __forceinline__ void dd_step(DD& s, uint64_t r, uint64_t m) {
   const bool pos = s.hi > 0.0;
-  DD q = div_nn((double)(pos ? m - r : r), (double)m);
+  const double mf = (double)m;
+  DD q = div_nn_m((double)(pos ? m - r : r), mf, recip(mf));
   if (pos) q = dd_neg(q);
   s = dd_add(s, q);
 }
@@ -329,7 +349,7 @@ __device__ DD item106(const R106JobDev& jb, uint64_t offset, uint64_t k_max, uin
       if (is64) {
         dd_step(s, r, m);
       } else {
-        s = dd_add(s, div_nn((double)r, (double)m));
+        s = dd_add(s, div_nn_m((double)r, (double)m, recip((double)m)));
         fix_one(s);
       }
     }
