@@ -27,6 +27,8 @@ sys.path.insert(0, ROOT)
 METRIC = "Bellard terms/sec (7 modexps each) at hex pos 10^12; frac err vs exact oracle"
 POSITION = 10**12
 PEAK_FP64_TFLOPS = 78.6  # MI355X published FP64 peak (one FMA = 2 flops per lane per cycle)
+# 32x32->64-bit multiplies (v_mad_u64_u32, v_mul_hi_u32) at full rate: 256 CUs x 64 lanes x 2.4 GHz
+PEAK_INT32_MUL_PER_S = 256 * 64 * 2.4e9
 NUMER_SUM = 32 + 1 + 256 + 64 + 4 + 4 + 1
 
 
@@ -64,6 +66,14 @@ def algorithmic_fp64_slots(d):
     bitlen(e) - 1 squares, plus 16 for the four 48-bit quotient digits."""
     squares = sum_bitlen_10x(d) - d
     return 7 * (6 * squares + 16 * d)
+
+
+def int32_word_products(d):
+    """32x32-bit word products a 2-limb (64-bit) Montgomery engine needs for one normal-mode
+    launch: per head term and formula, bitlen(e) - 1 squares and 3 quotient-word steps of
+    2 s^2 + s = 10 word products each (CIOS, s = 2 limbs)."""
+    squares = sum_bitlen_10x(d) - d
+    return 7 * 10 * (squares + 3 * d)
 
 
 def cpu_model() -> str:
@@ -336,6 +346,7 @@ def main():
     head_avg_s = sum(head_ms) / len(head_ms) / 1e3
     slots = algorithmic_fp64_slots(d)
     achieved = 2.0 * slots / head_avg_s / 1e12  # FMA-equivalent TFLOP/s
+    int_products = int32_word_products(d)
 
     # seeded windows: plain vs split, GPU vs oracle
     rng = splitmix64(2)
@@ -409,6 +420,15 @@ def main():
             "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS, "traffic": load_traffic(),
             "head_kernel_ms": 1e3 * head_avg_s, "algorithmic_fp64_slots": slots,
+        },
+        "int_mul_view": {
+            "what": "32x32-bit word products/s that a 2-limb Montgomery engine would need for the "
+                    "same terms (bitlen(e)-1 squares + 3 quotient words, 10 products each), against "
+                    "the full-rate integer-multiply issue peak; the split kernel does this work on "
+                    "the FP64 pipe, so the fraction can exceed 1",
+            "achieved": int_products / head_avg_s, "peak": PEAK_INT32_MUL_PER_S,
+            "unit": "products/s", "frac": int_products / head_avg_s / PEAK_INT32_MUL_PER_S,
+            "modular_squares_per_s": 7 * (sum_bitlen_10x(d) - d) / head_avg_s,
         },
         "plain_kernel": {"value": win_terms / plain_win_s, "unit": "terms/s",
                          "windows": f"{args.windows} x 2^{args.window_log2} k per formula, seed 2"},
