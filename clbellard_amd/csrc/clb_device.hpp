@@ -105,8 +105,9 @@ __device__ __forceinline__ double fdigit(double& rem, double m, double minv) {
   return q;
 }
 
-// 2^e mod m (balanced) for an exponent whose top (bl - s) bits form t < 32, followed by s bits
-// consumed in W-bit windows, most significant first.  s is a multiple of W, uniform per wave.
+// 2^e mod m (balanced) for an exponent whose bits above the lowest s form t = e >> s < 32, the
+// s low bits consumed in W-bit windows, most significant first.  s is a multiple of W, uniform
+// per wave.
 template <int W>
 __device__ __forceinline__ double fpow2(uint64_t e, int s, double m, double minv) {
   const uint32_t t = (uint32_t)(e >> s);
@@ -124,9 +125,10 @@ __device__ __forceinline__ double fpow2(uint64_t e, int s, double m, double minv
 // x <- 2^(H's bits pos..1) applied by squaring: for pos = npos..1, x <- x^2 mod m, or 2 x^2 mod m
 // where bit pos of H is set.  fsq's six instructions; the doubling is the output modifier
 // (mul:2) of the final add, so it costs nothing, and bit pos is a scalar test and branch (H and
-// pos are wave-uniform).  The hardware applies an output modifier only with the IEEE mode bit
-// clear and FP64 denormals flushed, which the build sets for the device code (no value here is a
-// NaN or a denormal: every one is an integer, 0, or 1/m for m < 2^64).
+// pos are wave-uniform).  The hardware ignores output modifiers while the IEEE mode bit is set, so
+// the build clears it for the device code, and flushes FP64 denormals as well, the mode in which
+// the compiler uses output modifiers itself (no value here is a NaN or a denormal: every one is an
+// integer, 0, or 1/m for m < 2^64).
 // Bounds, with |x| <= y m and the estimate of x^2 / m off by at most y^2 m 1.5 2^-52: for
 // m < 2^42, 2^46, 2^48 the result stays within y = 1.02, 1.06, 1.34 m, and fsq_shift (m 2^c <=
 // 2^50.25) on such an input returns at most 0.97, 1.0, 1.3 m; every intermediate is an exact
@@ -160,8 +162,8 @@ __device__ __forceinline__ double fsq_bits(double x, uint64_t H, uint32_t npos, 
   return x;
 }
 
-// 2^(1024 H + v) mod m (balanced) for a wave-uniform H >= 1 and a per-lane even v in [0, 2048): the
-// lanes of a wave have exponents E - 10 L, which this splits into a common high part H and a
+// 2^(1024 H + v) mod m (balanced) for a wave-uniform H >= 1 and a per-lane even v in [0, 2048):
+// the lanes of a wave have exponents E - 10 L, which this splits into a common high part H and a
 // per-lane low part v.  The leading bits t of H (everything above the lowest s bits, s the
 // smallest count that leaves t <= tmax) give the start value 2^t - rint(2^t / m) m in one exact
 // step: with 2^t / m <= 2^49 the quotient is off by at most 0.5 + 1.5 2^-3, so |x| <= 0.69 m;

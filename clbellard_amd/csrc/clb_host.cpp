@@ -1288,7 +1288,10 @@ static int read3(clb_ctx* ctx, const uint64_t* dev, uint64_t w[3]) {
   return CLB_OK;
 }
 
-int clb_read_partial(clb_ctx* ctx, uint64_t w[3]) { return read3(ctx, ctx ? ctx->partial : nullptr, w); }
+int clb_read_partial(clb_ctx* ctx, uint64_t w[3]) {
+  return read3(ctx, ctx ? ctx->partial : nullptr, w);
+}
+
 int clb_read_ans(clb_ctx* ctx, uint64_t w[3]) { return read3(ctx, ctx ? ctx->ans : nullptr, w); }
 
 

@@ -103,7 +103,8 @@ __device__ __forceinline__ void tile_split(const JobDev& jb, uint64_t kbase, int
   const bool full = kbase >= jb.k0 && kbase + tile_k(TPL) <= jb.k1;  // every lane and term valid
   const bool newton1 = jb.den0 * (kbase + kWave) + jb.den1 >= (1ull << 37);  // m of terms i >= 1
   const uint64_t E0 = 10ull * (jb.d - kbase);                    // exponent of lane 0, term 0
-  const bool big = E0 - 10ull * kWave * (uint64_t)(nt - 1) >= 1664;  // every term in 1024 H + v form
+  // every term of the tile in 1024 H + v form
+  const bool big = E0 - 10ull * kWave * (uint64_t)(nt - 1) >= 1664;
   const int tmax = 49 + 63 - __builtin_clzll(jb.den0 * kbase + jb.den1);  // smallest m of the tile
   double f[4] = {0.0, 0.0, 0.0, 0.0};
   uint32_t corr = 0;
