@@ -2,8 +2,9 @@
 //
 // Work decomposition.  A launch holds jobs (formula j, loop count d, head range [k0, k1)).  Each
 // job's range is cut into tiles of 64 * TPL consecutive k aligned to multiples of the tile size;
-// the tiles of all jobs form one flat index space that a persistent grid consumes in chunks of
-// consecutive tiles taken from a global counter.  Lane L of a tile evaluates k = base + L + 64 i,
+// the tiles of all jobs form one flat index space that a persistent grid consumes, in static
+// per-wave ranges for small launches and otherwise in chunks of consecutive tiles taken from a
+// global counter (k_head).  Lane L of a tile evaluates k = base + L + 64 i,
 // i < TPL, so every lane sees one parity of k and hence one sign (KernelFuncs.as:10 and the
 // (numesign + k) % 2 rule of Bellard_div2by1.cl:335).  Each lane keeps the 192-bit sum Q of its
 // terms; when the wave moves to another job (and at the end) the lane applies its sign (-Q - n for
