@@ -1,6 +1,6 @@
-// clb_replay.hip -- reference replay: the rounding of the reference's precision-1 and -2 kernels
-// reproduced to the bit, for callers that need the reference's exact output bits rather than the
-// exact sum the other kernels compute.
+// clb_replay.hip -- reference replay: the rounding of the reference's kernels of all three
+// precisions reproduced to the bit, for callers that need the reference's exact output bits rather
+// than the exact sum the other kernels compute.
 //
 // The reference's 192-bit kernels keep, per work-item, two double-double sums over the
 // work-item's strided terms (of r/m and of (r 2^96 mod m)/m, r = 1024^(d-k) mod m) and convert
@@ -9,7 +9,10 @@
 // replay must reproduce each work-item's double-double arithmetic operation by operation, and may
 // add the work-items' 192-bit values in any order.  Here one lane is one work-item: a tile is 64
 // consecutive work-items of one reference launch, tiles come from a queue in chunks, and each
-// wave adds its lanes' values into the job's accumulator slot as the exact kernels do.
+// wave adds its lanes' values into the job's accumulator slot as the exact kernels do.  The
+// 106-bit mode (precision 0) keeps double-doubles to the end, so its work-group trees and its
+// 65536 group accumulators, updated launch by launch, are replayed in the reference's order too
+// (k_r106_groups, k_r106_finish, k_r106_sum7).
 //
 // This file is built in the default FP mode (IEEE mode on, denormals kept) and with contraction
 // off: each fused multiply-add in the double-double code below is one that the reference's OpenCL
