@@ -100,9 +100,10 @@ int clb_set_tiling(clb_ctx *ctx, int terms_per_lane, int min_tiles_per_wave);
 int clb_autotune(clb_ctx *ctx, uint64_t n, double budget_ms, int *terms_per_lane,
                  int *blocks_per_cu);
 /* Launch mode (no reference counterpart; the reference enqueues every kernel from the HSP host,
- * KernelFuncs.as:25-74): 1 = run each call's kernel sequence (clear, head terms, tails, combine)
- * as a HIP graph, captured on the first call with a given buffer layout and kernel choice and
- * replayed by later calls; 0 (default) = launch the kernels one by one.  Results are identical. */
+ * KernelFuncs.as:25-74): 1 = run each call's kernel sequence (head and tail terms, combine) as a
+ * HIP graph, captured on the first call with a given buffer layout and kernel choice and replayed
+ * by later calls (the call's host-to-device copy, which also clears the accumulators, stays
+ * outside the graph); 0 (default) = launch the kernels one by one.  Results are identical. */
 int clb_set_graph(clb_ctx *ctx, int enable);
 /* Reference replay.  By default every term is evaluated exactly (floor(r' 2^192 / m), the rule of
  * the reference's Sglobal64mtg_192_Refresh, Bellard_div2by1.cl:313-359), so results do not depend
