@@ -156,17 +156,17 @@ __device__ U192 to192(DD s0, DD s1) {
 }
 
 // One term's update of a sum (:786-796, :1037-1048): while the sum is positive the numerator is
-// m - r and the quotient is negated, which keeps the sum in (-1, 1).
-__device__ __forceinline__ void dd_step(DD& s, uint64_t r, uint64_t m) {
+// m - r and the quotient is negated, which keeps the sum in (-1, 1).  r and m are exact integers
+// held in doubles (m < 2^53, so m - r is exact too), minv = recip(m).
+__device__ __forceinline__ void dd_step(DD& s, double r, double m, double minv) {
   const bool pos = s.hi > 0.0;
-  const double mf = (double)m;
-  DD q = div_nn_m((double)(pos ? m - r : r), mf, recip(mf));
+  DD q = div_nn_m(pos ? m - r : r, m, minv);
   if (pos) q = dd_neg(q);
   s = dd_add(s, q);
 }
 
 // r = 2^e mod m and b = 2^(e + 96) mod m = r 2^96 mod m (:798-799, :1052-1053), exact, with
-// e < 2^bl (bl uniform per wave).
+// e < 2^bl (bl uniform per wave), as integer-valued doubles in [0, m) (m < 2^53).
 //  * fp64 engine (KIND kSplitW3/W2/W1, m < 2^42 / 2^46 / 2^48): per-lane W-bit windows of plain
 //    and windowed balanced squares (fpow2, |x| <= 0.75 m), then two exact shift-and-reduce
 //    steps of 48 bits (|x 2^48 / m - q| <= 0.5 + 0.75 2^48 1.5 2^-52 < 0.6, so the residue stays
@@ -174,32 +174,31 @@ __device__ __forceinline__ void dd_step(DD& s, uint64_t r, uint64_t m) {
 //  * int engine (kPlainW3, odd m < 2^55): Montgomery form h = 2^(e + 64) mod m, r = REDC(h), and
 //    b = REDC(h (2^96 mod m)) with 2^96 mod m the Montgomery form of 2^32.
 template <int KIND>
-__device__ __forceinline__ void residues(uint64_t e, int bl, uint64_t m, uint64_t& r,
-                                         uint64_t& b) {
+__device__ __forceinline__ void residues(uint64_t e, int bl, uint64_t m, double& r, double& b) {
   if constexpr (KIND == kPlainW3) {
     const uint64_t mp = neg_inv64(m);
     const uint64_t one = (0ull - m) % m;
     const uint64_t h = ipow2_mont<3>(e, bl, m, mp, one);
     const uint64_t c = ipow2_mont<3>(32, 6, m, mp, one);
-    r = redc(0, h, m, mp);
-    if (r >= m) r -= m;
-    b = redc(mulhi64(h, c), h * c, m, mp);
-    if (b >= m) b -= m;
+    uint64_t ri = redc(0, h, m, mp);
+    if (ri >= m) ri -= m;
+    uint64_t bi = redc(mulhi64(h, c), h * c, m, mp);
+    if (bi >= m) bi -= m;
+    r = (double)ri;
+    b = (double)bi;
   } else {
     constexpr int W = KIND == kSplitW3 ? 3 : KIND == kSplitW2 ? 2 : 1;
     const double mf = (double)m;
     const double minv = recip(mf);
     const int s = bl > 3 ? W * ((bl - 3) / W) : 0;  // top bits t = e >> s < 2^(3 + W - 1)
     double x = fpow2<W>(e, s, mf, minv);
-    double y = x < 0.0 ? x + mf : x;
-    r = (uint64_t)y;
+    r = x < 0.0 ? x + mf : x;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const double z = __builtin_amdgcn_ldexp(x, 48);
       x = __builtin_fma(-__builtin_rint(z * minv), mf, z);
     }
-    y = x < 0.0 ? x + mf : x;
-    b = (uint64_t)y;
+    b = x < 0.0 ? x + mf : x;
   }
 }
 
@@ -215,10 +214,12 @@ __device__ U192 replay_item(const ReplayJobDev& jb, uint64_t tile, int lane) {
     if (k < jb.k_max) {
       const uint64_t m = jb.den0 * k + jb.den1;
       const int bl = 64 - __builtin_clzll(10ull * (jb.d - kw));
-      uint64_t r, b;
+      double r, b;
       residues<KIND>(10ull * (jb.d - k), bl, m, r, b);
-      dd_step(s0, r, m);
-      dd_step(s1, b, m);
+      const double mf = (double)m;
+      const double minv = recip(mf);  // the fp64 engine's own, shared
+      dd_step(s0, r, mf, minv);
+      dd_step(s1, b, mf, minv);
     }
   }
   // The reference's k after its loop has the parity of offset + idx (gsize is even).
@@ -304,20 +305,20 @@ __device__ __forceinline__ void fix_one(DD& s) {  // sumfix, :491-501
   }
 }
 
-// r = 2^e mod m alone, exact (see residues).
+// r = 2^e mod m alone, exact, as an integer-valued double (see residues).
 template <int KIND>
-__device__ __forceinline__ uint64_t residue(uint64_t e, int bl, uint64_t m) {
+__device__ __forceinline__ double residue(uint64_t e, int bl, uint64_t m) {
   if constexpr (KIND == kPlainW3) {
     const uint64_t mp = neg_inv64(m);
     const uint64_t one = (0ull - m) % m;
     uint64_t r = redc(0, ipow2_mont<3>(e, bl, m, mp, one), m, mp);
-    return r >= m ? r - m : r;
+    return (double)(r >= m ? r - m : r);
   } else {
     constexpr int W = KIND == kSplitW3 ? 3 : KIND == kSplitW2 ? 2 : 1;
     const double mf = (double)m;
     const int s = bl > 3 ? W * ((bl - 3) / W) : 0;
     const double x = fpow2<W>(e, s, mf, recip(mf));
-    return (uint64_t)(x < 0.0 ? x + mf : x);
+    return x < 0.0 ? x + mf : x;
   }
 }
 
@@ -348,11 +349,12 @@ __device__ DD item106(const R106JobDev& jb, uint64_t offset, uint64_t k_max, uin
     if (k < k_max) {
       const uint64_t m = jb.den0 * k + jb.den1;
       const int bl = 64 - __builtin_clzll(10ull * (jb.d - kw));
-      const uint64_t r = residue<KIND>(10ull * (jb.d - k), bl, m);
+      const double r = residue<KIND>(10ull * (jb.d - k), bl, m);
+      const double mf = (double)m;
       if (is64) {
-        dd_step(s, r, m);
+        dd_step(s, r, mf, recip(mf));
       } else {
-        s = dd_add(s, div_nn_m((double)r, (double)m, recip((double)m)));
+        s = dd_add(s, div_nn_m(r, mf, recip(mf)));
         fix_one(s);
       }
     }
