@@ -304,8 +304,9 @@ def main():
         return 0
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the digit "
-              "extraction shards by term ranges, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); across GPUs the "
+              "digit extraction would shard by term ranges (work-mode jobs), one process per GPU, "
+              "with one 24-byte partial per job summed on the host", file=sys.stderr)
         return 2
 
     import clbellard_amd as clb
