@@ -1323,7 +1323,9 @@ int clb_batch_positions(clb_ctx* ctx, const uint64_t* n, size_t count, uint64_t*
   if (int rc = check_ctx(ctx)) return rc;
   if (count == 0) return CLB_OK;
   if (!n || !frac) return fail(CLB_EINVAL, "null array");
-  if (count > (1u << 26)) return fail(CLB_EINVAL, "too many positions");
+  // About 1.2 KB of the call's block per position (7 jobs, 7 tails, accumulators, result), which
+  // must stay below 4 GiB.
+  if (count > (1u << 21)) return fail(CLB_EINVAL, "too many positions (at most 2^21 per call)");
   std::vector<HostJob> jobs;
   std::vector<uint8_t> shift(7 * count);
   std::vector<GroupDev> groups(count);
@@ -1345,7 +1347,8 @@ int clb_run_jobs(clb_ctx* ctx, const clb_job* jobs_in, size_t count, uint64_t* p
   if (int rc = check_ctx(ctx)) return rc;
   if (count == 0) return CLB_OK;
   if (!jobs_in || !partial) return fail(CLB_EINVAL, "null array");
-  if (count > (1u << 28)) return fail(CLB_EINVAL, "too many jobs");
+  // About 200 bytes of the call's block per job, which must stay below 4 GiB.
+  if (count > (1u << 24)) return fail(CLB_EINVAL, "too many jobs (at most 2^24 per call)");
   std::vector<HostJob> jobs;
   std::vector<uint8_t> shift(count);
   std::vector<GroupDev> groups(count);

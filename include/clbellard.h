@@ -159,10 +159,12 @@ int clb_sglobal_after(clb_ctx *ctx, uint64_t d, int64_t numesign, uint64_t den0,
 int clb_normal(clb_ctx *ctx, uint64_t n, uint64_t frac[3], char hex[49], uint64_t *n_out);
 
 /* Many digit positions in one persistent launch (no reference counterpart; the reference
- * computes one position per run).  frac[3*i..] = frac(16^(N'_i - 1) pi). */
+ * computes one position per run).  frac[3*i..] = frac(16^(N'_i - 1) pi).  At most 2^21
+ * positions per call (CLB_EINVAL beyond). */
 int clb_batch_positions(clb_ctx *ctx, const uint64_t *n, size_t count, uint64_t *frac);
 /* Many term-range jobs in one persistent launch (work mode's jobs batched, Bellard_192.hsp:
- * 298-340).  partial[3*i..] = c_j * (job i's sum) mod 1, the value fsave writes. */
+ * 298-340).  partial[3*i..] = c_j * (job i's sum) mod 1, the value fsave writes.  At most 2^24
+ * jobs per call (CLB_EINVAL beyond). */
 int clb_run_jobs(clb_ctx *ctx, const clb_job *jobs, size_t count, uint64_t *partial);
 
 /* hexadecimal_i32 with the host word swap (Bellard_192.hsp:354-359, hexsademical.as:104-127). */

@@ -1,6 +1,7 @@
 """GPU parity tests: the HIP kernels of libclbellard.so (through the C ABI) against the CPU
 oracle, the golden pi digits, and the size-independent properties of the exact-integer sums
 (partition invariance, batched == single, plain == split)."""
+import ctypes
 import os
 import random
 import sys
@@ -307,3 +308,14 @@ def test_out_of_range_moduli_are_refused(ctx):
         ctx.run_jobs([(10**18 - 1, 10**18 - 10, 10**18, 2)])  # tail and head moduli near 10^19
     assert e.value.code == clb.ERANGE
     assert w2i(ctx.normal(1000)[0]) != 0  # the context stays usable
+
+
+def test_call_size_limits(ctx):
+    """More positions or jobs than one call's block can hold fail with CLB_EINVAL before anything
+    is read or allocated (a one-element array with a larger count is never read past its end)."""
+    one = (ctypes.c_uint64 * 1)(10**6)
+    out = (ctypes.c_uint64 * 3)()
+    assert clb.lib.clb_batch_positions(ctx._ctx, one, (1 << 21) + 1, out) == clb.EINVAL
+    job = (clb._Job * 1)()
+    assert clb.lib.clb_run_jobs(ctx._ctx, job, (1 << 24) + 1, out) == clb.EINVAL
+    assert w2i(ctx.batch_positions([10**6])[0]) == w2i(ctx.normal(10**6)[0])
