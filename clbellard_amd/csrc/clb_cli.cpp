@@ -6,12 +6,19 @@
 //   clbellard work <worklist.txt> [--workdir work] [--batch-size B] [--device i]
 //                  [--replay | --replay-gws GWS]
 //   clbellard sum [--workdir work]
+//   clbellard batch <positions.txt | -> [--precision 0|1|2] [--device i] [--graph]
+//                   [--autotune MS] [--replay | --replay-gws GWS]
 // --replay: the reference kernels' own output bits (clb_set_replay) with the reference's
 // 16777216 work-items per launch; --replay-gws: with GWS work-items per launch.
+// batch: one position per line (file, or standard input for "-"), all of them in one persistent
+// launch (clb_batch_positions); prints "N' digits" per position in input order.  --graph replays
+// the captured launch sequence (clb_set_graph); --autotune MS first tunes tile length and grid on
+// the largest position for about MS milliseconds (clb_autotune).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/clbellard.h"
 
@@ -22,7 +29,9 @@ static int usage() {
           "                   [--replay | --replay-gws GWS]\n"
           "  clbellard work <worklist.txt> [--workdir DIR] [--batch-size B] [--device i]\n"
           "                 [--replay | --replay-gws GWS]\n"
-          "  clbellard sum [--workdir DIR]\n");
+          "  clbellard sum [--workdir DIR]\n"
+          "  clbellard batch <positions.txt | -> [--precision 0|1|2] [--device i] [--graph]\n"
+          "                  [--autotune MS] [--replay | --replay-gws GWS]\n");
   return 2;
 }
 
@@ -31,12 +40,49 @@ static int die(int rc) {
   return 1;
 }
 
+// Positions, one per line; blank lines skipped.  Returns false on an unreadable file or a line
+// that is not a decimal number.
+static bool read_positions(const std::string& path, std::vector<uint64_t>* out) {
+  FILE* f = path == "-" ? stdin : fopen(path.c_str(), "r");
+  if (!f) return false;
+  char line[256];
+  bool ok = true;
+  while (ok && fgets(line, sizeof line, f)) {
+    char* p = line;
+    while (*p == ' ' || *p == '\t') ++p;
+    if (*p == '\n' || *p == '\r' || *p == 0) continue;
+    char* end = nullptr;
+    const unsigned long long v = strtoull(p, &end, 10);
+    while (end && (*end == ' ' || *end == '\t' || *end == '\r' || *end == '\n')) ++end;
+    if (end == p || (end && *end != 0)) ok = false;
+    out->push_back(v);
+  }
+  if (f != stdin) fclose(f);
+  return ok;
+}
+
+// The display string of a result: the 106-bit mode prints a double-double (hexadecimal_dd), which
+// the replay of that mode returns as its bit patterns and the exact mode as 192 bits.
+static int format(const uint64_t w[3], int precision, bool replay, char hex[49]) {
+  if (precision != 0) return clb_hex48(w, hex);
+  double dd[2];
+  if (replay) {
+    memcpy(&dd[0], &w[0], 8);
+    memcpy(&dd[1], &w[1], 8);
+  } else if (int rc = clb_u192_to_dd(w, dd)) {
+    return rc;
+  }
+  return clb_hex_dd(dd, hex);
+}
+
 int main(int argc, char** argv) {
   if (argc < 2) return usage();
   const std::string cmd = argv[1];
   int precision = 1, device = 0, kernel = CLB_KERNEL_AUTO;
   std::string workdir = "work", positional;
   unsigned long long batch = 0, replay = 0;
+  bool graph = false;
+  double tune_ms = 0.0;
   for (int i = 2; i < argc; ++i) {
     const std::string a = argv[i];
     auto next = [&]() -> const char* { return i + 1 < argc ? argv[++i] : nullptr; };
@@ -52,11 +98,15 @@ int main(int argc, char** argv) {
     } else if (a == "--kernel" && (v = next())) {
       kernel = !strcmp(v, "plain") ? CLB_KERNEL_PLAIN : !strcmp(v, "split") ? CLB_KERNEL_SPLIT
                                                                             : CLB_KERNEL_AUTO;
+    } else if (a == "--graph") {
+      graph = true;
+    } else if (a == "--autotune" && (v = next())) {
+      tune_ms = atof(v);
     } else if (a == "--replay") {
       replay = CLB_REF_GLOBAL_WORK_SIZE;
     } else if (a == "--replay-gws" && (v = next())) {
       replay = strtoull(v, nullptr, 10);
-    } else if (positional.empty() && a[0] != '-') {
+    } else if (positional.empty() && (a[0] != '-' || a == "-")) {
       positional = a;
     } else {
       return usage();
@@ -72,6 +122,12 @@ int main(int argc, char** argv) {
     return 0;
   }
   if (positional.empty()) return usage();
+  std::vector<uint64_t> pos;
+  if (cmd == "batch" && !read_positions(positional, &pos)) {
+    fprintf(stderr, "cannot read positions from %s (one decimal number per line)\n",
+            positional.c_str());
+    return 1;
+  }
   clb_ctx* ctx = nullptr;
   if (int rc = clb_ctx_create(device, precision, &ctx)) return die(rc);
   int rc = clb_set_kernel(ctx, kernel);
@@ -84,6 +140,25 @@ int main(int argc, char** argv) {
       double head, total;
       clb_last_timing(ctx, &head, &total);
       printf("digits=%llu\n%s\nAll calc time %.0fms\n", (unsigned long long)n_out, hex, total);
+    }
+  } else if (!rc && cmd == "batch") {
+    uint64_t largest = 0;
+    for (uint64_t n : pos) largest = n > largest ? n : largest;
+    if (!pos.empty() && tune_ms > 0.0) rc = clb_autotune(ctx, largest, tune_ms, nullptr, nullptr);
+    if (!rc && graph) rc = clb_set_graph(ctx, 1);
+    std::vector<uint64_t> frac(3 * pos.size());
+    if (!rc && !pos.empty()) rc = clb_batch_positions(ctx, pos.data(), pos.size(), frac.data());
+    for (size_t i = 0; !rc && i < pos.size(); ++i) {
+      uint64_t n_out = 0;
+      char hex[49];
+      rc = clb_normalize(pos[i], &n_out, nullptr, nullptr);
+      if (!rc) rc = format(&frac[3 * i], precision, replay != 0, hex);
+      if (!rc) printf("%llu %s\n", (unsigned long long)n_out, hex);
+    }
+    if (!rc) {
+      double head, total;
+      clb_last_timing(ctx, &head, &total);
+      fprintf(stderr, "%zu position(s), All calc time %.0fms\n", pos.size(), total);
     }
   } else if (!rc && cmd == "work") {
     int ran = 0;

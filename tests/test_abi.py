@@ -201,6 +201,21 @@ def test_cli_sum_and_usage(tmp_path):
     assert subprocess.run([cli, "bogus", "1"], capture_output=True).returncode != 0
 
 
+def test_cli_batch_rejects_bad_positions(tmp_path):
+    """`clbellard batch` reads every position before it touches a GPU: an unreadable file or a
+    line that is not a decimal number fails with exit status 1 and a message."""
+    import subprocess
+    cli = os.path.join(ROOT, "clbellard_amd", "clbellard")
+    if not os.path.exists(cli):
+        pytest.skip("CLI not built")
+    bad = tmp_path / "bad.txt"
+    bad.write_text("1000000\n12x4\n")
+    for args, stdin in (([str(bad)], None), ([str(tmp_path / "missing.txt")], None),
+                        (["-"], "250000\n1e6\n")):
+        out = subprocess.run([cli, "batch", *args], input=stdin, capture_output=True, text=True)
+        assert out.returncode == 1 and "cannot read positions" in out.stderr, (args, out.stderr)
+
+
 def test_write_work_file_round_trip(tmp_path, oracle):
     d = str(tmp_path / "work")
     vals = [R.randrange(0, 2**192) for _ in range(5)]

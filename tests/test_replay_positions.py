@@ -55,3 +55,29 @@ def test_cli_replay_prints_the_reference_digits(p):
                          capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     assert out.stdout.splitlines()[1] == want
+
+
+@pytest.mark.parametrize("p,replay", [(1, False), (0, False), (1, True), (0, True)])
+def test_cli_batch_equals_normal(tmp_path, p, replay):
+    """`clbellard batch` over positions in one launch prints, per position and in input order,
+    the N' and digits `clbellard normal` prints for that position alone (exact mode and replay)."""
+    import subprocess
+    from clb_testlib import ROOT
+    cli = os.path.join(ROOT, "clbellard_amd", "clbellard")
+    if not os.path.exists(cli):
+        pytest.skip("CLI not built")
+    positions = [1000000, 250003, 10**7, 99999]
+    path = tmp_path / "positions.txt"
+    path.write_text("\n".join(str(n) for n in positions) + "\n\n")
+    extra = ["--replay"] if replay else []
+    out = subprocess.run([cli, "batch", str(path), "--precision", str(p), "--graph", *extra],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.splitlines()
+    assert len(lines) == len(positions)
+    for n, line in zip(positions, lines):
+        one = subprocess.run([cli, "normal", str(n), "--precision", str(p), *extra],
+                             capture_output=True, text=True, timeout=120)
+        assert one.returncode == 0, one.stderr
+        digits_line, hexs = one.stdout.splitlines()[:2]
+        assert line == f"{digits_line.split('=')[1]} {hexs}", (n, line, hexs)
