@@ -165,16 +165,48 @@ __device__ __forceinline__ void dd_step(DD& s, double r, double m, double minv) 
   s = dd_add(s, q);
 }
 
+// Window width of the fp64 engine for a kernel kind (0 for the integer engine).
+template <int KIND>
+constexpr int kWindow = KIND == kSplitW3 ? 3 : KIND == kSplitW2 ? 2 : KIND == kSplitW1 ? 1 : 0;
+
+// Uniform start shift of the fp64 engine for one round of a wave: the smallest multiple s of W
+// with (E0 >> s) <= tmax = 49 + floor(log2 m0), E0 the wave's largest exponent and m0 its
+// smallest modulus (both lane 0's).  Every lane's start 2^t, t = e >> s <= tmax, then has
+// 2^t / m <= 2^49, which fpow2's one-step reduction of 2^t needs (|x| <= 0.69 m).
+template <int W>
+__device__ __forceinline__ int start_shift(uint64_t E0, uint64_t m0) {
+  const int tmax = 49 + 63 - __builtin_clzll(m0);
+  const int bl = 64 - __builtin_clzll(E0 | 1);
+  const int bt = 32 - __builtin_clz((uint32_t)tmax);
+  int s = bl > bt ? W * ((bl - bt + W - 1) / W) : 0;  // now E0 >> s < 2^bt
+  if ((E0 >> s) > (uint64_t)tmax) s += W;             // E0 >> s < 2^(bt - 1) <= tmax
+  return s;
+}
+
+// start_shift for a kernel kind; 0 for the integer engine, which does not use it.
+template <int KIND>
+__device__ __forceinline__ int engine_shift(uint64_t E0, uint64_t m0) {
+  if constexpr (kWindow<KIND> > 0) {
+    return start_shift<kWindow<KIND>>(E0, m0);
+  } else {
+    return 0;
+  }
+}
+
 // r = 2^e mod m and b = 2^(e + 96) mod m = r 2^96 mod m (:798-799, :1052-1053), exact, with
-// e < 2^bl (bl uniform per wave), as integer-valued doubles in [0, m) (m < 2^53).
-//  * fp64 engine (KIND kSplitW3/W2/W1, m < 2^42 / 2^46 / 2^48): per-lane W-bit windows of plain
-//    and windowed balanced squares (fpow2, |x| <= 0.75 m), then two exact shift-and-reduce
+// e < 2^bl (bl uniform per wave), as integer-valued doubles in [0, m) (m < 2^53).  s is the fp64
+// engine's start shift (start_shift), unused by the integer engine.
+//  * fp64 engine (KIND kSplitW3/W2/W1, m < 2^42 / 2^46 / 2^48): the start 2^t reduced in one
+//    step (|x| <= 0.69 m, as 2^t / m <= 2^49), per-lane W-bit windows of plain and windowed
+//    balanced squares (fpow2; with |x| <= 0.75 m and m 2^c <= 2^49 each returns within
+//    0.5 + 0.75^2 2^49 1.5 2^-52 < 0.61 m), then two exact shift-and-reduce
 //    steps of 48 bits (|x 2^48 / m - q| <= 0.5 + 0.75 2^48 1.5 2^-52 < 0.6, so the residue stays
 //    below 0.6 m and every intermediate is an exact integer below 2^53);
 //  * int engine (kPlainW3, odd m < 2^55): Montgomery form h = 2^(e + 64) mod m, r = REDC(h), and
 //    b = REDC(h (2^96 mod m)) with 2^96 mod m the Montgomery form of 2^32.
 template <int KIND>
-__device__ __forceinline__ void residues(uint64_t e, int bl, uint64_t m, double& r, double& b) {
+__device__ __forceinline__ void residues(uint64_t e, int bl, int s, uint64_t m, double& r,
+                                         double& b) {
   if constexpr (KIND == kPlainW3) {
     const uint64_t mp = neg_inv64(m);
     const uint64_t one = (0ull - m) % m;
@@ -190,7 +222,6 @@ __device__ __forceinline__ void residues(uint64_t e, int bl, uint64_t m, double&
     constexpr int W = KIND == kSplitW3 ? 3 : KIND == kSplitW2 ? 2 : 1;
     const double mf = (double)m;
     const double minv = recip(mf);
-    const int s = bl > 3 ? W * ((bl - 3) / W) : 0;  // top bits t = e >> s < 2^(3 + W - 1)
     double x = fpow2<W>(e, s, mf, minv);
     r = x < 0.0 ? x + mf : x;
 #pragma unroll
@@ -213,9 +244,11 @@ __device__ U192 replay_item(const ReplayJobDev& jb, uint64_t tile, int lane) {
     const uint64_t k = kw + (uint64_t)lane;
     if (k < jb.k_max) {
       const uint64_t m = jb.den0 * k + jb.den1;
-      const int bl = 64 - __builtin_clzll(10ull * (jb.d - kw));
+      const uint64_t E0 = 10ull * (jb.d - kw);
+      const int bl = 64 - __builtin_clzll(E0);
+      const int sh = engine_shift<KIND>(E0, jb.den0 * kw + jb.den1);
       double r, b;
-      residues<KIND>(10ull * (jb.d - k), bl, m, r, b);
+      residues<KIND>(10ull * (jb.d - k), bl, sh, m, r, b);
       const double mf = (double)m;
       const double minv = recip(mf);  // the fp64 engine's own, shared
       dd_step(s0, r, mf, minv);
@@ -307,7 +340,7 @@ __device__ __forceinline__ void fix_one(DD& s) {  // sumfix, :491-501
 
 // r = 2^e mod m alone, exact, as an integer-valued double (see residues).
 template <int KIND>
-__device__ __forceinline__ double residue(uint64_t e, int bl, uint64_t m) {
+__device__ __forceinline__ double residue(uint64_t e, int bl, int s, uint64_t m) {
   if constexpr (KIND == kPlainW3) {
     const uint64_t mp = neg_inv64(m);
     const uint64_t one = (0ull - m) % m;
@@ -316,7 +349,6 @@ __device__ __forceinline__ double residue(uint64_t e, int bl, uint64_t m) {
   } else {
     constexpr int W = KIND == kSplitW3 ? 3 : KIND == kSplitW2 ? 2 : 1;
     const double mf = (double)m;
-    const int s = bl > 3 ? W * ((bl - 3) / W) : 0;
     const double x = fpow2<W>(e, s, mf, recip(mf));
     return x < 0.0 ? x + mf : x;
   }
@@ -344,12 +376,18 @@ __device__ DD item106(const R106JobDev& jb, uint64_t offset, uint64_t k_max, uin
   DD s = {0.0, 0.0};
   const uint64_t idx = (uint64_t)g * 256 + tid;
   const uint64_t lane = tid & (kWave - 1);
-  for (uint64_t kw = offset + (idx - lane); kw < k_max; kw += gsize) {  // kw: the wave's lane 0
+  // The wave's first work-item, made scalar (readfirstlane): the compiler cannot tell that
+  // tid - lane is uniform, and per-wave values (exponent length, start shift) would take VALU.
+  const uint64_t wave0 =
+      (uint64_t)g * 256 + __builtin_amdgcn_readfirstlane(tid & ~(uint32_t)(kWave - 1));
+  for (uint64_t kw = offset + wave0; kw < k_max; kw += gsize) {  // kw: the wave's lane 0
     const uint64_t k = kw + lane;
     if (k < k_max) {
       const uint64_t m = jb.den0 * k + jb.den1;
-      const int bl = 64 - __builtin_clzll(10ull * (jb.d - kw));
-      const double r = residue<KIND>(10ull * (jb.d - k), bl, m);
+      const uint64_t E0 = 10ull * (jb.d - kw);
+      const int bl = 64 - __builtin_clzll(E0);
+      const int sh = engine_shift<KIND>(E0, jb.den0 * kw + jb.den1);
+      const double r = residue<KIND>(10ull * (jb.d - k), bl, sh, m);
       const double mf = (double)m;
       if (is64) {
         dd_step(s, r, mf, recip(mf));
