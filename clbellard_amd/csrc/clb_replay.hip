@@ -38,21 +38,15 @@ __device__ __forceinline__ DD two_sum(double a, double b) {  // twosum, Bellard.
   return {x, (a - (x - t)) + (b - t)};
 }
 
-__device__ __forceinline__ DD split27(double a) {  // dsplit, :9-15
-  const double t = a * 134217729.0;
-  const double hi = t - (t - a);
-  return {hi, a - hi};
-}
-
-// twoproduct, :17-27: the error term is one expression, so its four products are fused.
+// twoproduct, :17-27: Dekker's product of the Veltkamp halves, its error term one expression whose
+// four products the reference's build fuses, in the order (ah bh - p) + al bh + ah bl + al bl.
+// Dekker's product is error-free (Dekker 1971; with no underflow or overflow every partial sum
+// is exact, and the fused order above is his algorithm applied to (b, a)), so the error term is
+// exactly a b - p, which one fma returns: the same two doubles.  The operands here are quotients
+// of magnitude at least 2^-300 (or 0) and integers below 2^53, far from underflow and overflow.
 __device__ __forceinline__ DD two_prod(double a, double b) {
   const double p = a * b;
-  const DD x = split27(a), y = split27(b);
-  double e = __builtin_fma(x.hi, y.hi, -p);
-  e = __builtin_fma(x.lo, y.hi, e);
-  e = __builtin_fma(x.hi, y.lo, e);
-  e = __builtin_fma(x.lo, y.lo, e);
-  return {p, e};
+  return {p, __builtin_fma(a, b, -p)};
 }
 
 __device__ __forceinline__ DD dd_add(DD a, DD b) {  // addcab, :29-35

@@ -2,8 +2,9 @@
 rounded fma, checked against exact integers.  It pins the error analysis the kernels rely on:
 the balanced modular squares (plain, windowed, doubled) stay exact and within 1.02-1.34 m, the
 exponent chain gives 2^e mod m, the lazy 48-bit quotient digits and the per-lane digit sums
-reconstruct floor(r 2^192 / m), and the tail's long division of 2^E (whose first digit may
-estimate -1) gives floor(2^E / m).  The GPU tests check the kernels themselves."""
+reconstruct floor(r 2^192 / m), the tail's long division of 2^E (whose first digit may
+estimate -1) gives floor(2^E / m), and the reference's fused Dekker product (which the replay
+computes with one fma) is error-free.  The GPU tests check the kernels themselves."""
 import math
 import random
 from fractions import Fraction
@@ -241,3 +242,37 @@ def test_tail_first_digit_estimate_can_be_negative():
     n = math.ldexp(1.0, E % 48)
     assert math.floor(fma(n, recip(float(m), 0), -0.25)) == -1
     assert tail_quot(E, m, 0) == (1 << E) // m
+
+
+def _dekker_fused(a: float, b: float):
+    """twoproduct of the reference (Bellard.cl:17-27) as its build runs it: Veltkamp halves, the
+    error term's four products fused in the order (ah bh - p) + al bh + ah bl + al bl."""
+    def split(x):
+        t = x * 134217729.0
+        hi = t - (t - x)
+        return hi, x - hi
+    p = a * b
+    ah, al = split(a)
+    bh, bl = split(b)
+    e = fma(ah, bh, -p)
+    e = fma(al, bh, e)
+    e = fma(ah, bl, e)
+    e = fma(al, bl, e)
+    return p, e
+
+
+def test_dekker_product_is_error_free():
+    """The replay computes twoproduct as (a b, fma(a, b, -a b)): the same two doubles, because
+    Dekker's product is error-free.  Checked on the operands the replay divides with: quotients
+    x / m of either sign down to 2^-300 against integer moduli m < 2^53, and hard cases."""
+    rng = random.Random(17)
+    cases = [(0.5, 3.0), (-(2.0 ** -300), 9.0), (1.0 - 2.0**-53, 2.0**53 - 1), (2.0**-53, 2.0**52 + 1)]
+    for _ in range(20000):
+        m = rng.randrange(1, 1 << rng.choice((20, 33, 42, 48, 53)))
+        a = rng.random() * 2.0 ** -rng.randrange(0, 300) * rng.choice((1, -1))
+        cases.append((a, float(m)))
+    for a, b in cases:
+        p, e = _dekker_fused(a, b)
+        assert p == a * b
+        assert Fraction(e) == Fraction(a) * Fraction(b) - Fraction(p), (a, b)
+        assert e == fma(a, b, -p)
