@@ -991,20 +991,26 @@ int clb_ctx_create(int device, int precision, clb_ctx** out) {
   if ((e = hipMalloc((void**)&ctx->partial, 3 * sizeof(uint64_t))) != hipSuccess ||
       (e = hipMalloc((void**)&ctx->ans, 3 * sizeof(uint64_t))) != hipSuccess)
     return cleanup(fail(CLB_ENOMEM, "hipMalloc: %s", hipGetErrorString(e)));
-  // The split kernels' doubling squares need the FP64 output modifier, which the hardware applies
-  // only in device code built with the IEEE mode bit clear (clbellard_amd/build.py): refuse a
-  // build whose kernels would silently compute other numbers.
-  uint64_t probe[2] = {0, 0};
+  // The doubling squares (term kernels and replay kernels, two code objects) need the FP64
+  // output modifier, which the hardware applies only in device code built with the IEEE mode bit
+  // clear (clbellard_amd/build.py): refuse a build whose kernels would silently compute other
+  // numbers.
+  uint64_t probe[4] = {0, 0, 0, 0};
   if ((e = launch_selfcheck(ctx->partial, ctx->stream)) != hipSuccess ||
-      (e = hipMemcpyAsync(probe, ctx->partial, sizeof probe, hipMemcpyDeviceToHost,
+      (e = launch_replay_selfcheck(ctx->ans, ctx->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(probe, ctx->partial, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                          ctx->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(probe + 2, ctx->ans, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost,
                           ctx->stream)) != hipSuccess ||
       (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
     return cleanup(fail(CLB_EDEVICE, "self-check: %s", hipGetErrorString(e)));
-  if (probe[0] != 18 || probe[1] != 9)
-    return cleanup(fail(CLB_EDEVICE,
-                        "self-check: the device code does not apply FP64 output modifiers "
-                        "(got %llu, %llu for 18, 9); build it with -mno-amdgpu-ieee",
-                        (unsigned long long)probe[0], (unsigned long long)probe[1]));
+  for (int i = 0; i < 4; i += 2)
+    if (probe[i] != 18 || probe[i + 1] != 9)
+      return cleanup(fail(CLB_EDEVICE,
+                          "self-check: the %s device code does not apply FP64 output modifiers "
+                          "(got %llu, %llu for 18, 9); build it with -mno-amdgpu-ieee",
+                          i == 0 ? "term-kernel" : "replay", (unsigned long long)probe[i],
+                          (unsigned long long)probe[i + 1]));
   if ((e = hipMemsetAsync(ctx->partial, 0, 3 * sizeof(uint64_t), ctx->stream)) != hipSuccess ||
       (e = hipMemsetAsync(ctx->ans, 0, 3 * sizeof(uint64_t), ctx->stream)) != hipSuccess ||
       (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)

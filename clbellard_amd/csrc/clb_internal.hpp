@@ -136,6 +136,7 @@ hipError_t launch_zero(unsigned long long* p, uint64_t nwords, hipStream_t strea
 hipError_t launch_sum7(uint64_t* ans, uint64_t* partial, int shift, hipStream_t stream);
 // Writes 18 and 9 to out2[0..1] when the device code applies FP64 output modifiers.
 hipError_t launch_selfcheck(uint64_t* out2, hipStream_t stream);
+hipError_t launch_replay_selfcheck(uint64_t* out2, hipStream_t stream);
 
 // Reference replay (clb_replay.hip): persistent grid over the replay tiles and tails of `hdr`.
 // Reference replay of the 106-bit mode (precision 0): per BatchCalc job, the reference's 65536

@@ -14,13 +14,15 @@
 // 65536 group accumulators, updated launch by launch, are replayed in the reference's order too
 // (k_r106_groups, k_r106_finish, k_r106_sum7).
 //
-// This file is built in the default FP mode (IEEE mode on, denormals kept) and with contraction
-// off: each fused multiply-add in the double-double code below is one that the reference's OpenCL
-// build makes (OpenCL C contracts a*b + c within one expression) and is written out; every other
-// operation is rounded on its own.  The residues r are exact integers in the reference (its
-// Montgomery exponentiations, Bellard.cl:395-461), so they come from this library's exact
-// engines: the fp64 engine's windowed squares (not its doubling squares, whose output modifier
-// needs the IEEE mode bit clear) for m < 2^48, the integer engine above.
+// Contraction is off: each fused multiply-add in the double-double code below is one that the
+// reference's OpenCL build makes (OpenCL C contracts a*b + c within one expression) and is written
+// out; every other operation is rounded on its own, to nearest, as there.  The file is built in
+// the device FP mode of the other kernels (IEEE mode bit clear, FP64 denormals flushed, NaNs
+// assumed absent), which changes no result here: the mode bit only affects NaN handling and
+// output modifiers, and every value is finite and either 0 or of magnitude above 2^-600, far
+// from the denormal range.  The residues r are exact integers in the reference (its Montgomery
+// exponentiations, Bellard.cl:395-461), so they come from this library's exact engines: the fp64
+// engine of the exact kernels (doubling squares included) for m < 2^48, the integer engine above.
 #include "clb_device.hpp"
 #include "clb_internal.hpp"
 #include "clb_wave.hpp"
@@ -177,34 +179,39 @@ __device__ __forceinline__ int start_shift(uint64_t E0, uint64_t m0) {
   return s;
 }
 
-// start_shift for a kernel kind; 0 for the integer engine, which does not use it.
-template <int KIND>
-__device__ __forceinline__ int engine_shift(uint64_t E0, uint64_t m0) {
-  if constexpr (kWindow<KIND> > 0) {
-    return start_shift<kWindow<KIND>>(E0, m0);
-  } else {
-    return 0;
+// 2^e mod m, balanced (|x| < m), for lane `lane` of a wave whose lane 0 has the exponent E0 and
+// the smallest modulus m0, e = E0 - 10 lane (consecutive k).  For E0 >= 1664 this is the exact
+// kernels' form (fpow2_split): a uniform H = (E0 - 640) >> 10 whose bits take plain or doubling
+// squares, then the lane's v = ((E0 - 640) & 1023) + 640 - 10 lane in [10, 1663] in W-bit windows;
+// the result is within 0.92 m (0.67 m after the final plain square of W = 3 and 1).  Smaller
+// exponents take per-lane windows from the start shift above (within 0.61 m).
+template <int W>
+__device__ __forceinline__ double pow2_lane(uint64_t E0, uint32_t lane, uint64_t m0, double m,
+                                            double minv) {
+  if (E0 >= 1664) {
+    const uint64_t F = E0 - 640;
+    const int tmax = 49 + 63 - __builtin_clzll(m0);
+    return fpow2_split<W>(F >> 10, (uint32_t)(F & 1023) + 640u - 10u * lane, tmax, m, minv);
   }
+  return fpow2<W>(E0 - 10ull * lane, start_shift<W>(E0, m0), m, minv);
 }
 
-// r = 2^e mod m and b = 2^(e + 96) mod m = r 2^96 mod m (:798-799, :1052-1053), exact, with
-// e < 2^bl (bl uniform per wave), as integer-valued doubles in [0, m) (m < 2^53).  s is the fp64
-// engine's start shift (start_shift), unused by the integer engine.
-//  * fp64 engine (KIND kSplitW3/W2/W1, m < 2^42 / 2^46 / 2^48): the start 2^t reduced in one
-//    step (|x| <= 0.69 m, as 2^t / m <= 2^49), per-lane W-bit windows of plain and windowed
-//    balanced squares (fpow2; with |x| <= 0.75 m and m 2^c <= 2^49 each returns within
-//    0.5 + 0.75^2 2^49 1.5 2^-52 < 0.61 m), then two exact shift-and-reduce
-//    steps of 48 bits (|x 2^48 / m - q| <= 0.5 + 0.75 2^48 1.5 2^-52 < 0.6, so the residue stays
-//    below 0.6 m and every intermediate is an exact integer below 2^53);
+// r = 2^e mod m and b = 2^(e + 96) mod m = r 2^96 mod m (:798-799, :1052-1053), exact, for
+// e = E0 - 10 lane (E0 and m0 = lane 0's exponent and modulus, uniform), as integer-valued
+// doubles in [0, m) (m < 2^53).
+//  * fp64 engine (KIND kSplitW3/W2/W1, m < 2^42 / 2^46 / 2^48): pow2_lane (|x| < 0.92 m), then
+//    two exact shift-and-reduce steps of 48 bits (|x 2^48 / m - q| <= 0.5 + 2^48 1.5 2^-52 < 0.6,
+//    so the residue stays below 0.6 m and every intermediate is an exact integer below 2^53);
 //  * int engine (kPlainW3, odd m < 2^55): Montgomery form h = 2^(e + 64) mod m, r = REDC(h), and
 //    b = REDC(h (2^96 mod m)) with 2^96 mod m the Montgomery form of 2^32.
 template <int KIND>
-__device__ __forceinline__ void residues(uint64_t e, int bl, int s, uint64_t m, double& r,
-                                         double& b) {
+__device__ __forceinline__ void residues(uint64_t E0, uint32_t lane, uint64_t m0, uint64_t m,
+                                         double& r, double& b) {
   if constexpr (KIND == kPlainW3) {
     const uint64_t mp = neg_inv64(m);
     const uint64_t one = (0ull - m) % m;
-    const uint64_t h = ipow2_mont<3>(e, bl, m, mp, one);
+    const int bl = 64 - __builtin_clzll(E0);  // every lane's e < 2^bl
+    const uint64_t h = ipow2_mont<3>(E0 - 10ull * lane, bl, m, mp, one);
     const uint64_t c = ipow2_mont<3>(32, 6, m, mp, one);
     uint64_t ri = redc(0, h, m, mp);
     if (ri >= m) ri -= m;
@@ -213,10 +220,9 @@ __device__ __forceinline__ void residues(uint64_t e, int bl, int s, uint64_t m, 
     r = (double)ri;
     b = (double)bi;
   } else {
-    constexpr int W = KIND == kSplitW3 ? 3 : KIND == kSplitW2 ? 2 : 1;
     const double mf = (double)m;
     const double minv = recip(mf);
-    double x = fpow2<W>(e, s, mf, minv);
+    double x = pow2_lane<kWindow<KIND>>(E0, lane, m0, mf, minv);
     r = x < 0.0 ? x + mf : x;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -238,11 +244,8 @@ __device__ U192 replay_item(const ReplayJobDev& jb, uint64_t tile, int lane) {
     const uint64_t k = kw + (uint64_t)lane;
     if (k < jb.k_max) {
       const uint64_t m = jb.den0 * k + jb.den1;
-      const uint64_t E0 = 10ull * (jb.d - kw);
-      const int bl = 64 - __builtin_clzll(E0);
-      const int sh = engine_shift<KIND>(E0, jb.den0 * kw + jb.den1);
       double r, b;
-      residues<KIND>(10ull * (jb.d - k), bl, sh, m, r, b);
+      residues<KIND>(10ull * (jb.d - kw), (uint32_t)lane, jb.den0 * kw + jb.den1, m, r, b);
       const double mf = (double)m;
       const double minv = recip(mf);  // the fp64 engine's own, shared
       dd_step(s0, r, mf, minv);
@@ -334,16 +337,16 @@ __device__ __forceinline__ void fix_one(DD& s) {  // sumfix, :491-501
 
 // r = 2^e mod m alone, exact, as an integer-valued double (see residues).
 template <int KIND>
-__device__ __forceinline__ double residue(uint64_t e, int bl, int s, uint64_t m) {
+__device__ __forceinline__ double residue(uint64_t E0, uint32_t lane, uint64_t m0, uint64_t m) {
   if constexpr (KIND == kPlainW3) {
     const uint64_t mp = neg_inv64(m);
     const uint64_t one = (0ull - m) % m;
-    uint64_t r = redc(0, ipow2_mont<3>(e, bl, m, mp, one), m, mp);
+    const int bl = 64 - __builtin_clzll(E0);
+    uint64_t r = redc(0, ipow2_mont<3>(E0 - 10ull * lane, bl, m, mp, one), m, mp);
     return (double)(r >= m ? r - m : r);
   } else {
-    constexpr int W = KIND == kSplitW3 ? 3 : KIND == kSplitW2 ? 2 : 1;
     const double mf = (double)m;
-    const double x = fpow2<W>(e, s, mf, recip(mf));
+    const double x = pow2_lane<kWindow<KIND>>(E0, lane, m0, mf, recip(mf));
     return x < 0.0 ? x + mf : x;
   }
 }
@@ -378,10 +381,8 @@ __device__ DD item106(const R106JobDev& jb, uint64_t offset, uint64_t k_max, uin
     const uint64_t k = kw + lane;
     if (k < k_max) {
       const uint64_t m = jb.den0 * k + jb.den1;
-      const uint64_t E0 = 10ull * (jb.d - kw);
-      const int bl = 64 - __builtin_clzll(E0);
-      const int sh = engine_shift<KIND>(E0, jb.den0 * kw + jb.den1);
-      const double r = residue<KIND>(10ull * (jb.d - k), bl, sh, m);
+      const double r =
+          residue<KIND>(10ull * (jb.d - kw), (uint32_t)lane, jb.den0 * kw + jb.den1, m);
       const double mf = (double)m;
       if (is64) {
         dd_step(s, r, mf, recip(mf));
@@ -421,8 +422,8 @@ template <int KIND>
 __global__ void __launch_bounds__(256) k_r106_groups(const R106JobDev* __restrict__ jobs,
                                                     uint64_t gsize) {
   __shared__ DD p[256];
-  const R106JobDev jb = jobs[blockIdx.y];
-  const uint32_t g = blockIdx.x, tid = threadIdx.x;
+  const R106JobDev jb = jobs[block_y()];
+  const uint32_t g = block_x(), tid = thread_x();
   const uint64_t span = jb.k1 > jb.k0 ? jb.k1 - jb.k0 : 0;
   if ((uint64_t)g * 256 >= (span < gsize ? span : gsize)) return;
   double* acc_ptr = reinterpret_cast<double*>(jb.bigsum) + 2 * (size_t)g;
@@ -460,8 +461,8 @@ __global__ void __launch_bounds__(256) k_r106_groups(const R106JobDev* __restric
 __global__ void __launch_bounds__(256) k_r106_finish(const R106JobDev* __restrict__ jobs) {
   __shared__ DD p[256];
   __shared__ DD small[256];
-  const R106JobDev jb = jobs[blockIdx.x];
-  const uint32_t tid = threadIdx.x;
+  const R106JobDev jb = jobs[block_x()];
+  const uint32_t tid = thread_x();
   double* bs = reinterpret_cast<double*>(jb.bigsum);
   if (tid == 0 && jb.tail) {
     const DD b0 = dd_add({bs[0], bs[1]}, tail106(jb));
@@ -490,7 +491,7 @@ __global__ void __launch_bounds__(256) k_r106_finish(const R106JobDev* __restric
 __global__ void k_r106_sum7(const double* parts, const uint32_t* first, const uint32_t* count,
                             const uint8_t* shift, double* ans, double* slot_val,
                             uint32_t ngroups) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t g = block_x() * block_dim_x() + thread_x();
   if (g >= ngroups) return;
   DD a = {ans[2 * g], ans[2 * g + 1]};
   for (uint32_t i = 0; i < count[g]; ++i) {
@@ -521,10 +522,10 @@ __global__ void __launch_bounds__(kBlock) k_replay(LaunchDev* __restrict__ hdr) 
   const int njobs = (int)hdr->nrjobs;
   const uint64_t total = hdr->rtotal_tiles;
   const uint64_t chunk = hdr->rchunk;
-  const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
-  const uint64_t wave_id = (uint64_t)blockIdx.x * (kBlock / kWave) +
-                           (uint64_t)__builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int lane = (int)(threadIdx.x & (kWave - 1));
+  const uint64_t nwaves = (uint64_t)grid_dim_x() * (kBlock / kWave);
+  const uint64_t wave_id = (uint64_t)block_x() * (kBlock / kWave) +
+                           (uint64_t)__builtin_amdgcn_readfirstlane(thread_x() / kWave);
+  const int lane = (int)(thread_x() & (kWave - 1));
   const uint32_t replicas = hdr->replicas;
   const uint32_t rep = (uint32_t)wave_id & (replicas - 1);
   const TailDev* __restrict__ tails = hdr_rtails(hdr);
@@ -613,6 +614,22 @@ hipError_t launch_replay106_sum7(const double* parts, const uint32_t* first, con
   if (ngroups == 0) return hipSuccess;
   hipLaunchKernelGGL(k_r106_sum7, dim3((ngroups + 63) / 64), dim3(64), 0, stream, parts, first,
                      count, shift, ans, slot_val, ngroups);
+  return hipGetLastError();
+}
+
+// This file's own start-up check of the FP64 output modifier (its code object is built apart from
+// clb_kernels.hip's): one lane runs fsq_bits on 3 mod 1000003 with the exponent bit set (18) and
+// clear (9).
+__global__ void k_replay_selfcheck(uint64_t* out) {
+  if (thread_x() != 0) return;
+  const double m = 1000003.0;
+  const double minv = recip(m);
+  out[0] = (uint64_t)(int64_t)fsq_bits(3.0, 2ull, 1u, m, minv);
+  out[1] = (uint64_t)(int64_t)fsq_bits(3.0, 1ull, 1u, m, minv);
+}
+
+hipError_t launch_replay_selfcheck(uint64_t* out2, hipStream_t stream) {
+  hipLaunchKernelGGL(k_replay_selfcheck, dim3(1), dim3(64), 0, stream, out2);
   return hipGetLastError();
 }
 

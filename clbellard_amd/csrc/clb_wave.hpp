@@ -1,11 +1,27 @@
-// clb_wave.hpp -- wave-level 192-bit reductions and the accumulator's atomic limb adds, shared by
-// the term kernels (clb_kernels.hip) and the reference-replay kernel (clb_replay.hip).
+// clb_wave.hpp -- work-item indices, wave-level 192-bit reductions and the accumulator's atomic
+// limb adds, shared by the term kernels (clb_kernels.hip) and the reference-replay kernels
+// (clb_replay.hip).
 #pragma once
 
 #include "clb_device.hpp"
 #include "clb_internal.hpp"
 
 namespace clb {
+
+// Work-item and work-group indices from the hardware registers, block size and count from the
+// hidden kernel arguments (code object v5: hidden_block_count_x at offset 0, hidden_group_size_x
+// at offset 12, both in the kernarg segment in device memory).  threadIdx and friends go through
+// device-library functions built for the default FP mode, which cannot be inlined into code built
+// with the IEEE mode bit clear, and the calls would cost registers and spills.  The grid size is
+// not taken from the dispatch packet: that lives in the queue's host-memory ring buffer, and every
+// wave reading it added up to 13 us to a small launch.
+__device__ __forceinline__ uint32_t thread_x() { return __builtin_amdgcn_workitem_id_x(); }
+__device__ __forceinline__ uint32_t block_x() { return __builtin_amdgcn_workgroup_id_x(); }
+__device__ __forceinline__ uint32_t block_y() { return __builtin_amdgcn_workgroup_id_y(); }
+__device__ __forceinline__ uint32_t block_dim_x() { return __builtin_amdgcn_workgroup_size_x(); }
+__device__ __forceinline__ uint32_t grid_dim_x() {
+  return *(const uint32_t*)__builtin_amdgcn_implicitarg_ptr();  // C cast: address space 4
+}
 
 __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int mask) {
   const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, mask, kWave);
