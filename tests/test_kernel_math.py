@@ -276,3 +276,29 @@ def test_dekker_product_is_error_free():
         assert p == a * b
         assert Fraction(e) == Fraction(a) * Fraction(b) - Fraction(p), (a, b)
         assert e == fma(a, b, -p)
+
+
+def test_nn_div_final_sum_is_fast():
+    """nn_div's last step two_sum(z1, z2) may be Fast2Sum: with z1 = RN(x / y) (correctly rounded,
+    as the device division is) and z2 the rounded quotient of the residual, |z2| <= |z1|, and
+    both sums return the same two doubles.  Numerators in [0, y], integer moduli y < 2^53."""
+    def two_sum(a, b):
+        s = a + b
+        t = s - a
+        return s, (a - (s - t)) + (b - t)
+
+    def fast_two_sum(a, b):
+        s = a + b
+        return s, b - (s - a)
+
+    rng = random.Random(23)
+    cases = [(0.0, 7.0), (1.0, 1.0), (3.0, 3.0), (2.0**53 - 2, 2.0**53 - 1)]
+    for _ in range(20000):
+        y = rng.randrange(1, 1 << rng.choice((20, 33, 42, 48, 53)))
+        cases.append((float(rng.randrange(0, y + 1)), float(y)))
+    for x, y in cases:
+        z1 = x / y
+        p, e = _dekker_fused(-z1, y)
+        z2 = ((p + x) + e) / y
+        assert abs(z2) <= abs(z1), (x, y)
+        assert two_sum(z1, z2) == fast_two_sum(z1, z2), (x, y)
