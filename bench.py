@@ -371,7 +371,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
-        from clb_testlib import load_oracle, w2i, SHIFT  # the checker, timed as the baseline
+        from clb_testlib import load_oracle, w2i, SHIFT  # the oracle, timed as the baseline
         orc = load_oracle()
         threads = min(16, os.cpu_count() or 1)
         # sample: the first window of each formula, shortened to fit the CPU time budget

@@ -89,7 +89,7 @@ def build_oracle() -> str:
 
 def build_reference_kernels(ref_dir: str = "/root/reference") -> bool:
     """The reference's OpenCL kernels compiled from their sources into oracle/_ref/, with the
-    replay host (GPU-side checker for the tests; not part of the product).  Skipped, returning
+    replay host (GPU-side reference for the tests; not part of the product).  Skipped, returning
     False, where the reference sources are absent."""
     if not os.path.exists(os.path.join(ref_dir, "Bellard.cl")):
         return False

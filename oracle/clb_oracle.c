@@ -1,7 +1,7 @@
 /*
  * clb_oracle.c -- CPU restatement of CLBellard's exact-integer digit-extraction path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  Only tests/, the smoke check and bench.py's
+ * TEST INFRASTRUCTURE ONLY (the oracle).  Only tests/, the smoke check and bench.py's
  * cpu_baseline leg load this library; the product never does.  See clb_oracle.h for the
  * word order, the exact-integer semantics and the pinning status of each function.
  */

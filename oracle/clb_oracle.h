@@ -1,7 +1,7 @@
 /*
  * clb_oracle.h -- CPU restatement of CLBellard's exact-integer digit-extraction path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/, the smoke check in
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle: only tests/, the smoke check in
  * __graft_entry__.py and the cpu_baseline leg of bench.py may load it.  The product
  * (clbellard_amd, libclbellard.so) never links, loads or calls it.
  *

@@ -3,7 +3,7 @@
  * compiled from the reference sources into oracle/_ref/ by oracle/Makefile) through
  * the OpenCL runtime, replaying the reference host's launch sequence.
  *
- * TEST INFRASTRUCTURE ONLY (a checker, and the reference's own GPU rate beside ours).  Only the
+ * TEST INFRASTRUCTURE ONLY (the reference's own kernels for parity tests, and their GPU rate beside ours).  Only the
  * GPU tests load it; the product never does.
  *
  * Replayed host code (toropippi/CLBellard):

@@ -1,6 +1,6 @@
 """Shared test helpers: the CPU oracle (oracle/liboracle.so) and exact big-integer references.
 
-The oracle is test infrastructure (the checker); the product library never loads it.
+The oracle is test infrastructure; the product library never loads it.
 """
 from __future__ import annotations
 

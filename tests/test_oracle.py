@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/clb_oracle.c) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/clb_oracle.c) before the parity tests trust it.
 
 Each oracle function is checked against something the mathematics fixes: closed forms, Python's
 exact integers (pow, //), Bellard's series for pi as an exact rational, and the hex digits of pi
