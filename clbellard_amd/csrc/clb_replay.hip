@@ -160,14 +160,16 @@ __device__ U192 to192(DD s0, DD s1) {
   return {u2, u1, u0};
 }
 
-// One term's update of a sum (:786-796, :1037-1048): while the sum is positive the numerator is
-// m - r and the quotient is negated, which keeps the sum in (-1, 1).  r and m are exact integers
-// held in doubles (m < 2^53, so m - r is exact too), minv = recip(m).
-__device__ __forceinline__ void dd_step(DD& s, double r, double m, double minv) {
-  const bool pos = s.hi > 0.0;
-  DD q = div_nn_m(pos ? m - r : r, m, minv);
-  if (pos) q = dd_neg(q);
-  s = dd_add(s, q);
+// One term's update of a sum (:786-796, :1037-1048) from the residue x of 2^e mod m, |x| < m
+// (r = x, or x + m when x < 0): while the sum is positive the reference divides m - r and negates
+// the quotient, which keeps the sum in (-1, 1).  nn_div and addcab are odd-symmetric under round
+// to nearest (every operation is), so dividing r - m instead gives the same sum: equal doubles,
+// a zero at most differing in sign, which no later operation here turns into a nonzero
+// difference.  The numerator is then one exact fma, x + c m with c = [x < 0] - [sum > 0], an
+// integer below 2^53 in magnitude.  m is an exact integer held in a double, minv = recip(m).
+__device__ __forceinline__ void dd_step(DD& s, double x, double m, double minv) {
+  const double c = (x < 0.0 ? 1.0 : 0.0) - (s.hi > 0.0 ? 1.0 : 0.0);
+  s = dd_add(s, div_nn_m(__builtin_fma(c, m, x), m, minv));
 }
 
 // Window width of the fp64 engine for a kernel kind (0 for the integer engine).
@@ -205,17 +207,18 @@ __device__ __forceinline__ double pow2_lane(uint64_t E0, uint32_t lane, uint64_t
   return fpow2<W>(E0 - 10ull * lane, start_shift<W>(E0, m0), m, minv);
 }
 
-// r = 2^e mod m and b = 2^(e + 96) mod m = r 2^96 mod m (:798-799, :1052-1053), exact, for
+// x = 2^e mod m and b = 2^(e + 96) mod m = x 2^96 mod m (:798-799, :1052-1053), exact, for
 // e = E0 - 10 lane (E0 and m0 = lane 0's exponent and modulus, uniform), as integer-valued
-// doubles in [0, m) (m < 2^53).
-//  * fp64 engine (KIND kSplitW3/W2/W1, m < 2^42 / 2^46 / 2^48): pow2_lane (|x| < 0.92 m), then
-//    two exact shift-and-reduce steps of 48 bits (|x 2^48 / m - q| <= 0.5 + 2^48 1.5 2^-52 < 0.6,
-//    so the residue stays below 0.6 m and every intermediate is an exact integer below 2^53);
-//  * int engine (kPlainW3, odd m < 2^55): Montgomery form h = 2^(e + 64) mod m, r = REDC(h), and
-//    b = REDC(h (2^96 mod m)) with 2^96 mod m the Montgomery form of 2^32.
+// doubles of magnitude below m (m < 2^53; mf = m as a double, minv = recip(mf)).
+//  * fp64 engine (KIND kSplitW3/W2/W1, m < 2^42 / 2^46 / 2^48): balanced residues, pow2_lane
+//    (|x| < 0.92 m), then two exact shift-and-reduce steps of 48 bits (|x 2^48 / m - q| <=
+//    0.5 + 2^48 1.5 2^-52 < 0.6, so |b| < 0.6 m and every intermediate is an exact integer below
+//    2^53);
+//  * int engine (kPlainW3, odd m < 2^55): residues in [0, m), Montgomery form h = 2^(e + 64) mod m,
+//    x = REDC(h), and b = REDC(h (2^96 mod m)) with 2^96 mod m the Montgomery form of 2^32.
 template <int KIND>
 __device__ __forceinline__ void residues(uint64_t E0, uint32_t lane, uint64_t m0, uint64_t m,
-                                         double& r, double& b) {
+                                         double mf, double minv, double& x, double& b) {
   if constexpr (KIND == kPlainW3) {
     const uint64_t mp = neg_inv64(m);
     const uint64_t one = (0ull - m) % m;
@@ -226,19 +229,16 @@ __device__ __forceinline__ void residues(uint64_t E0, uint32_t lane, uint64_t m0
     if (ri >= m) ri -= m;
     uint64_t bi = redc(mulhi64(h, c), h * c, m, mp);
     if (bi >= m) bi -= m;
-    r = (double)ri;
+    x = (double)ri;
     b = (double)bi;
   } else {
-    const double mf = (double)m;
-    const double minv = recip(mf);
-    double x = pow2_lane<kWindow<KIND>>(E0, lane, m0, mf, minv);
-    r = x < 0.0 ? x + mf : x;
+    x = pow2_lane<kWindow<KIND>>(E0, lane, m0, mf, minv);
+    b = x;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      const double z = __builtin_amdgcn_ldexp(x, 48);
-      x = __builtin_fma(-__builtin_rint(z * minv), mf, z);
+      const double z = __builtin_amdgcn_ldexp(b, 48);
+      b = __builtin_fma(-__builtin_rint(z * minv), mf, z);
     }
-    b = x < 0.0 ? x + mf : x;
   }
 }
 
@@ -248,18 +248,28 @@ __device__ __forceinline__ void residues(uint64_t E0, uint32_t lane, uint64_t m0
 template <int KIND>
 __device__ U192 replay_item(const ReplayJobDev& jb, uint64_t tile, int lane) {
   DD s0 = {0.0, 0.0}, s1 = {0.0, 0.0};
+  const uint64_t d = jb.d, k_max = jb.k_max, gsize = jb.gsize, den0 = jb.den0, den1 = jb.den1;
   const uint64_t k_lane0 = jb.offset + tile * kWave;
-  for (uint64_t kw = k_lane0; kw < jb.k_max; kw += jb.gsize) {
+  // The lane's modulus advances by den0 gsize per term: for the fp64 engine one exact add of
+  // doubles (every m used is below 2^53), for the int engine a 64-bit add and a conversion.
+  uint64_t m = den0 * (k_lane0 + (uint64_t)lane) + den1;
+  double mf = (double)m;
+  const uint64_t mstep = den0 * gsize;
+  const double mstepf = (double)mstep;
+  for (uint64_t kw = k_lane0; kw < k_max; kw += gsize) {
     const uint64_t k = kw + (uint64_t)lane;
-    if (k < jb.k_max) {
-      const uint64_t m = jb.den0 * k + jb.den1;
-      double r, b;
-      residues<KIND>(10ull * (jb.d - kw), (uint32_t)lane, jb.den0 * kw + jb.den1, m, r, b);
-      const double mf = (double)m;
-      const double minv = recip(mf);  // the fp64 engine's own, shared
-      dd_step(s0, r, mf, minv);
+    if (k < k_max) {
+      if constexpr (KIND == kPlainW3) mf = (double)m;
+      const double minv = recip(mf);  // the exponent engine's and the divisions'
+      double x, b;
+      residues<KIND>(10ull * (d - kw), (uint32_t)lane, den0 * kw + den1, m, mf, minv, x, b);
+      dd_step(s0, x, mf, minv);
       dd_step(s1, b, mf, minv);
     }
+    if constexpr (KIND == kPlainW3)
+      m += mstep;
+    else
+      mf += mstepf;
   }
   // The reference's k after its loop has the parity of offset + idx (gsize is even).
   if ((jb.numesign + (k_lane0 + (uint64_t)lane) % 2) % 2 == 1) {
@@ -344,9 +354,11 @@ __device__ __forceinline__ void fix_one(DD& s) {  // sumfix, :491-501
   }
 }
 
-// r = 2^e mod m alone, exact, as an integer-valued double (see residues).
+// x = 2^e mod m alone, exact, as an integer-valued double: balanced for the fp64 engine, in
+// [0, m) for the int engine (see residues).
 template <int KIND>
-__device__ __forceinline__ double residue(uint64_t E0, uint32_t lane, uint64_t m0, uint64_t m) {
+__device__ __forceinline__ double residue(uint64_t E0, uint32_t lane, uint64_t m0, uint64_t m,
+                                          double mf, double minv) {
   if constexpr (KIND == kPlainW3) {
     const uint64_t mp = neg_inv64(m);
     const uint64_t one = (0ull - m) % m;
@@ -354,9 +366,7 @@ __device__ __forceinline__ double residue(uint64_t E0, uint32_t lane, uint64_t m
     uint64_t r = redc(0, ipow2_mont<3>(E0 - 10ull * lane, bl, m, mp, one), m, mp);
     return (double)(r >= m ? r - m : r);
   } else {
-    const double mf = (double)m;
-    const double x = pow2_lane<kWindow<KIND>>(E0, lane, m0, mf, recip(mf));
-    return x < 0.0 ? x + mf : x;
+    return pow2_lane<kWindow<KIND>>(E0, lane, m0, mf, minv);
   }
 }
 
@@ -386,20 +396,29 @@ __device__ DD item106(const R106JobDev& jb, uint64_t offset, uint64_t k_max, uin
   // tid - lane is uniform, and per-wave values (exponent length, start shift) would take VALU.
   const uint64_t wave0 =
       (uint64_t)g * 256 + __builtin_amdgcn_readfirstlane(tid & ~(uint32_t)(kWave - 1));
+  // m advances by den0 gsize per term (as in replay_item).
+  uint64_t m = jb.den0 * (offset + idx) + jb.den1;
+  double mf = (double)m;
+  const uint64_t mstep = jb.den0 * gsize;
+  const double mstepf = (double)mstep;
   for (uint64_t kw = offset + wave0; kw < k_max; kw += gsize) {  // kw: the wave's lane 0
     const uint64_t k = kw + lane;
     if (k < k_max) {
-      const uint64_t m = jb.den0 * k + jb.den1;
-      const double r =
-          residue<KIND>(10ull * (jb.d - kw), (uint32_t)lane, jb.den0 * kw + jb.den1, m);
-      const double mf = (double)m;
+      if constexpr (KIND == kPlainW3) mf = (double)m;
+      const double minv = recip(mf);
+      const double x =
+          residue<KIND>(10ull * (jb.d - kw), (uint32_t)lane, jb.den0 * kw + jb.den1, m, mf, minv);
       if (is64) {
-        dd_step(s, r, mf, recip(mf));
+        dd_step(s, x, mf, minv);
       } else {
-        s = dd_add(s, div_nn_m(r, mf, recip(mf)));
+        s = dd_add(s, div_nn_m(x < 0.0 ? x + mf : x, mf, minv));
         fix_one(s);
       }
     }
+    if constexpr (KIND == kPlainW3)
+      m += mstep;
+    else
+      mf += mstepf;
   }
   if ((jb.numesign + (offset + idx) % 2) % 2 == 1) s = dd_neg(s);
   return s;

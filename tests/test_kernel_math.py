@@ -3,8 +3,9 @@ rounded fma, checked against exact integers.  It pins the error analysis the ker
 the balanced modular squares (plain, windowed, doubled) stay exact and within 1.02-1.34 m, the
 exponent chain gives 2^e mod m, the lazy 48-bit quotient digits and the per-lane digit sums
 reconstruct floor(r 2^192 / m), the tail's long division of 2^E (whose first digit may
-estimate -1) gives floor(2^E / m), and the reference's fused Dekker product (which the replay
-computes with one fma) is error-free.  The GPU tests check the kernels themselves."""
+estimate -1) gives floor(2^E / m), the reference's fused Dekker product (which the replay
+computes with one fma) is error-free, and the replay's signed numerator gives the reference's
+negated quotient.  The GPU tests check the kernels themselves."""
 import math
 import random
 from fractions import Fraction
@@ -302,3 +303,61 @@ def test_nn_div_final_sum_is_fast():
         z2 = ((p + x) + e) / y
         assert abs(z2) <= abs(z1), (x, y)
         assert two_sum(z1, z2) == fast_two_sum(z1, z2), (x, y)
+
+
+def test_replay_step_numerator_fold():
+    """The replay's dd_step divides r - m, one fma from the balanced residue x (r = x, or x + m
+    when x < 0), where the reference divides m - r and negates the quotient while its sum is
+    positive (Bellard.cl:1037-1048).  Every operation of nn_div and addcab is odd-symmetric under
+    round to nearest, so the new sum is the same: equal doubles, a zero at most differing in
+    sign.  Sums in (-1, 1) (cancelling ones too), |x| < m, integer moduli m < 2^53."""
+    def two_sum(a, b):
+        s = a + b
+        t = s - a
+        return s, (a - (s - t)) + (b - t)
+
+    def nn_div(x, y):  # correctly rounded quotients, twoproduct as one fma, final Fast2Sum
+        z1 = x / y
+        p = -z1 * y
+        e = fma(-z1, y, -p)
+        z2 = ((p + x) + e) / y
+        s = z1 + z2
+        return s, z2 - (s - z1)
+
+    def addcab(a, b):
+        c = two_sum(a[0], b[0])
+        return two_sum(c[0], (c[1] + a[1]) + b[1])
+
+    def ref_step(s, x, m):
+        r = x + m if x < 0 else x
+        pos = s[0] > 0.0
+        q = nn_div(m - r if pos else r, m)
+        if pos:
+            q = (-q[0], -q[1])
+        return addcab(s, q)
+
+    def folded_step(s, x, m):
+        c = (1.0 if x < 0.0 else 0.0) - (1.0 if s[0] > 0.0 else 0.0)
+        return addcab(s, nn_div(fma(c, m, x), m))
+
+    rng = random.Random(29)
+    seen = set()
+    for i in range(30000):
+        m = rng.randrange(1, 1 << rng.choice((20, 33, 42, 48, 53)))
+        x = rng.randrange(-m + 1, m) if m > 1 else 0
+        r = x + m if x < 0 else x
+        kind = i % 4
+        if kind == 0:  # a general sum in (-1, 1)
+            hi = rng.uniform(-1.0, 1.0)
+        elif kind == 1:  # one that the positive-branch quotient -(m - r) / m cancels
+            hi = (m - r) / m * (1.0 + rng.choice((0.0, 2.0**-52, -(2.0**-52))))
+        elif kind == 2:  # one that the non-positive-branch quotient r / m cancels
+            hi = -(r / m)
+        else:  # zero or tiny
+            hi = rng.choice((0.0, -0.0, 2.0**-60, -(2.0**-70)))
+        lo = rng.uniform(-0.5, 0.5) * math.ulp(hi) if hi != 0.0 else 0.0
+        s = two_sum(hi, lo)
+        a, b = ref_step(s, float(x), float(m)), folded_step(s, float(x), float(m))
+        assert a == b, (s, x, m, a, b)
+        seen.add((s[0] > 0.0, x < 0))
+    assert len(seen) == 4
