@@ -14,6 +14,7 @@
 // launch (clb_batch_positions); prints "N' digits" per position in input order.  --graph replays
 // the captured launch sequence (clb_set_graph); --autotune MS first tunes tile length and grid on
 // the largest position for about MS milliseconds (clb_autotune).
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -40,21 +41,45 @@ static int die(int rc) {
   return 1;
 }
 
-// Positions, one per line; blank lines skipped.  Returns false on an unreadable file or a line
-// that is not a decimal number.
+// A decimal number: digits only (no sign, no blanks), at most 2^64 - 1.
+static bool parse_u64(const char* s, size_t len, unsigned long long* out) {
+  if (len == 0) return false;
+  unsigned long long v = 0;
+  for (size_t i = 0; i < len; ++i) {
+    if (s[i] < '0' || s[i] > '9') return false;
+    const unsigned d = (unsigned)(s[i] - '0');
+    if (v > (ULLONG_MAX - d) / 10) return false;
+    v = v * 10 + d;
+  }
+  *out = v;
+  return true;
+}
+
+static bool parse_u64(const char* s, unsigned long long* out) {
+  return s && parse_u64(s, strlen(s), out);
+}
+
+// Positions, one per line (blanks around the number allowed); blank lines skipped.  Returns
+// false on an unreadable file or a line that is not a decimal number.
 static bool read_positions(const std::string& path, std::vector<uint64_t>* out) {
   FILE* f = path == "-" ? stdin : fopen(path.c_str(), "r");
   if (!f) return false;
-  char line[256];
+  char line[4096];
   bool ok = true;
   while (ok && fgets(line, sizeof line, f)) {
-    char* p = line;
-    while (*p == ' ' || *p == '\t') ++p;
-    if (*p == '\n' || *p == '\r' || *p == 0) continue;
-    char* end = nullptr;
-    const unsigned long long v = strtoull(p, &end, 10);
-    while (end && (*end == ' ' || *end == '\t' || *end == '\r' || *end == '\n')) ++end;
-    if (end == p || (end && *end != 0)) ok = false;
+    const size_t n = strlen(line);
+    if (n + 1 == sizeof line && line[n - 1] != '\n') {  // longer than any number
+      ok = false;
+      break;
+    }
+    size_t a = 0, b = n;
+    while (a < b && (line[a] == ' ' || line[a] == '\t')) ++a;
+    while (b > a && (line[b - 1] == ' ' || line[b - 1] == '\t' || line[b - 1] == '\r' ||
+                     line[b - 1] == '\n'))
+      --b;
+    if (a == b) continue;
+    unsigned long long v = 0;
+    ok = parse_u64(line + a, b - a, &v);
     out->push_back(v);
   }
   if (f != stdin) fclose(f);
@@ -83,35 +108,42 @@ int main(int argc, char** argv) {
   unsigned long long batch = 0, replay = 0;
   bool graph = false;
   double tune_ms = 0.0;
+  // Every option is checked here, before any GPU call: a malformed one prints the usage.
   for (int i = 2; i < argc; ++i) {
     const std::string a = argv[i];
     auto next = [&]() -> const char* { return i + 1 < argc ? argv[++i] : nullptr; };
     const char* v = nullptr;
-    if (a == "--precision" && (v = next())) {
-      precision = atoi(v);
-    } else if (a == "--device" && (v = next())) {
-      device = atoi(v);
+    unsigned long long u = 0;
+    if (a == "--precision" && parse_u64(v = next(), &u) && u <= 2) {
+      precision = (int)u;
+    } else if (a == "--device" && parse_u64(v = next(), &u) && u <= 1024) {
+      device = (int)u;
     } else if (a == "--workdir" && (v = next())) {
       workdir = v;
-    } else if (a == "--batch-size" && (v = next())) {
-      batch = strtoull(v, nullptr, 10);
-    } else if (a == "--kernel" && (v = next())) {
+    } else if (a == "--batch-size" && parse_u64(v = next(), &u)) {
+      batch = u;
+    } else if (a == "--kernel" && (v = next()) &&
+               (!strcmp(v, "auto") || !strcmp(v, "plain") || !strcmp(v, "split"))) {
       kernel = !strcmp(v, "plain") ? CLB_KERNEL_PLAIN : !strcmp(v, "split") ? CLB_KERNEL_SPLIT
                                                                             : CLB_KERNEL_AUTO;
     } else if (a == "--graph") {
       graph = true;
     } else if (a == "--autotune" && (v = next())) {
-      tune_ms = atof(v);
+      char* end = nullptr;
+      tune_ms = strtod(v, &end);
+      if (end == v || *end != 0 || !(tune_ms > 0.0 && tune_ms < 1e7)) return usage();
     } else if (a == "--replay") {
       replay = CLB_REF_GLOBAL_WORK_SIZE;
-    } else if (a == "--replay-gws" && (v = next())) {
-      replay = strtoull(v, nullptr, 10);
+    } else if (a == "--replay-gws" && parse_u64(v = next(), &u) && u > 0) {
+      replay = u;
     } else if (positional.empty() && (a[0] != '-' || a == "-")) {
       positional = a;
     } else {
       return usage();
     }
   }
+  unsigned long long digits = 0;
+  if (cmd == "normal" && !parse_u64(positional.c_str(), &digits)) return usage();
   if (cmd == "sum") {
     char hex[49];
     int counts[7];
@@ -135,7 +167,7 @@ int main(int argc, char** argv) {
   if (!rc && cmd == "normal") {
     uint64_t w[3], n_out = 0;
     char hex[49];
-    rc = clb_normal(ctx, strtoull(positional.c_str(), nullptr, 10), w, hex, &n_out);
+    rc = clb_normal(ctx, digits, w, hex, &n_out);
     if (!rc) {
       double head, total;
       clb_last_timing(ctx, &head, &total);

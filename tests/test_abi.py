@@ -211,9 +211,33 @@ def test_cli_batch_rejects_bad_positions(tmp_path):
     bad = tmp_path / "bad.txt"
     bad.write_text("1000000\n12x4\n")
     for args, stdin in (([str(bad)], None), ([str(tmp_path / "missing.txt")], None),
-                        (["-"], "250000\n1e6\n")):
+                        (["-"], "250000\n1e6\n"), (["-"], "-5\n"), (["-"], "+250000\n"),
+                        (["-"], "18446744073709551616\n"), (["-"], "1" * 5000 + "\n")):
         out = subprocess.run([cli, "batch", *args], input=stdin, capture_output=True, text=True)
         assert out.returncode == 1 and "cannot read positions" in out.stderr, (args, out.stderr)
+
+
+def test_cli_rejects_malformed_options():
+    """Every option and the position of `clbellard normal` are parsed strictly before any GPU
+    call: trailing characters, signs, overflow, out-of-range precision, unknown kernel names and
+    missing values print the usage (exit status 2)."""
+    import subprocess
+    cli = os.path.join(ROOT, "clbellard_amd", "clbellard")
+    if not os.path.exists(cli):
+        pytest.skip("CLI not built")
+    for args in (["normal", "12abc"], ["normal", "-5"], ["normal", "+1000000"], ["normal", ""],
+                 ["normal", "18446744073709551616"], ["normal"],
+                 ["normal", "1000000", "--precision", "3"],
+                 ["normal", "1000000", "--precision", "1x"],
+                 ["normal", "1000000", "--precision"],
+                 ["normal", "1000000", "--device", "-1"],
+                 ["normal", "1000000", "--kernel", "fast"],
+                 ["normal", "1000000", "--replay-gws", "0"],
+                 ["normal", "1000000", "--replay-gws", "2e24"],
+                 ["batch", "-", "--autotune", "0"], ["batch", "-", "--autotune", "5ms"],
+                 ["work", "w.txt", "--batch-size", "-1"]):
+        out = subprocess.run([cli, *args], input="", capture_output=True, text=True)
+        assert out.returncode == 2 and "usage" in out.stderr, (args, out.returncode, out.stderr)
 
 
 def test_write_work_file_round_trip(tmp_path, oracle):
