@@ -319,3 +319,38 @@ def test_call_size_limits(ctx):
     job = (clb._Job * 1)()
     assert clb.lib.clb_run_jobs(ctx._ctx, job, (1 << 24) + 1, out) == clb.EINVAL
     assert w2i(ctx.batch_positions([10**6])[0]) == w2i(ctx.normal(10**6)[0])
+
+
+def test_contexts_on_threads(oracle):
+    """Calls on different contexts may run from different threads (include/clbellard.h): four
+    threads, each with its own context and stream, compute different positions at the same time,
+    three times each; every result equals the oracle's, and each thread's error message (an
+    invalid position of its own) is its own."""
+    import threading
+    ns = [65540, 100000, 250000, 777775]
+    want = {n: oracle.normal(n) for n in ns}
+    got, errors, failures = {}, {}, []
+    start = threading.Barrier(len(ns))
+
+    def work(n):
+        try:
+            with clb.Context(0, clb.PREC_U192) as c:
+                start.wait()
+                got[n] = [w2i(c.normal(n)[0]) for _ in range(3)]
+                with pytest.raises(clb.ClbError) as e:
+                    c.batch_calc(n, 0, n, 7 + ns.index(n))  # formula ids past 6, one per thread
+                errors[n] = str(e.value)
+        except BaseException as e:  # reported by the main thread
+            failures.append((n, repr(e)))
+
+    note(f"threads: positions {ns}")
+    threads = [threading.Thread(target=work, args=(n,)) for n in ns]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(120)
+    assert not any(t.is_alive() for t in threads), "a thread did not finish"
+    assert not failures, failures
+    for n in ns:
+        assert got[n] == [want[n]] * 3, n
+        assert f"formula {7 + ns.index(n)} (" in errors[n], errors[n]
