@@ -142,8 +142,14 @@ int main(int argc, char** argv) {
       return usage();
     }
   }
+  // The command, the work-item count of the replay and every digit position are checked here too,
+  // still before the first GPU call (clb_normalize is host-only).
+  if (cmd != "normal" && cmd != "work" && cmd != "sum" && cmd != "batch") return usage();
+  if (replay && (replay % 256 != 0 || replay > CLB_REF_GLOBAL_WORK_SIZE)) return usage();
   unsigned long long digits = 0;
-  if (cmd == "normal" && !parse_u64(positional.c_str(), &digits)) return usage();
+  if (cmd == "normal" && (!parse_u64(positional.c_str(), &digits) ||
+                          clb_normalize(digits, nullptr, nullptr, nullptr) != CLB_OK))
+    return usage();
   if (cmd == "sum") {
     char hex[49];
     int counts[7];
@@ -160,6 +166,11 @@ int main(int argc, char** argv) {
             positional.c_str());
     return 1;
   }
+  for (uint64_t n : pos)
+    if (clb_normalize(n, nullptr, nullptr, nullptr) != CLB_OK) {
+      fprintf(stderr, "position %llu: %s\n", (unsigned long long)n, clb_last_error());
+      return usage();
+    }
   clb_ctx* ctx = nullptr;
   if (int rc = clb_ctx_create(device, precision, &ctx)) return die(rc);
   int rc = clb_set_kernel(ctx, kernel);
@@ -196,9 +207,6 @@ int main(int argc, char** argv) {
     int ran = 0;
     rc = clb_run_worklist(ctx, positional.c_str(), workdir.c_str(), batch, nullptr, &ran);
     if (!rc) printf("%d job file(s) written to %s\n", ran, workdir.c_str());
-  } else if (!rc) {
-    clb_ctx_destroy(ctx);
-    return usage();
   }
   clb_ctx_destroy(ctx);
   return rc ? die(rc) : 0;

@@ -198,7 +198,7 @@ def test_cli_sum_and_usage(tmp_path):
     assert f"{(v1 + v2) % 2**192:048x}" in out.stdout
     assert "0:1" in out.stdout and "5:1" in out.stdout and "3:0" in out.stdout
     assert subprocess.run([cli], capture_output=True).returncode == 2
-    assert subprocess.run([cli, "bogus", "1"], capture_output=True).returncode != 0
+    assert subprocess.run([cli, "bogus", "1"], capture_output=True).returncode == 2
 
 
 def test_cli_batch_rejects_bad_positions(tmp_path):
@@ -234,10 +234,28 @@ def test_cli_rejects_malformed_options():
                  ["normal", "1000000", "--kernel", "fast"],
                  ["normal", "1000000", "--replay-gws", "0"],
                  ["normal", "1000000", "--replay-gws", "2e24"],
+                 ["normal", "1000000", "--replay-gws", "1000"],
+                 ["normal", "1000000", "--replay-gws", str(2**24 + 256)],
+                 ["normal", "4"], ["normal", "1000000000000000000"],
+                 ["bogus", "1"], ["sum", "--replay-gws", "100"],
                  ["batch", "-", "--autotune", "0"], ["batch", "-", "--autotune", "5ms"],
                  ["work", "w.txt", "--batch-size", "-1"]):
         out = subprocess.run([cli, *args], input="", capture_output=True, text=True)
         assert out.returncode == 2 and "usage" in out.stderr, (args, out.returncode, out.stderr)
+
+
+def test_cli_batch_checks_positions_before_the_gpu():
+    """`clbellard batch` runs clb_normalize (host-only) on every position before it creates a
+    context: a position outside [5, 999999999999999999] prints its message and the usage (exit
+    status 2), also on a host without a GPU."""
+    import subprocess
+    cli = os.path.join(ROOT, "clbellard_amd", "clbellard")
+    if not os.path.exists(cli):
+        pytest.skip("CLI not built")
+    for stdin in ("1000000\n4\n", "1000000000000000000\n", "0\n"):
+        out = subprocess.run([cli, "batch", "-"], input=stdin, capture_output=True, text=True)
+        assert out.returncode == 2 and "usage" in out.stderr, (stdin, out.stderr)
+        assert "invalid digits" in out.stderr, out.stderr
 
 
 def test_write_work_file_round_trip(tmp_path, oracle):
