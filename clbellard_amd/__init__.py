@@ -101,6 +101,8 @@ _SIGNATURES = {
     "clb_u192_to_dd": ([_P64, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "clb_run_worklist": ([_CTX, ctypes.c_char_p, ctypes.c_char_p, _U64, ctypes.c_char_p,
                           ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "clb_set_worklist_launch": ([_CTX, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32],
+                                ctypes.c_int),
     "clb_work_sum": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)],
                      ctypes.c_int),
     "clb_write_work_file": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int64, _P64, ctypes.c_int,
@@ -355,6 +357,12 @@ class Context:
         _check(lib.clb_run_worklist(self._ctx, os.fsencode(worklist), os.fsencode(work_dir),
                                     batch_size, label, ctypes.byref(ran)))
         return ran.value
+
+    def set_worklist_launch(self, max_jobs: int = 0, max_seconds: float = 60.0,
+                            max_launches: int = 0) -> None:
+        """Work mode: jobs per launch (0 = any), estimated seconds per launch (0 = any) and
+        launches per run_worklist call (0 = until done); files are written launch by launch."""
+        _check(lib.clb_set_worklist_launch(self._ctx, max_jobs, max_seconds, max_launches))
 
     def last_timing(self) -> Tuple[float, float]:
         """(head-term kernel ms, whole call ms) of the last computation, from HIP events."""

@@ -4,10 +4,12 @@
 //   clbellard normal <digits> [--precision 0|1|2] [--device i] [--kernel auto|plain|split]
 //                    [--replay | --replay-gws GWS]
 //   clbellard work <worklist.txt> [--workdir work] [--batch-size B] [--device i]
-//                  [--replay | --replay-gws GWS]
+//                  [--launch-seconds S] [--launch-jobs J] [--replay | --replay-gws GWS]
 //   clbellard sum [--workdir work]
 //   clbellard batch <positions.txt | -> [--precision 0|1|2] [--device i] [--graph]
 //                   [--autotune MS] [--replay | --replay-gws GWS]
+// work: the pending jobs run in launches of at most S estimated seconds (default 60) and J jobs
+// (default: any number), each launch's files written before the next (clb_set_worklist_launch).
 // --replay: the reference kernels' own output bits (clb_set_replay) with the reference's
 // 16777216 work-items per launch; --replay-gws: with GWS work-items per launch.
 // batch: one position per line (file, or standard input for "-"), all of them in one persistent
@@ -29,7 +31,7 @@ static int usage() {
           "  clbellard normal <digits> [--precision 0|1|2] [--device i] [--kernel auto|plain|split]\n"
           "                   [--replay | --replay-gws GWS]\n"
           "  clbellard work <worklist.txt> [--workdir DIR] [--batch-size B] [--device i]\n"
-          "                 [--replay | --replay-gws GWS]\n"
+          "                 [--launch-seconds S] [--launch-jobs J] [--replay | --replay-gws GWS]\n"
           "  clbellard sum [--workdir DIR]\n"
           "  clbellard batch <positions.txt | -> [--precision 0|1|2] [--device i] [--graph]\n"
           "                  [--autotune MS] [--replay | --replay-gws GWS]\n");
@@ -105,7 +107,8 @@ int main(int argc, char** argv) {
   const std::string cmd = argv[1];
   int precision = 1, device = 0, kernel = CLB_KERNEL_AUTO;
   std::string workdir = "work", positional;
-  unsigned long long batch = 0, replay = 0;
+  unsigned long long batch = 0, replay = 0, launch_jobs = 0;
+  double launch_seconds = 60.0;
   bool graph = false;
   double tune_ms = 0.0;
   // Every option is checked here, before any GPU call: a malformed one prints the usage.
@@ -126,6 +129,12 @@ int main(int argc, char** argv) {
                (!strcmp(v, "auto") || !strcmp(v, "plain") || !strcmp(v, "split"))) {
       kernel = !strcmp(v, "plain") ? CLB_KERNEL_PLAIN : !strcmp(v, "split") ? CLB_KERNEL_SPLIT
                                                                             : CLB_KERNEL_AUTO;
+    } else if (a == "--launch-jobs" && parse_u64(v = next(), &u) && u <= 0xffffffffull) {
+      launch_jobs = u;
+    } else if (a == "--launch-seconds" && (v = next())) {
+      char* end = nullptr;
+      launch_seconds = strtod(v, &end);
+      if (end == v || *end != 0 || !(launch_seconds >= 0.0 && launch_seconds <= 1e9)) return usage();
     } else if (a == "--graph") {
       graph = true;
     } else if (a == "--autotune" && (v = next())) {
@@ -205,8 +214,9 @@ int main(int argc, char** argv) {
     }
   } else if (!rc && cmd == "work") {
     int ran = 0;
-    rc = clb_run_worklist(ctx, positional.c_str(), workdir.c_str(), batch, nullptr, &ran);
-    if (!rc) printf("%d job file(s) written to %s\n", ran, workdir.c_str());
+    rc = clb_set_worklist_launch(ctx, (uint32_t)launch_jobs, launch_seconds, 0);
+    if (!rc) rc = clb_run_worklist(ctx, positional.c_str(), workdir.c_str(), batch, nullptr, &ran);
+    printf("%d job file(s) written to %s\n", ran, workdir.c_str());  // also after a failed launch
   }
   clb_ctx_destroy(ctx);
   return rc ? die(rc) : 0;

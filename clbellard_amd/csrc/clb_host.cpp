@@ -140,6 +140,13 @@ struct clb_ctx {
   bool use_graph = false;
   // Reference replay: 0, or the work-items per replayed launch of the reference's kernels.
   uint64_t replay_gws = 0;
+  // Work mode: bounds of one launch of a worklist's jobs (0 = no bound), and how many launches one
+  // clb_run_worklist call may make (0 = until the list is done).  Files are written per launch.
+  uint32_t wl_max_jobs = 0;
+  double wl_max_seconds = 60.0;
+  uint32_t wl_max_launches = 0;
+  // Estimated modular squares per second of the term kernels, refined by every worklist launch.
+  double wl_rate = 5e12;
   int replay_resident[5] = {-1, -1, -1, -1, -1};  // resident k_replay workgroups per CU, per kind
   // 106-bit-mode replay: the call's tables and results, the jobs' group accumulators, and the
   // context's own bigSumMem (65536 double-doubles) for BatchCalc / AddAns at the operator level.
@@ -838,6 +845,30 @@ int u192_to_dd(const uint64_t w[3], double dd[2]) {
 
 // ------------------------------------------------------------------ work mode helpers
 
+// Sum of bitlen(10 x) over x in [a, b] (a >= 1): the exponent lengths 10 (d - k) of the head
+// terms k in [d - b, d - a], i.e. the modular squares the term kernels spend on them.
+double sum_bitlen_10x(uint64_t a, uint64_t b) {
+  if (a > b) return 0.0;
+  double total = 0.0;
+  for (int bl = 1; bl <= 68; ++bl) {
+    const unsigned __int128 lo_v = ((unsigned __int128)1 << (bl - 1)) + 9;  // ceil(2^(bl-1) / 10)
+    const unsigned __int128 hi_v = ((unsigned __int128)1 << bl) - 1;
+    const unsigned __int128 lo = lo_v / 10 > a ? lo_v / 10 : a;
+    const unsigned __int128 hi = hi_v / 10 < b ? hi_v / 10 : b;
+    if (hi >= lo) total += (double)bl * (double)(uint64_t)(hi - lo + 1);
+  }
+  return total;
+}
+
+// Cost of one BatchCalc job in modular squares: bitlen(10 (d - k)) squares plus about three
+// squares' worth of quotient and bookkeeping per head term, and the tail's twenty terms.
+double job_squares(uint64_t d, uint64_t k_begin, uint64_t k_end) {
+  const uint64_t hi = k_end < d ? k_end : d;
+  double c = 60.0;
+  if (k_begin < hi) c += sum_bitlen_10x(d - hi + 1, d - k_begin) + 3.0 * (double)(hi - k_begin);
+  return c;
+}
+
 bool file_exists(const std::string& p) {
   struct stat st;
   return stat(p.c_str(), &st) == 0;
@@ -1484,23 +1515,59 @@ int clb_run_worklist(clb_ctx* ctx, const char* worklist_path, const char* work_d
     formulas.push_back(j);
   }
   if (todo.empty()) return CLB_OK;
-  std::vector<uint64_t> part(3 * todo.size());
-  const int saved_precision = ctx->precision;
-  ctx->precision = precision;
-  int rc = clb_run_jobs(ctx, todo.data(), todo.size(), part.data());
-  ctx->precision = saved_precision;
-  if (rc) return rc;
-  const long long ms = (long long)std::llround(ctx->total_ms);
   const std::string label =
       device_label ? std::string(device_label)
                    : std::to_string(ctx->device) + "\t" + ctx->device_name;
-  for (size_t i = 0; i < todo.size(); ++i) {
-    const int64_t b = (int64_t)(todo[i].k_begin / batch);
-    if ((rc = write_work_file(work_dir, formulas[i], b, &part[3 * i], out_format(ctx, precision),
-                              label, ms)))
-      return rc;
-    if (jobs_run) ++*jobs_run;
+  // The reference runs and saves one job at a time (:309-333), so a crash loses one job.  Here
+  // consecutive jobs share a persistent launch up to the context's bounds (an estimated duration,
+  // a job count), and each launch's files are written before the next launch starts; each file's
+  // third line is its job's share of the launch time, by estimated cost.
+  std::vector<double> cost(todo.size());
+  for (size_t i = 0; i < todo.size(); ++i)
+    cost[i] = job_squares(d, todo[i].k_begin, todo[i].k_end);
+  std::vector<uint64_t> part;
+  uint32_t launches = 0;
+  for (size_t first = 0; first < todo.size();) {
+    if (ctx->wl_max_launches && launches == ctx->wl_max_launches) break;
+    size_t end = first + 1;
+    double est = cost[first];
+    while (end < todo.size() && (!ctx->wl_max_jobs || end - first < ctx->wl_max_jobs) &&
+           (ctx->wl_max_seconds <= 0.0 || (est + cost[end]) / ctx->wl_rate <= ctx->wl_max_seconds)) {
+      est += cost[end];
+      ++end;
+    }
+    part.assign(3 * (end - first), 0);
+    const int saved_precision = ctx->precision;
+    ctx->precision = precision;
+    int rc = clb_run_jobs(ctx, todo.data() + first, end - first, part.data());
+    ctx->precision = saved_precision;
+    if (rc) return rc;
+    ++launches;
+    if (ctx->head_ms > 0.f) {  // refine the rate estimate for the next launch's bound
+      const double r = est / (ctx->head_ms * 1e-3);
+      if (r > 1e9) ctx->wl_rate = r;
+    }
+    for (size_t i = first; i < end; ++i) {
+      const int64_t b = (int64_t)(todo[i].k_begin / batch);
+      const long long ms = (long long)std::llround(ctx->total_ms * cost[i] / est);
+      if ((rc = write_work_file(work_dir, formulas[i], b, &part[3 * (i - first)],
+                                out_format(ctx, precision), label, ms)))
+        return rc;
+      if (jobs_run) ++*jobs_run;
+    }
+    first = end;
   }
+  return CLB_OK;
+}
+
+int clb_set_worklist_launch(clb_ctx* ctx, uint32_t max_jobs, double max_seconds,
+                            uint32_t max_launches) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!(max_seconds >= 0.0) || max_seconds > 1e9)
+    return fail(CLB_EINVAL, "max_seconds must be in [0, 1e9]");
+  ctx->wl_max_jobs = max_jobs;
+  ctx->wl_max_seconds = max_seconds;
+  ctx->wl_max_launches = max_launches;
   return CLB_OK;
 }
 

@@ -176,11 +176,24 @@ int clb_u192_to_dd(const uint64_t w[3], double dd[2]);
 
 /* Work mode (Bellard_192.hsp:88-340, :514-533).  Reads the worklist (digits=, precision=, j_b
  * lines), runs every job whose range starts at or below d and whose file
- * <work_dir>/<j>/<j>_<b>.txt does not exist yet, and writes each file.  batch_size 0 selects the
- * reference's 2^37 terms per batch.  device_label is written on the file's second line.
- * *jobs_run receives the number of files written (may be NULL). */
+ * <work_dir>/<j>/<j>_<b>.txt does not exist yet, in launches bounded by clb_set_worklist_launch,
+ * and writes each launch's files before the next launch starts.  batch_size 0 selects the
+ * reference's 2^37 terms per batch.  device_label is written on the file's second line (NULL:
+ * "<device>\t<device name>").  *jobs_run receives the number of files written, also when a later
+ * launch fails (may be NULL). */
 int clb_run_worklist(clb_ctx *ctx, const char *worklist_path, const char *work_dir,
                      uint64_t batch_size, const char *device_label, int *jobs_run);
+/* Work-mode launch bounds (the reference launches and saves one job at a time,
+ * Bellard_192.hsp:309-333; here consecutive pending jobs share one persistent launch).  A launch
+ * takes jobs in worklist order while their estimated time stays within max_seconds (0: no time
+ * bound; default 60) and their count within max_jobs (0: no count bound, the default); it holds at
+ * least one job.  Each launch's files are written before the next launch starts, so an
+ * interrupted run loses at most one launch, and a file's third line is its job's share of its
+ * launch's time (by estimated cost).  max_launches > 0 makes clb_run_worklist return after that
+ * many launches (the files written so far stay; a later call resumes), 0 (default) runs the whole
+ * list.  CLB_EINVAL for a negative or non-finite max_seconds. */
+int clb_set_worklist_launch(clb_ctx *ctx, uint32_t max_jobs, double max_seconds,
+                            uint32_t max_launches);
 /* *chktxtstr + *workStart2 (Bellard_192.hsp:176-247): parse a worklist without running it.
  * digits, precision: the values of lines 0 and 1 (digits clamped to [0, 999999999999999999]);
  * up to cap jobs (formula, batch index) from the following lines into jobs[2*i], jobs[2*i+1];

@@ -390,3 +390,40 @@ def dd_error_ulps(hi: float, lo: float, exact: int) -> int:
 def u192_error_ulps(w, exact: int) -> int:
     diff = (w2i(w) - exact) % (1 << 192)
     return min(diff, (1 << 192) - diff)
+
+
+# ----------------------------------------------------------------- seeded synthetic workloads
+
+def splitmix64(seed: int):
+    """SplitMix64 (Steele, Lea, Flood 2014): the explicit generator of every seeded workload."""
+    x = seed & 0xFFFFFFFFFFFFFFFF
+    while True:
+        x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        yield z ^ (z >> 31)
+
+
+def batched_positions(count: int = 4096, seed: int = 0):
+    """Digit positions log-uniform in [10^6, 10^11], multiples of 5: u = (x >> 11) 2^-53,
+    N = 5 round(10^(6 + 5u) / 5)."""
+    rng = splitmix64(seed)
+    out = []
+    for _ in range(count):
+        u = (next(rng) >> 11) * 2.0**-53
+        out.append(5 * round(10 ** (6 + 5 * u) / 5))
+    return out
+
+
+def uneven_jobs(d: int, count: int = 64, seed: int = 1):
+    """Term-range cut points of `count` jobs whose lengths are drawn uniformly from
+    [10^8, 10^11] and rescaled so that the jobs tile [0, d) exactly: c_i = round(d sum_{<=i} L /
+    sum L).  Returns the count + 1 cut points (0 first, d last)."""
+    rng = splitmix64(seed)
+    lens = [10**8 + next(rng) % (10**11 - 10**8 + 1) for _ in range(count)]
+    total, acc, cuts = sum(lens), 0, [0]
+    for ln in lens:
+        acc += ln
+        cuts.append((d * acc + total // 2) // total)
+    return cuts

@@ -236,6 +236,48 @@ def test_worklist_end_to_end(tmp_path, precision):
         assert line0.endswith("0" * 20)
 
 
+def test_worklist_writes_files_launch_by_launch(tmp_path):
+    """Work mode keeps the reference's per-job checkpoint (Bellard_192.hsp:307-333) at launch
+    granularity: with at most 5 jobs per launch and one launch per call, each call writes exactly
+    the next 5 pending files of the list, a rerun skips what exists, and the finished files sum to
+    the normal-mode digits.  A time bound below any job's cost gives one job per launch.  Each
+    file's third line is its job's share of its launch's time."""
+    n = 1000000
+    _, d = py_normalize(n)
+    batch = 50000
+    nb = d // batch + 1
+    order = [(j, b) for j in range(7) for b in range(nb)]
+    wl = os.path.join(str(tmp_path), "worklist.txt")
+    _worklist(wl, n, 1, [f"{j}_{b}" for j, b in order])
+    work = os.path.join(str(tmp_path), "work")
+
+    def done():
+        return [(j, b) for j, b in order
+                if os.path.exists(os.path.join(work, str(j), f"{j}_{b}.txt"))]
+
+    with clb.Context(0, clb.PREC_U192) as c:
+        c.set_worklist_launch(max_jobs=5, max_seconds=0, max_launches=1)
+        for call in range(3):
+            note(f"worklist call {call}: one launch of at most 5 jobs")
+            assert c.run_worklist(wl, work, batch_size=batch) == 5
+            assert done() == order[:5 * (call + 1)]
+        c.set_worklist_launch(max_jobs=0, max_seconds=1e-9, max_launches=4)
+        assert c.run_worklist(wl, work, batch_size=batch) == 4  # one job per launch
+        assert done() == order[:19]
+        c.set_worklist_launch()  # defaults: 60 s launches, until the list is done
+        assert c.run_worklist(wl, work, batch_size=batch) == len(order) - 19
+        assert c.run_worklist(wl, work, batch_size=batch) == 0
+        hexs = c.normal(n)[1]
+        with pytest.raises(clb.ClbError):
+            c.set_worklist_launch(max_seconds=-1.0)
+    total, counts = clb.work_sum(work)
+    assert counts == [nb] * 7 and total == hexs
+    for j, b in order:
+        body = open(os.path.join(work, str(j), f"{j}_{b}.txt"), "rb").read().decode()
+        ms = body.split("\r\n")[2]
+        assert ms.endswith("ms") and int(ms[:-2]) >= 0
+
+
 def test_graph_mode_replays_match_eager(ctx, oracle):
     """Graph mode (clb_set_graph): every replay of a captured sequence, with eager work on another
     context and synchronisations in between, equals the eager result; calls of other sizes reuse
