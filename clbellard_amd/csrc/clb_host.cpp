@@ -228,6 +228,54 @@ int graph_for(clb_ctx* ctx, int kind, int tpl, int grid, hipGraphExec_t* exec) {
   return CLB_OK;
 }
 
+// Launch plan of an exact-mode call.
+struct LaunchPlan {
+  int tpl = 0;                // index into kTplChoices
+  uint64_t grid = 0;          // persistent workgroups
+  uint64_t nwaves = 0;
+  uint64_t flat = 0;          // tiles of the call's head jobs
+  uint64_t static_tiles = 0;  // tiles per wave of the static phase (0: none)
+  uint64_t chunk = 1;         // tiles per queue grab of the dynamic phase
+  uint32_t replicas = 1;      // copies per accumulator slot
+};
+
+// Terms per lane and grid: the longest tiles that still give every resident wave
+// ctx->min_tiles_per_wave tiles, or by default one tile to every 8th wave (only the smallest
+// launches, e.g. N = 10^5, take shorter tiles: 4 terms per lane measured 11% faster there).
+// flat_of(tile_k) gives the call's tile count for a tile length.  Then the tile schedule: up to 64
+// tiles per wave, each wave takes one static range of consecutive tiles (no atomics; atomics on
+// one address serialise at ~35 ns each); larger launches take all tiles from the queue in chunks,
+// at least 8 per wave and at most 256 tiles (~15 ms of one wave's work at N = 10^12, the
+// granularity of the final balance).  And the copies per accumulator slot: enough that the waves'
+// flushes spread over about nwaves / 16 addresses per limb (a power of 2, at most 64), so short
+// launches do not queue on one slot.
+template <class FlatOf>
+void plan_launch(clb_ctx* ctx, int kind, size_t nslots, FlatOf flat_of, LaunchPlan* p) {
+  int tpl = ctx->tpl_index;
+  for (int t = (tpl >= 0 ? tpl : 0); t < kNumTpl; ++t) {
+    p->grid = (uint64_t)ctx->num_cus * (uint64_t)blocks_per_cu(ctx, kind, t);
+    p->flat = flat_of(tile_k(kTplChoices[t]));
+    tpl = t;
+    const uint64_t waves = p->grid * (uint64_t)(kBlock / kWave);
+    const uint64_t need =
+        ctx->min_tiles_per_wave > 0 ? (uint64_t)ctx->min_tiles_per_wave * waves : waves / 8;
+    if (ctx->tpl_index >= 0 || p->flat >= need) break;
+  }
+  p->tpl = tpl;
+  p->nwaves = p->grid * (uint64_t)(kBlock / kWave);
+  const uint64_t nwaves = p->nwaves, flat = p->flat;
+  p->static_tiles = 0;
+  p->chunk = 1;
+  if (flat <= 64 * nwaves) {
+    p->static_tiles = (flat + nwaves - 1) / nwaves;
+  } else {
+    const uint64_t per = flat / (8 * nwaves);
+    p->chunk = per < 1 ? 1 : per > 256 ? 256 : per;
+  }
+  p->replicas = 1;
+  while (p->replicas < 64 && (uint64_t)p->replicas * 2 * nslots * 16 <= nwaves) p->replicas *= 2;
+}
+
 // Device-side copies a call leaves behind (clb_normal keeps the reference's Allloop state).
 struct Mirrors {
   uint64_t* total = nullptr;  // receives group 0's result
@@ -495,25 +543,17 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
   if (nhead && (rc = choose_kind(ctx->kernel, max_m, &kind))) return rc;
   if (nhead > 0x7fffffff || tails.size() > 0x7fffffff || groups.size() > 0x7fffffff)
     return fail(CLB_EINVAL, "too many jobs");
-  // Terms per lane and grid: the longest tiles that still give every resident wave
-  // ctx->min_tiles_per_wave tiles, or by default one tile to every 8th wave (only the smallest
-  // launches, e.g. N = 10^5, take shorter tiles: 4 terms per lane measured 11% faster there).
-  int tpl = ctx->tpl_index;
-  uint64_t grid = 0, flat = 0;
-  for (int t = (tpl >= 0 ? tpl : 0); t < kNumTpl; ++t) {
-    grid = (uint64_t)ctx->num_cus * (uint64_t)blocks_per_cu(ctx, kind, t);
-    const uint64_t tk = tile_k(kTplChoices[t]);
-    flat = 0;
+  LaunchPlan plan;
+  plan_launch(ctx, kind, nslots, [&](uint64_t tk) {
+    uint64_t f = 0;
     for (const HostJob& h : head_jobs)
-      if (h.k0 < h.k1) flat += (h.k1 + tk - 1) / tk - h.k0 / tk;
-    tpl = t;
-    const uint64_t waves = grid * (uint64_t)(kBlock / kWave);
-    const uint64_t need =
-        ctx->min_tiles_per_wave > 0 ? (uint64_t)ctx->min_tiles_per_wave * waves : waves / 8;
-    if (ctx->tpl_index >= 0 || flat >= need) break;
-  }
+      if (h.k0 < h.k1) f += (h.k1 + tk - 1) / tk - h.k0 / tk;
+    return f;
+  }, &plan);
+  const int tpl = plan.tpl;
+  const uint64_t grid = plan.grid;
+  uint64_t flat = 0;
   const uint64_t tk = tile_k(kTplChoices[tpl]);
-  flat = 0;
   for (const HostJob& h : head_jobs) {
     if (h.k0 >= h.k1) continue;
     JobDev j{};
@@ -529,11 +569,7 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
     flat += (h.k1 + tk - 1) / tk - j.tile_first;
     jobs.push_back(j);
   }
-  const uint64_t nwaves = grid * (uint64_t)(kBlock / kWave);
-  // Copies per accumulator slot: enough that the waves' flushes spread over about nwaves / 16
-  // addresses per limb (a power of 2, at most 64), so short launches do not queue on one slot.
-  uint32_t replicas = 1;
-  while (replicas < 64 && (uint64_t)replicas * 2 * nslots * 16 <= nwaves) replicas *= 2;
+  const uint32_t replicas = plan.replicas;
   // The block: header, jobs, tails, groups, shifts, accumulator (copied in as zeros), results
   // (group results, then slot values), each 64-byte aligned.
   auto align64 = [](size_t x) { return (x + 63) & ~(size_t)63; };
@@ -549,17 +585,7 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
   const size_t bytes = off_out + out_words * sizeof(uint64_t);
   if (bytes > 0xffffffffull) return fail(CLB_EINVAL, "too many jobs for one call");
   if ((rc = ctx->block.reserve(bytes)) || (rc = ctx->block_host.reserve(bytes))) return rc;
-  // Tile schedule.  Up to 64 tiles per wave: each wave takes one static range of consecutive
-  // tiles (no atomics; atomics on one address serialise at ~35 ns each).  Larger launches: all
-  // tiles from the queue in chunks, at least 8 per wave and at most 256 tiles (~15 ms of one
-  // wave's work at N = 10^12, the granularity of the final balance).
-  uint64_t static_tiles = 0, chunk = 1;
-  if (flat <= 64 * nwaves) {
-    static_tiles = (flat + nwaves - 1) / nwaves;
-  } else {
-    const uint64_t per = flat / (8 * nwaves);
-    chunk = per < 1 ? 1 : per > 256 ? 256 : per;
-  }
+  const uint64_t static_tiles = plan.static_tiles, chunk = plan.chunk;
   uint8_t* dev = ctx->block.p;
   LaunchDev h{};
   h.counter = 0;
