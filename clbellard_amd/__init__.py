@@ -29,13 +29,14 @@ __all__ = [
     "u192_to_int", "int_to_u192",
     "OK", "EINVAL", "EDEVICE", "ERANGE", "EIO", "ENOMEM", "EFORMAT",
     "PREC_DD106", "PREC_U192", "PREC_U192_BETA", "KERNEL_AUTO", "KERNEL_PLAIN", "KERNEL_SPLIT",
-    "NUMER", "NUMESIGN", "DEN0", "DEN1", "REF_GLOBAL_WORK_SIZE",
+    "NUMER", "NUMESIGN", "DEN0", "DEN1", "REF_GLOBAL_WORK_SIZE", "NOALLOC_MAX_JOBS",
 ]
 
 OK, EINVAL, EDEVICE, ERANGE, EIO, ENOMEM, EFORMAT = 0, -1, -2, -3, -4, -5, -6
 PREC_DD106, PREC_U192, PREC_U192_BETA = 0, 1, 2
 KERNEL_AUTO, KERNEL_PLAIN, KERNEL_SPLIT = 0, 1, 2
 REF_GLOBAL_WORK_SIZE = 65536 * 256  # the reference's work-items per launch, Bellard_192.hsp:5-7
+NOALLOC_MAX_JOBS = 4096  # CLB_NOALLOC_MAX_JOBS
 
 # KernelFuncs.as:9-12
 NUMER = (32, 1, 256, 64, 4, 4, 1)
@@ -96,6 +97,7 @@ _SIGNATURES = {
     "clb_normal": ([_CTX, _U64, _P64, ctypes.c_char_p, _P64], ctypes.c_int),
     "clb_batch_positions": ([_CTX, _P64, ctypes.c_size_t, _P64], ctypes.c_int),
     "clb_run_jobs": ([_CTX, ctypes.POINTER(_Job), ctypes.c_size_t, _P64], ctypes.c_int),
+    "clb_run_jobs_noalloc": ([_CTX, ctypes.POINTER(_Job), ctypes.c_size_t, _P64], ctypes.c_int),
     "clb_hex48": ([_P64, ctypes.c_char_p], ctypes.c_int),
     "clb_hex_dd": ([ctypes.POINTER(ctypes.c_double), ctypes.c_char_p], ctypes.c_int),
     "clb_u192_to_dd": ([_P64, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
@@ -348,6 +350,20 @@ class Context:
             arr[i].d, arr[i].k_begin, arr[i].k_end, arr[i].formula = d, kb, ke, j
         out = (ctypes.c_uint64 * (3 * n))()
         _check(lib.clb_run_jobs(self._ctx, arr, n, out))
+        return [tuple(out[3 * i:3 * i + 3]) for i in range(n)]
+
+    def run_jobs_noalloc(self, jobs: Sequence[Tuple[int, int, int, int]]
+                         ) -> List[Tuple[int, int, int]]:
+        """run_jobs through clb_run_jobs_noalloc: no allocation in the call (at most
+        NOALLOC_MAX_JOBS jobs)."""
+        n = len(jobs)
+        if n == 0:
+            return []
+        arr = (_Job * n)()
+        for i, (d, kb, ke, j) in enumerate(jobs):
+            arr[i].d, arr[i].k_begin, arr[i].k_end, arr[i].formula = d, kb, ke, j
+        out = (ctypes.c_uint64 * (3 * n))()
+        _check(lib.clb_run_jobs_noalloc(self._ctx, arr, n, out))
         return [tuple(out[3 * i:3 * i + 3]) for i in range(n)]
 
     def run_worklist(self, worklist: str, work_dir: str, batch_size: int = 0,

@@ -159,6 +159,7 @@ struct clb_ctx {
     hipGraphExec_t exec;
   };
   std::vector<Graph> graphs;
+  hipError_t noalloc_err = hipSuccess;  // set-up result of this device's no-allocation area
 };
 
 namespace {
@@ -1042,6 +1043,9 @@ int clb_ctx_create(int device, int precision, clb_ctx** out) {
            prop.name[0] ? prop.name : prop.gcnArchName);
   if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess)
     return cleanup(fail(CLB_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e)));
+  // The no-allocation entry point's device array and staging, once per device (the device is
+  // current); a failure only disables that entry point, which then reports it.
+  ctx->noalloc_err = noalloc_setup(device);
   for (auto& ev : ctx->ev)
     if ((e = hipEventCreate(&ev)) != hipSuccess)
       return cleanup(fail(CLB_EDEVICE, "hipEventCreate: %s", hipGetErrorString(e)));
@@ -1427,6 +1431,136 @@ int clb_run_jobs(clb_ctx* ctx, const clb_job* jobs_in, size_t count, uint64_t* p
     groups[i] = {(uint32_t)i, 1};
   }
   return run(ctx, jobs, shift, groups, partial, nullptr, nullptr);
+}
+
+// clb_run_jobs without allocating: the call's block is built in the device's pinned staging
+// buffer and copied into its file-scope device array (clb_noalloc.hip), then the same kernels run
+// on it.  No heap, no device or pinned allocation, no graph (direct launches).
+int clb_run_jobs_noalloc(clb_ctx* ctx, const clb_job* jobs_in, size_t count, uint64_t* partial) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (count == 0) return CLB_OK;
+  if (!jobs_in || !partial) return fail(CLB_EINVAL, "null array");
+  if (count > kNoAllocMaxJobs)
+    return fail(CLB_EINVAL, "%zu jobs: the no-allocation entry point takes at most %u per call",
+                count, kNoAllocMaxJobs);
+  if (ctx->replay_gws)
+    return fail(CLB_EINVAL, "the no-allocation entry point does not replay the reference");
+  const auto t_call = std::chrono::steady_clock::now();
+  uint8_t *dev = nullptr, *stage = nullptr;
+  std::mutex* call_lock = nullptr;
+  if (ctx->noalloc_err != hipSuccess || !noalloc_area(ctx->device, &dev, &stage, &call_lock))
+    return fail(CLB_EDEVICE, "no-allocation area of device %d not set up: %s", ctx->device,
+                hipGetErrorString(ctx->noalloc_err));
+  if (int rc = set_device(ctx)) return rc;
+  auto job_of = [&](size_t i) {
+    const int j = jobs_in[i].formula;
+    return batch_job(jobs_in[i].d, jobs_in[i].k_begin, jobs_in[i].k_end, kDen0[j], kDen1[j],
+                     (uint32_t)kNumeSign[j], (uint32_t)i);
+  };
+  uint64_t max_m = 1;
+  size_t nhead = 0, ntails = 0;
+  for (size_t i = 0; i < count; ++i) {
+    const clb_job& jb = jobs_in[i];
+    if (jb.formula < 0 || jb.formula > 6)
+      return fail(CLB_EINVAL, "job %zu: formula %d", i, jb.formula);
+    if (jb.d == 0 || jb.d > kMaxDigits) return fail(CLB_EINVAL, "job %zu: bad d", i);
+    const HostJob h = job_of(i);
+    if (h.tail) {
+      if (h.d + kTailTerms > ((1ull << 63) - h.den1) / h.den0)
+        return fail(CLB_ERANGE, "tail modulus of d = %llu exceeds 2^63", (unsigned long long)h.d);
+      ++ntails;
+    }
+    if (h.k0 >= h.k1) continue;
+    const uint64_t m = h.den0 * (h.k1 - 1) + h.den1;
+    if (m > max_m) max_m = m;
+    ++nhead;
+  }
+  int kind = kSplitW3;
+  if (nhead)
+    if (int rc = choose_kind(ctx->kernel, max_m, &kind)) return rc;
+  LaunchPlan plan;
+  plan_launch(ctx, kind, count, [&](uint64_t tk) {
+    uint64_t f = 0;
+    for (size_t i = 0; i < count; ++i) {
+      const HostJob h = job_of(i);
+      if (h.k0 < h.k1) f += (h.k1 + tk - 1) / tk - h.k0 / tk;
+    }
+    return f;
+  }, &plan);
+  auto align64 = [](size_t x) { return (x + 63) & ~(size_t)63; };
+  const size_t off_jobs = align64(sizeof(LaunchDev));
+  const size_t off_tails = off_jobs + align64(nhead * sizeof(JobDev));
+  const size_t off_groups = off_tails + align64(ntails * sizeof(TailDev));
+  const size_t off_shifts = off_groups + align64(count * sizeof(GroupDev));
+  const size_t off_acc = off_shifts + align64(count);
+  const size_t off_out = off_acc + count * plan.replicas * kAccWords * sizeof(uint64_t);
+  const size_t bytes = off_out + 3 * count * sizeof(uint64_t);
+  if (bytes > kNoAllocBytes)
+    return fail(CLB_EINVAL, "the call's block (%zu bytes) exceeds the no-allocation area (%zu)",
+                bytes, kNoAllocBytes);
+  std::lock_guard<std::mutex> lock(*call_lock);  // one call at a time uses this device's area
+  LaunchDev h{};
+  h.total_tiles = plan.flat;
+  h.static_tiles = plan.static_tiles;
+  h.dst = (uint64_t)(uintptr_t)(dev + off_out);
+  h.njobs = (uint32_t)nhead;
+  h.chunk = (uint32_t)plan.chunk;
+  h.ntails = (uint32_t)ntails;
+  h.ngroups = (uint32_t)count;
+  h.replicas = plan.replicas;
+  h.nslots = (uint32_t)count;
+  h.off_jobs = (uint32_t)off_jobs;
+  h.off_tails = (uint32_t)off_tails;
+  h.off_groups = (uint32_t)off_groups;
+  h.off_shifts = (uint32_t)off_shifts;
+  h.off_acc = (uint32_t)off_acc;
+  h.off_out = (uint32_t)off_out;
+  h.off_rjobs = h.off_rtails = (uint32_t)off_groups;  // no replay tables
+  memcpy(stage, &h, sizeof h);
+  const uint64_t tk = tile_k(kTplChoices[plan.tpl]);
+  JobDev* jd = reinterpret_cast<JobDev*>(stage + off_jobs);
+  TailDev* td = reinterpret_cast<TailDev*>(stage + off_tails);
+  GroupDev* gd = reinterpret_cast<GroupDev*>(stage + off_groups);
+  uint64_t flat = 0;
+  for (size_t i = 0; i < count; ++i) {
+    const HostJob hj = job_of(i);
+    gd[i] = {(uint32_t)i, 1};
+    stage[off_shifts + i] = kShift[jobs_in[i].formula];
+    if (hj.tail) *td++ = {hj.d, hj.den0, hj.den1, hj.numesign, hj.slot};
+    if (hj.k0 >= hj.k1) continue;
+    JobDev j{};
+    j.d = hj.d;
+    j.k0 = hj.k0;
+    j.k1 = hj.k1;
+    j.tile_first = hj.k0 / tk;
+    j.flat_begin = flat;
+    j.den0 = hj.den0;
+    j.den1 = hj.den1;
+    j.numesign = hj.numesign;
+    j.slot = hj.slot;
+    flat += (hj.k1 + tk - 1) / tk - j.tile_first;
+    *jd++ = j;
+  }
+  memset(stage + off_acc, 0, off_out - off_acc);  // the accumulator starts at zero every call
+  LaunchDev* hdr = reinterpret_cast<LaunchDev*>(dev);
+  hipStream_t s = ctx->stream;
+  HIP_TRY(hipMemcpyAsync(dev, stage, off_out, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipEventRecord(ctx->ev[0], s));
+  HIP_TRY(launch_head(kind, plan.tpl, hdr, (int)plan.grid, s));
+  HIP_TRY(hipEventRecord(ctx->ev[1], s));
+  HIP_TRY(launch_combine(hdr, s));
+  HIP_TRY(hipMemcpyAsync(stage + off_out, dev + off_out, 3 * count * sizeof(uint64_t),
+                         hipMemcpyDeviceToHost, s));
+  hipError_t e2 = hipStreamSynchronize(s);
+  float head = 0.f;
+  if (e2 == hipSuccess) e2 = hipEventElapsedTime(&head, ctx->ev[0], ctx->ev[1]);
+  if (e2 != hipSuccess) return fail(CLB_EDEVICE, "term launch failed: %s", hipGetErrorString(e2));
+  memcpy(partial, stage + off_out, 3 * count * sizeof(uint64_t));
+  ctx->head_ms = head;
+  ctx->total_ms = (float)std::chrono::duration<double, std::milli>(
+                      std::chrono::steady_clock::now() - t_call)
+                      .count();
+  return CLB_OK;
 }
 
 int clb_hex48(const uint64_t w[3], char hex[49]) {

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 namespace clb {
 
 constexpr int kWave = 64;
@@ -162,5 +164,16 @@ hipError_t launch_replay106_sum7(const double* parts, const uint32_t* first, con
                                  const uint8_t* shift, double* ans, double* slot_val,
                                  uint32_t ngroups, hipStream_t stream);
 constexpr uint64_t kRefGlobalWorkSize = 65536ull * 256ull;  // Bellard_192.hsp:5-7
+
+// The no-allocation entry point (clb_noalloc.hip): a fixed device array per GPU and a pinned
+// staging buffer, each kNoAllocBytes, set up once per device (with that device current).  A call
+// of at most kNoAllocMaxJobs jobs always fits: its block is at most 512 B of header, 64 B per job,
+// 32 B per tail, 8 + 1 + 24 B per result and 64 B per accumulator copy, and the copies number at
+// most max(jobs, waves / 32).
+constexpr size_t kNoAllocBytes = 1u << 20;
+constexpr uint32_t kNoAllocMaxJobs = 4096;
+hipError_t noalloc_setup(int device);
+// The device's array address, its staging buffer and the lock a call holds; false before setup.
+bool noalloc_area(int device, uint8_t** dev, uint8_t** host, std::mutex** call);
 
 }  // namespace clb

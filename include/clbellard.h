@@ -167,6 +167,20 @@ int clb_batch_positions(clb_ctx *ctx, const uint64_t *n, size_t count, uint64_t 
  * jobs per call (CLB_EINVAL beyond). */
 int clb_run_jobs(clb_ctx *ctx, const clb_job *jobs, size_t count, uint64_t *partial);
 
+/* clb_run_jobs without any allocation (no counterpart in the reference; for callers that cannot
+ * allocate: code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same jobs, same results, bit for bit.  The call's tables, accumulator and results live in a
+ * fixed array of CLB_NOALLOC_BYTES in each GPU's memory (and a pinned host staging buffer of the
+ * same size), set up once per device by the first clb_ctx_create on it and held until the process
+ * ends; the call itself allocates nothing.  At most CLB_NOALLOC_MAX_JOBS jobs per call
+ * (CLB_EINVAL beyond), not in replay mode (CLB_EINVAL), never as a captured graph.  The array
+ * keeps the last call's contents; every call clears what must start at zero.  Calls on contexts
+ * of the same GPU share the array: within one process the library runs them one at a time (a
+ * lock held for the call); unsynchronised access to it from outside the library would race. */
+#define CLB_NOALLOC_BYTES 1048576u
+#define CLB_NOALLOC_MAX_JOBS 4096u
+int clb_run_jobs_noalloc(clb_ctx *ctx, const clb_job *jobs, size_t count, uint64_t *partial);
+
 /* hexadecimal_i32 with the host word swap (Bellard_192.hsp:354-359, hexsademical.as:104-127). */
 int clb_hex48(const uint64_t w[3], char hex[49]);
 /* hexadecimal_dd (hexsademical.as:21-101): double-double -> 28 hex digits + 20 zeros. */
