@@ -17,6 +17,7 @@ Also reported, outside the timed steps:
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -27,6 +28,8 @@ sys.path.insert(0, ROOT)
 METRIC = "Bellard terms/sec (7 modexps each) at hex pos 10^12; frac err vs exact oracle"
 POSITION = 10**12
 PEAK_FP64_TFLOPS = 78.6  # MI355X published FP64 peak (one FMA = 2 flops per lane per cycle)
+# VALU issue: one wave64 instruction per CU per clock = 64 lane-operations; 256 CUs at 2.4 GHz
+PEAK_VALU_LANE_SLOTS_PER_S = 256 * 64 * 2.4e9
 # 32x32->64-bit multiplies (v_mad_u64_u32, v_mul_hi_u32) at full rate: 256 CUs x 64 lanes x 2.4 GHz
 PEAK_INT32_MUL_PER_S = 256 * 64 * 2.4e9
 NUMER_SUM = 32 + 1 + 256 + 64 + 4 + 4 + 1
@@ -68,6 +71,13 @@ def algorithmic_fp64_slots(d):
     return 7 * (6 * squares + 16 * d)
 
 
+def algorithmic_fp64_flops(d):
+    """FP64 flops in those slots (fma = 2, mul and add = 1, rounding = 0): 7 per modular square
+    (mul, fma, mul, rint, fma, add), 20 per term's four digits (two fmas each, and the sums)."""
+    squares = sum_bitlen_10x(d) - d
+    return 7 * (7 * squares + 20 * d)
+
+
 def int32_word_products(d):
     """32x32-bit word products a 2-limb (64-bit) Montgomery engine needs for one normal-mode
     launch: per head term and formula, bitlen(e) - 1 squares and 3 quotient-word steps of
@@ -96,6 +106,110 @@ def load_traffic():
         return t.get("head_kernel_hbm_bytes_per_launch")
     except (OSError, ValueError):
         return None
+
+
+def reference_digits_check(clb, hexs, d):
+    """The exact mode's N = 10^12 output against the reference kernels' own outputs there
+    (tests/golden/ref_positions.json, measured on MI355X): leading hex digits in common and the
+    distance, which must stay within the double-double kernels' bound (2 units of 2^-192 per term
+    and c_j, plus the tail kernel; the 106-bit mode: 2^-104 per addition)."""
+    path = os.path.join(ROOT, "tests", "golden", "ref_positions.json")
+    with open(path) as f:
+        rows = json.load(f)["positions"]
+    ours = int(hexs, 16)
+    out = {}
+    for row in rows:
+        if row["n"] != 10**12:
+            continue
+        p = row["precision"]
+        if p == 0:
+            hi, lo = float.fromhex(row["hi"]), float.fromhex(row["lo"])
+            from fractions import Fraction
+            theirs = (Fraction(hi) + Fraction(lo)) * (1 << 192)
+            diff = abs(theirs - ours)
+            dist = float(diff) if diff else 0.0
+            bound = 7 * (d + 20) << (192 - 104)
+            common = 0
+            ref_hex = clb.hex_dd(hi, lo)
+            ours_hex = clb.hex_dd(*clb.u192_to_dd(clb.int_to_u192(ours)))
+        else:
+            theirs = int(row["hex"], 16)
+            dist = abs(theirs - ours)
+            bound = NUMER_SUM * (2 * (d + 20) + (1 << 16))
+            ref_hex, ours_hex = row["hex"], hexs
+        common = next((i for i, (a, b) in enumerate(zip(ref_hex, ours_hex)) if a != b),
+                      len(ours_hex))
+        import math
+        out[f"precision_{p}"] = {"hex_digits_equal": common,
+                                 "distance_log2": round(math.log2(dist) - 192, 2) if dist else None,
+                                 "bound_log2": round(math.log2(bound) - 192, 2)}
+        if dist > bound or common < (20 if p == 0 else 36):
+            raise SystemExit(f"exact mode {hexs} vs reference precision {p} {ref_hex}: "
+                             f"{common} digits equal, distance beyond the bound")
+    return out
+
+
+def measure_frac_err(clb, orc, ctx, d, starts, span, threads):
+    """Fraction error of each mode against the 256-bit oracle (orc_job256: every term to 2^-256,
+    the tail to k = d + 25) on sub-windows of `span` consecutive k at the seeded window starts,
+    all seven formulas.  Modes: the exact kernels (split and plain, bit-identical), and the replay
+    of the reference's precision 2, 1 and 0 kernels launched with span / 32 work-items, so that
+    every work-item sums 32 terms as in the reference's full launches (Bellard.cl:992-1084,
+    KernelFuncs.as:45-53).  Returns per mode the largest window error, the mean error per term
+    and the whole-position error extrapolated from the sample (the windows are uniform in
+    [0, d)), as log2 of the fraction, and the sampled sums."""
+    import math
+    import struct
+    from fractions import Fraction
+    from clb_testlib import frac_error_256
+    jobs = [(d, st, st + span, j) for j in range(7) for st in starts]
+    t0 = time.perf_counter()
+    exact256 = [orc.job256(dd, k0, k1, j, threads=threads) for dd, k0, k1, j in jobs]
+    cpu_s = time.perf_counter() - t0
+    modes = {}
+    ctx.set_kernel(clb.KERNEL_SPLIT)
+    modes["exact"] = [clb.u192_to_int(w) for w in ctx.run_jobs(jobs)]
+    ctx.set_kernel(clb.KERNEL_PLAIN)
+    plain = [clb.u192_to_int(w) for w in ctx.run_jobs(jobs)]
+    if plain != modes["exact"]:
+        raise SystemExit("plain and split kernels disagree on the error windows")
+    ctx.set_kernel(clb.KERNEL_AUTO)
+    gws = max(256, (span // 32) // 256 * 256)
+    for prec in (clb.PREC_U192_BETA, clb.PREC_U192, clb.PREC_DD106):
+        with clb.Context(0, prec) as rc:
+            rc.set_replay(gws)
+            res = rc.run_jobs(jobs)
+        if prec == clb.PREC_DD106:  # double-double bit patterns (hi, lo) -> 256-bit integers
+            vals = []
+            for w in res:
+                hi, lo = (struct.unpack("<d", struct.pack("<Q", x))[0] for x in w[:2])
+                v = (Fraction(hi) + Fraction(lo)) % 1
+                vals.append(int(v * (1 << 256)))
+            modes["replay_p0"] = [("dd", v) for v in vals]
+        else:
+            modes[f"replay_p{prec}"] = [clb.u192_to_int(w) for w in res]
+    terms = 7 * span * len(starts)
+    out = {}
+    for name, vals in modes.items():
+        errs = []
+        for v, x in zip(vals, exact256):
+            if isinstance(v, tuple):
+                e = (v[1] - x) & ((1 << 256) - 1)
+                e = e - (1 << 256) if e > (1 << 255) else e
+            else:
+                e = frac_error_256(v, x)
+            errs.append(e)
+        worst = max(abs(e) for e in errs)
+        total = sum(errs)
+        full = total * (d / (span * len(starts)))
+
+        def lg(u):
+            return round(math.log2(abs(u)) - 256, 2) if u else None
+        out[name] = {"max_window_log2": lg(worst), "sample_sum_log2": lg(total),
+                     "mean_per_term_units_2^-192": round(total / terms / 2**64, 4),
+                     "full_position_estimate_log2": lg(full)}
+    out["plain"] = "bit-identical to exact"
+    return out, cpu_s, terms
 
 
 def bench_batched(clb, args):
@@ -272,6 +386,8 @@ def main():
     ap.add_argument("--windows", type=int, default=64)
     ap.add_argument("--window-log2", type=int, default=26)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--err-window-log2", type=int, default=18,
+                    help="k per seeded window in the fraction-error measurement")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["normal", "batched", "worklist", "small", "replay"],
                     default="normal",
@@ -346,8 +462,10 @@ def main():
     value = d * len(step_s) / total_s
     head_avg_s = sum(head_ms) / len(head_ms) / 1e3
     slots = algorithmic_fp64_slots(d)
-    achieved = 2.0 * slots / head_avg_s / 1e12  # FMA-equivalent TFLOP/s
+    achieved_slots = slots / head_avg_s  # FP64 VALU lane-slots per second
+    flops = algorithmic_fp64_flops(d)
     int_products = int32_word_products(d)
+    ref_check = reference_digits_check(clb, clb.hex48(result), d) if n_pos == 10**12 else None
 
     # seeded windows: plain vs split, GPU vs oracle
     rng = splitmix64(2)
@@ -368,7 +486,7 @@ def main():
     log(f"windows: split {win_terms / split_win_s:.3e} terms/s, plain "
         f"{win_terms / plain_win_s:.3e} terms/s, bit-identical")
 
-    cpu = None
+    cpu, frac_err = None, None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from clb_testlib import load_oracle, w2i, SHIFT  # the oracle, timed as the baseline
@@ -395,6 +513,15 @@ def main():
                "sample": f"k in [{windows[0]}, {windows[0] + sample}) of N=10^12, all 7 formulas, "
                          f"bit-identical to the GPU sum; {threads} threads on {cpu_model()}"}
         log(f"cpu oracle: {sample / cpu_s:.3e} terms/s on {threads} threads")
+        frac_err, err_cpu_s, err_terms = measure_frac_err(
+            clb, orc, ctx, d, windows, 1 << args.err_window_log2, threads)
+        frac_err["sample"] = (f"{len(windows)} seeded windows x 2^{args.err_window_log2} k x 7 "
+                              f"formulas at N={n_pos} ({err_terms:.3e} term-formulas; the 256-bit "
+                              f"oracle took {err_cpu_s:.1f} s on {threads} threads)")
+        frac_err["bound_full_position_log2"] = \
+            round(-192 + math.log2(NUMER_SUM * (d + 20)), 2)
+        log(f"frac err: " + ", ".join(f"{k} {v['full_position_estimate_log2']}"
+                                      for k, v in frac_err.items() if isinstance(v, dict)))
 
     out = {
         "metric": METRIC,
@@ -414,13 +541,24 @@ def main():
                         "terms + 140 tail terms + combine, one persistent launch",
             "position": n_pos, "d": d, "kernel": "split (fp64 exact, 3-bit windows)",
             "digits": clb.hex48(result),
-            "frac_err_bound_log2": round(-192 + (NUMER_SUM * (d + 20)).bit_length(), 1),
+            "frac_err_bound_log2": round(-192 + math.log2(NUMER_SUM * (d + 20)), 2),
+            "reference_kernels_output": ref_check,
         },
+        "frac_err": frac_err,
         "roofline": {
-            "bound": "mfma", "pipe": "VALU FP64 (no MFMA; FP64 peak applies to both)",
-            "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP64_TFLOPS, "traffic": load_traffic(),
+            "bound": "valu-fp64-issue",
+            "what": "algorithmic FP64 VALU lane-slots per second of the head kernel (6 per modular "
+                    "square for bitlen(e)-1 squares, 16 per term's quotient digits) against the "
+                    "VALU issue peak of one wave64 instruction per CU per clock (256 CUs x 64 "
+                    "lanes x 2.4 GHz); no MFMA is used",
+            "achieved": achieved_slots / 1e12, "peak": PEAK_VALU_LANE_SLOTS_PER_S / 1e12,
+            "unit": "Tslot/s", "frac": achieved_slots / PEAK_VALU_LANE_SLOTS_PER_S,
+            "traffic": load_traffic(),
             "head_kernel_ms": 1e3 * head_avg_s, "algorithmic_fp64_slots": slots,
+            "fp64_flops": {"achieved": flops / head_avg_s / 1e12, "peak": PEAK_FP64_TFLOPS,
+                           "unit": "TFLOP/s",
+                           "frac": flops / head_avg_s / 1e12 / PEAK_FP64_TFLOPS,
+                           "what": "fma = 2, mul and add = 1, rounding = 0"},
         },
         "int_mul_view": {
             "what": "32x32-bit word products/s that a 2-limb Montgomery engine would need for the "

@@ -265,6 +265,121 @@ void orc_sum7_192(uint64_t ans[3], uint64_t part[3], uint64_t numer) {
   orc_add192(ans, part);
 }
 
+/* ---------------------------------------------------------------- 256-bit error oracle */
+
+void orc_term256(uint64_t d, uint64_t k, int64_t numesign, uint64_t den0, uint64_t den1,
+                 uint64_t q[4]) {
+  const uint64_t m = den0 * k + den1;
+  uint64_t r = orc_pow1024_mod(d - k, m);
+  if ((numesign + (int64_t)(k % 2)) % 2 == 1) r = m - r; /* frac(-x) = (m - r) / m */
+  const uint64_t u[5] = {0, 0, 0, 0, r};
+  uint64_t w[5];
+  orc_divnby1(5, u, m, w); /* w[4] = 1 only for r' = m, i.e. t = 0 (dropped mod 2^256) */
+  q[0] = w[3];
+  q[1] = w[2];
+  q[2] = w[1];
+  q[3] = w[0];
+}
+
+void orc_add256(uint64_t acc[4], const uint64_t q[4]) {
+  unsigned __int128 c = 0;
+  for (int i = 3; i >= 0; --i) {
+    c += (unsigned __int128)acc[i] + q[i];
+    acc[i] = (uint64_t)c;
+    c >>= 64;
+  }
+}
+
+void orc_tail256(uint64_t d, int64_t numesign, uint64_t den0, uint64_t den1, uint64_t out[4]) {
+  out[0] = out[1] = out[2] = out[3] = 0;
+  for (int i = 0; i <= 25; ++i) {
+    const uint64_t k = d + (uint64_t)i;
+    const uint64_t m = den0 * k + den1;
+    const int e = 256 - 10 * i;
+    uint64_t u[5] = {0, 0, 0, 0, 0};
+    u[e / 64] = 1ull << (e % 64);
+    uint64_t w[5];
+    orc_divnby1(5, u, m, w);
+    uint64_t q[4] = {w[3], w[2], w[1], w[0]};
+    if ((numesign + (int64_t)(k % 2)) % 2 == 1) /* 2^256 - ceil(x 2^256), x 2^256 not an integer */
+      for (int b = 0; b < 4; ++b) q[b] = ~q[b];
+    orc_add256(out, q);
+  }
+}
+
+struct range256_job {
+  uint64_t d, k0, k1, step;
+  int64_t numesign;
+  uint64_t den0, den1;
+  uint64_t acc[4];
+};
+
+static void *range256_worker(void *arg) {
+  struct range256_job *jb = (struct range256_job *)arg;
+  jb->acc[0] = jb->acc[1] = jb->acc[2] = jb->acc[3] = 0;
+  for (uint64_t k = jb->k0; k < jb->k1; k += jb->step) {
+    uint64_t q[4];
+    orc_term256(jb->d, k, jb->numesign, jb->den0, jb->den1, q);
+    orc_add256(jb->acc, q);
+  }
+  return NULL;
+}
+
+static void shl256(uint64_t v[4], int sh) {
+  if (sh == 0) return;
+  for (int i = 0; i < 3; ++i) v[i] = (v[i] << sh) | (v[i + 1] >> (64 - sh));
+  v[3] <<= sh;
+}
+
+void orc_job256(uint64_t d, uint64_t k0, uint64_t k1, int j, int nthreads, uint64_t out[4]) {
+  const uint64_t hi = k1 < d ? k1 : d;
+  uint64_t offset = k0;
+  out[0] = out[1] = out[2] = out[3] = 0;
+  if (k0 < hi) {
+    int nt = nthreads < 1 ? 1 : nthreads > 256 ? 256 : nthreads;
+    if ((uint64_t)nt > hi - k0) nt = (int)(hi - k0);
+    struct range256_job jobs[256];
+    pthread_t th[256];
+    for (int t = 0; t < nt; ++t) {
+      jobs[t].d = d;
+      jobs[t].k0 = k0 + (uint64_t)t;
+      jobs[t].k1 = hi;
+      jobs[t].step = (uint64_t)nt;
+      jobs[t].numesign = kNumeSign[j];
+      jobs[t].den0 = kDen0[j];
+      jobs[t].den1 = kDen1[j];
+      if (nt == 1 || pthread_create(&th[t], NULL, range256_worker, &jobs[t]) != 0) {
+        range256_worker(&jobs[t]);
+        th[t] = 0;
+      }
+    }
+    for (int t = 0; t < nt; ++t) {
+      if (th[t]) pthread_join(th[t], NULL);
+      orc_add256(out, jobs[t].acc);
+    }
+    offset = hi;
+  }
+  if (offset == d) {
+    uint64_t t[4];
+    orc_tail256(d, kNumeSign[j], kDen0[j], kDen1[j], t);
+    orc_add256(out, t);
+  }
+  shl256(out, kNumer[j] == 1 ? 0 : __builtin_ctzll(kNumer[j]));
+}
+
+int orc_normal256(uint64_t n, int nthreads, uint64_t ans[4]) {
+  uint64_t d;
+  const int rc = orc_normalize(n, NULL, &d, NULL);
+  ans[0] = ans[1] = ans[2] = ans[3] = 0;
+  if (rc) return rc;
+  for (int j = 0; j < 7; ++j) {
+    uint64_t part[4];
+    orc_job256(d, 0, d, j, nthreads, part);
+    orc_add256(ans, part);
+  }
+  return 0;
+}
+
 /* ---------------------------------------------------------------- range sums */
 
 struct range_job {

@@ -27,6 +27,9 @@
  *   orc_range_sum, orc_batch_calc       brute-force big-integer sums; partition invariance
  *   orc_sum7_192                        (c * S) mod 2^192
  *   orc_normal                          hex digits of pi from mpmath (tests/golden/)
+ *   orc_term256, orc_tail256,
+ *   orc_job256, orc_normal256           big-integer floor(r' 2^256 / m); floor(term256 / 2^64) ==
+ *                                       term192; 256 bits of pi from mpmath at 22 positions
  *   orc_hex48                           printf("%016llx") layout
  *   orc_work_sum_add                    big-integer sum mod 16^48
  *   orc_hex_dd                          exact 112-bit values, hand-traced borrow vectors, and
@@ -99,6 +102,28 @@ void orc_batch_calc(uint64_t d, uint64_t k_start, uint64_t k_end, int j, int nth
                     uint64_t out[3]);
 /* Bellard_192.hsp:264-296 (Allloop): frac(16^(N'-1) * pi) as 192 bits. Returns orc_normalize's code. */
 int orc_normal(uint64_t n, int nthreads, uint64_t ans[3]);
+
+/* The error oracle (256-bit).  The exact value of a term is t = frac(+-2^(10(d-k)) / m) in [0, 1);
+ * orc_term256 gives floor(t 2^256) (bigSum order, q[0] most significant) for a head term k < d,
+ * from the same residue r = 1024^(d-k) mod m (orc_pow1024_mod) and a 5-word division.  Not the
+ * reference's arithmetic: the reference truncates every term at 2^-192 (Bellard_div2by1.cl:338-341)
+ * and the tail after 20 terms (Bellard.cl:1164); this carries 64 more bits per term and the tail to
+ * k = d + 25 (the terms after that are below 2^-250 / m), so a sum of n terms here is below the
+ * exact value by less than (n + 1) 2^-256, and the error of any 192-bit result is measured to that
+ * accuracy. */
+void orc_term256(uint64_t d, uint64_t k, int64_t numesign, uint64_t den0, uint64_t den1,
+                 uint64_t q[4]);
+/* 256-bit add mod 2^256 (q[0] most significant). */
+void orc_add256(uint64_t acc[4], const uint64_t q[4]);
+/* Tail terms k = d..d+25 of the formula (numesign, den0, den1), each floor(2^256 frac(+-2^(-10i)/m)). */
+void orc_tail256(uint64_t d, int64_t numesign, uint64_t den0, uint64_t den1, uint64_t out[4]);
+/* Formula j's terms over [k0, min(k1, d)), plus the tail iff the range reaches d (BatchCalc's
+ * rule, KernelFuncs.as:56-59), multiplied by c_j (mod 2^256): the 256-bit counterpart of one
+ * clb_run_jobs result. */
+void orc_job256(uint64_t d, uint64_t k0, uint64_t k1, int j, int nthreads, uint64_t out[4]);
+/* frac(16^(N'-1) pi) to 256 bits (below the exact value by less than 362 (d + 27) 2^-256).
+ * Returns orc_normalize's code. */
+int orc_normal256(uint64_t n, int nthreads, uint64_t ans[4]);
 
 /* Reference replay of the precision-1 and -2 kernels (bit-exact with the reference's own results,
  * double-double rounding included).  One work-item of Sglobal32mtg_192 (Bellard.cl:771-823) or

@@ -59,6 +59,10 @@ class Oracle:
             "orc_hex48": ([p64, ctypes.c_char_p], None),
             "orc_hex_dd": ([ctypes.POINTER(ctypes.c_double), ctypes.c_char_p], None),
             "orc_work_sum_add": ([ctypes.POINTER(ctypes.c_int), ctypes.c_char_p], None),
+            "orc_term256": ([u64, u64, i64, u64, u64, p64], None),
+            "orc_tail256": ([u64, i64, u64, u64, p64], None),
+            "orc_job256": ([u64, u64, u64, c_int, c_int, p64], None),
+            "orc_normal256": ([u64, c_int, p64], c_int),
             "orc_ref_item192": ([u64, u64, u64, u64, u64, ctypes.c_int, U3], None),
             "orc_ref_tail192": ([u64, ctypes.c_int, U3], None),
             "orc_ref_batch192": ([u64, u64, u64, ctypes.c_int, ctypes.c_int, u64, ctypes.c_int,
@@ -107,6 +111,28 @@ class Oracle:
         rc = self.lib.orc_normal(n, threads or self.threads, q)
         assert rc == 0
         return w2i(q)
+
+    # -- the 256-bit error oracle (values as integers: floor(value * 2^256))
+    def term256(self, d, k, numesign, den0, den1):
+        q = U4()
+        self.lib.orc_term256(d, k, numesign, den0, den1, q)
+        return w4i(q)
+
+    def tail256(self, d, numesign, den0, den1):
+        q = U4()
+        self.lib.orc_tail256(d, numesign, den0, den1, q)
+        return w4i(q)
+
+    def job256(self, d, k0, k1, j, threads=None):
+        """c_j * (formula j over [k0, min(k1, d)), plus the tail iff it reaches d) mod 1."""
+        q = U4()
+        self.lib.orc_job256(d, k0, k1, j, threads or self.threads, q)
+        return w4i(q)
+
+    def normal256(self, n, threads=None):
+        q = U4()
+        assert self.lib.orc_normal256(n, threads or self.threads, q) == 0
+        return w4i(q)
 
     def sum7(self, ans, part, numer):
         a, p = U3(*i2w(ans)), U3(*i2w(part))
@@ -167,6 +193,27 @@ def w2i(w) -> int:
     return (int(w[0]) << 128) | (int(w[1]) << 64) | int(w[2])
 
 
+def w4i(w) -> int:
+    return (int(w[0]) << 192) | (int(w[1]) << 128) | (int(w[2]) << 64) | int(w[3])
+
+
+M256 = (1 << 256) - 1
+
+
+def frac_error_256(result192: int, exact256: int) -> int:
+    """Signed error of a 192-bit fraction against a 256-bit one, in units of 2^-256 (the
+    difference taken on the unit circle, in (-2^255, 2^255])."""
+    e = ((result192 << 64) - exact256) & M256
+    return e - (1 << 256) if e > (1 << 255) else e
+
+
+def error_log2(units256: int) -> float:
+    """log2 of |error| as a fraction (units of 2^-256); -inf for 0."""
+    import math
+    a = abs(units256)
+    return -math.inf if a == 0 else math.log2(a) - 256
+
+
 def i2w(v: int):
     v &= M192
     return (v >> 128, (v >> 64) & (2**64 - 1), v & (2**64 - 1))
@@ -213,6 +260,26 @@ def py_tail_term(d: int, i: int, numesign: int, den0: int, den1: int) -> int:
     if (numesign + k) % 2 == 1:
         q = M192 - q  # floor((1 - x) 2^192) for the non-integer x 2^192
     return q
+
+
+def py_term256(d: int, k: int, numesign: int, den0: int, den1: int) -> int:
+    m = den0 * k + den1
+    r = pow(2, 10 * (d - k), m)
+    if (numesign + k) % 2 == 1:
+        r = m - r
+    return ((r << 256) // m) & M256
+
+
+def py_tail256(d: int, j: int) -> int:
+    s = 0
+    for i in range(26):
+        k = d + i
+        m = DEN0[j] * k + DEN1[j]
+        q = (1 << (256 - 10 * i)) // m
+        if (NUMESIGN[j] + k) % 2 == 1:
+            q = M256 - q
+        s += q
+    return s & M256
 
 
 def py_range_sum(d: int, k0: int, k1: int, j: int) -> int:

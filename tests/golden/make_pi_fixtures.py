@@ -5,8 +5,10 @@ Run in the build container only (mpmath is a test-time dependency, never used by
 
 For each position N (1-based hex position after the point, a multiple of 5 as the reference's
 digit normalisation produces) the fixture holds floor(frac(16^(N-1) * pi) * 2^192) as 48 hex
-digits.  The exact value of the reference's 192-bit sum differs from this by less than
-362 * (d + 20) units of 2^-192 (d the loop count), so these digits pin every bit above that.
+digits ("digits"), and floor(frac(16^(N-1) * pi) * 2^256) as 64 hex digits ("digits256", the
+exact value against which the errors of 192-bit results are measured).  The exact value of the
+reference's 192-bit sum differs from this by less than 362 * (d + 20) units of 2^-192 (d the loop
+count), so these digits pin every bit above that.
 
 "published" holds entries of D. H. Bailey's table ("The BBP Algorithm for Pi", 2006) of 14 hex
 digits beginning at position N, as recalled (the table is not available offline).  The 10^6
@@ -41,18 +43,22 @@ def main() -> None:
     scale = mpmath.mpf(2) ** (4 * (top + 64))
     big = int(mpmath.floor(pi * scale))  # floor(pi * 2^(4 (top + 64)))
     total_bits = 4 * (top + 64)
-    out = {}
+    out, out256 = {}, {}
     for n in sorted(POSITIONS):
         assert n % 5 == 0
         # frac(16^(n-1) pi) * 2^192 = bits [4(n-1), 4(n-1) + 192) after the point
         shift = total_bits - 4 * (n - 1) - 192
         v = (big >> shift) & ((1 << 192) - 1)
         out[str(n)] = f"{v:048x}"
+        v256 = (big >> (shift - 64)) & ((1 << 256) - 1)
+        out256[str(n)] = f"{v256:064x}"
     assert out["1000000"].startswith(PUBLISHED["1000000"])
     data = {
         "source": "mpmath %s, pi to %d bits" % (mpmath.__version__, mpmath.mp.prec),
-        "definition": "floor(frac(16^(N-1) * pi) * 2^192), 48 hex digits",
+        "definition": "floor(frac(16^(N-1) * pi) * 2^192), 48 hex digits; digits256: "
+                      "floor(frac(16^(N-1) * pi) * 2^256), 64 hex digits",
         "digits": out,
+        "digits256": out256,
         "published": PUBLISHED,
     }
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pi_hex.json")
