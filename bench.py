@@ -341,6 +341,34 @@ def bench_worklist(clb, args):
     return 0
 
 
+def bench_peaks(clb, args):
+    """Issue rates of the instructions the term kernels are made of, and of the two engines' whole
+    modular squares (clb_issue_rate: 8 independent chains per lane, 8 waves per SIMD on every CU),
+    per CU per clock both at the nominal 2.4 GHz and at the shader clock measured during each
+    run.  The VALU can issue one wave64 instruction per CU per clock, i.e. 64 lane-operations."""
+    rows = {}
+    for op, name in enumerate(clb.PEAK_OPS):
+        r = clb.issue_rate(op, args.peak_ms)
+        cus = r["cus"]
+        rows[name] = {
+            "per_s": r["per_s"], "sclk_ghz": round(r["sclk_ghz"], 3), "ms": round(r["ms"], 1),
+            "per_cu_per_clk_nominal": round(r["per_s"] / (cus * 2.4e9), 2),
+            "per_cu_per_clk_measured": round(r["per_s"] / (cus * r["sclk_ghz"] * 1e9), 2)
+            if r["sclk_ghz"] > 0 else None}
+        log(f"{name:24s} {r['per_s']:.4e}/s  sclk {r['sclk_ghz']:.3f} GHz  "
+            f"{rows[name]['per_cu_per_clk_nominal']:6.2f} per CU per clock at 2.4 GHz, "
+            f"{rows[name]['per_cu_per_clk_measured']} at the measured clock")
+    fma = rows["v_fma_f64"]
+    print(json.dumps({
+        "metric": "VALU issue rate, v_fma_f64 lanes per CU per clock (measured clock)",
+        "value": fma["per_cu_per_clk_measured"], "unit": "lanes/CU/clk", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": fma["ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "issue-rate microbenchmark: 8 independent chains per lane, 8 "
+                               "waves per SIMD, every CU", "rates": rows}}), flush=True)
+    return 0
+
+
 def bench_replay(clb, args):
     """Normal mode at N (default 10^12) replaying the reference's kernels of precision 2, 1 and 0
     (clb_set_replay with the reference's 2^24 work-items per launch): the reference's own output
@@ -389,12 +417,16 @@ def main():
     ap.add_argument("--err-window-log2", type=int, default=18,
                     help="k per seeded window in the fraction-error measurement")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["normal", "batched", "worklist", "small", "replay"],
+    ap.add_argument("--peak-ms", type=float, default=200.0,
+                    help="peaks: target duration of each microbenchmark run")
+    ap.add_argument("--workload", choices=["normal", "batched", "worklist", "small", "replay",
+                                           "peaks"],
                     default="normal",
                     help="normal: the metric (N = 10^12); batched: many positions in one launch; "
                          "worklist: uneven term-range jobs written as work files and summed; "
                          "small: call latency of small positions, eager vs captured graph; "
-                         "replay: the reference's precision-1/2 output bits at N")
+                         "replay: the reference's precision-1/2 output bits at N; "
+                         "peaks: VALU issue-rate microbenchmarks")
     ap.add_argument("--batches", type=int, default=8,
                     help="batched: split the positions over this many launches (each launch "
                          "prints when it ends; 4096 positions in one launch run ~10 minutes "
@@ -435,6 +467,8 @@ def main():
         return bench_small(clb, args)
     if args.workload == "replay":
         return bench_replay(clb, args)
+    if args.workload == "peaks":
+        return bench_peaks(clb, args)
 
     n_pos, d, _ = clb.normalize(args.position)
     ctx = clb.Context(0, clb.PREC_U192, kernel=clb.KERNEL_SPLIT)

@@ -26,7 +26,7 @@ from typing import Iterable, List, Optional, Sequence, Tuple
 __all__ = [
     "ClbError", "lib", "LIB_PATH", "normalize", "formula", "hex48", "hex_dd", "u192_to_dd",
     "work_sum", "parse_worklist", "write_work_file", "device_count", "device_name", "Context",
-    "u192_to_int", "int_to_u192",
+    "u192_to_int", "int_to_u192", "issue_rate", "PEAK_OPS",
     "OK", "EINVAL", "EDEVICE", "ERANGE", "EIO", "ENOMEM", "EFORMAT",
     "PREC_DD106", "PREC_U192", "PREC_U192_BETA", "KERNEL_AUTO", "KERNEL_PLAIN", "KERNEL_SPLIT",
     "NUMER", "NUMESIGN", "DEN0", "DEN1", "REF_GLOBAL_WORK_SIZE", "NOALLOC_MAX_JOBS",
@@ -112,6 +112,8 @@ _SIGNATURES = {
     "clb_parse_worklist": ([ctypes.c_char_p, _P64, ctypes.POINTER(ctypes.c_int),
                             ctypes.POINTER(ctypes.c_int64), ctypes.c_size_t,
                             ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+    "clb_issue_rate": ([ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                        ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "clb_last_timing": ([_CTX, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
                         ctypes.c_int),
 }
@@ -224,6 +226,21 @@ def parse_worklist(path: str) -> Tuple[int, int, List[Tuple[int, int]]]:
     _check(lib.clb_parse_worklist(os.fsencode(path), ctypes.byref(digits), ctypes.byref(prec),
                                   arr, n.value, ctypes.byref(n)))
     return digits.value, prec.value, [(arr[2 * i], arr[2 * i + 1]) for i in range(n.value)]
+
+
+# Instructions (and whole modular squares) of the issue-rate microbenchmark, CLB_PEAK_* order.
+PEAK_OPS = ("v_fma_f64", "v_mul_f64", "v_add_f64", "v_rndne_f64", "v_ldexp_f64", "v_mad_u64_u32",
+            "v_mul_hi_u32", "v_mul_lo_u32", "v_add_u32", "v_fma_f32", "fp64 modular square",
+            "int64 Montgomery square")
+
+
+def issue_rate(op: int, target_ms: float = 100.0, device: int = 0) -> dict:
+    """One microbenchmark run: lane-operations (or modular squares) per second, the shader clock
+    measured during the run (GHz), the run's ms and the CU count."""
+    out = (ctypes.c_double * 4)()
+    _check(lib.clb_issue_rate(device, op, target_ms, out))
+    return {"op": PEAK_OPS[op], "per_s": out[0], "sclk_ghz": out[1], "ms": out[2],
+            "cus": int(out[3])}
 
 
 def device_count() -> int:

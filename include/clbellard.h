@@ -222,6 +222,29 @@ int clb_write_work_file(const char *work_dir, int j, int64_t b, const uint64_t w
  * file, adds line 0 of <work_dir>/<j>/<j>_<b>.txt mod 16^48.  counts[j] = files used. */
 int clb_work_sum(const char *work_dir, char hex[49], int counts[7]);
 
+/* Issue-rate microbenchmark (no reference counterpart; measurement for the roofline).  Runs one
+ * VALU instruction (CLB_PEAK_FMA_F64 ... CLB_PEAK_FMA_F32) or one engine's whole modular square
+ * (CLB_PEAK_FSQ: the FP64 engine's six-instruction square; CLB_PEAK_ISQ: the integer engine's
+ * 64-bit Montgomery square) on 8 independent chains per lane, 8 waves per SIMD on every CU, for
+ * about target_ms.  out[0] = lane-operations (or modular squares) per second, out[1] = the shader
+ * clock measured during the run (GHz: the shader-clock and 100 MHz real-time counters read by the
+ * first wave of the first 8 workgroups, one per XCD, averaged),
+ * out[2] = the run's ms, out[3] = compute units.  Allocates its own small buffers. */
+#define CLB_PEAK_FMA_F64 0
+#define CLB_PEAK_MUL_F64 1
+#define CLB_PEAK_ADD_F64 2
+#define CLB_PEAK_RNDNE_F64 3
+#define CLB_PEAK_LDEXP_F64 4
+#define CLB_PEAK_MAD_U64_U32 5
+#define CLB_PEAK_MUL_HI_U32 6
+#define CLB_PEAK_MUL_LO_U32 7
+#define CLB_PEAK_ADD_U32 8
+#define CLB_PEAK_FMA_F32 9
+#define CLB_PEAK_FSQ 10
+#define CLB_PEAK_ISQ 11
+#define CLB_PEAK_COUNT 12
+int clb_issue_rate(int device, int op, double target_ms, double out[4]);
+
 /* Timing of the last clb_normal / clb_batch_positions / clb_run_jobs / clb_batch_calc call on
  * the context's stream (HIP events): the head-term kernel (in graph mode: the whole replayed
  * graph) and the whole call, in ms. */

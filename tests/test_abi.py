@@ -138,6 +138,14 @@ def test_context_without_gpu_fails_loudly():
     assert e.value.code == -2
 
 
+def test_issue_rate_rejects_bad_arguments():
+    out = (ctypes.c_double * 4)()
+    assert clb.lib.clb_issue_rate(0, -1, 10.0, out) == clb.EINVAL
+    assert clb.lib.clb_issue_rate(0, 12, 10.0, out) == clb.EINVAL
+    assert clb.lib.clb_issue_rate(0, 0, 0.0, out) == clb.EINVAL
+    assert clb.lib.clb_issue_rate(0, 0, 10.0, None) == clb.EINVAL
+
+
 def test_null_and_range_errors():
     ctx = ctypes.c_void_p()
     assert clb.lib.clb_ctx_create(0, 7, ctypes.byref(ctx)) == -1

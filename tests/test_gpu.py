@@ -396,3 +396,14 @@ def test_contexts_on_threads(oracle):
     for n in ns:
         assert got[n] == [want[n]] * 3, n
         assert f"formula {7 + ns.index(n)} (" in errors[n], errors[n]
+
+
+def test_issue_rate_microbenchmark():
+    """clb_issue_rate runs, measures a plausible shader clock, and the FP64 FMA issues within the
+    VALU's one wave64 instruction per CU per clock."""
+    for op in (0, 5, 10):
+        r = clb.issue_rate(op, 20.0)
+        note(f"{r['op']}: {r['per_s']:.3e}/s at {r['sclk_ghz']:.3f} GHz")
+        assert r["per_s"] > 0 and 0.5 < r["sclk_ghz"] < 3.5 and r["cus"] > 0
+        if op == 0:
+            assert r["per_s"] / (r["cus"] * r["sclk_ghz"] * 1e9) <= 64 * 1.05
