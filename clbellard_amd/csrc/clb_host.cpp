@@ -132,7 +132,7 @@ struct clb_ctx {
   DevBuf<uint8_t> block;        // the call's block: header, tables, accumulator, results
   HostBuf block_host;           // its pinned staging copy (results land here too)
   float head_ms = 0.f, total_ms = 0.f;
-  int resident[5][kNumTpl] = {};  // resident k_head workgroups per CU, per kind and tile length
+  int resident[kNumKinds][kNumTpl] = {};  // resident k_head workgroups per CU, per kind and tile length
   int tpl_index = -1;            // index into kTplChoices, or -1: chosen per launch
   int min_tiles_per_wave = 0;    // launch-size threshold of that choice (0: a tile per 8 waves)
   // Graph mode: the kernel sequence of a call, captured once per buffer layout and kernel choice
@@ -147,7 +147,7 @@ struct clb_ctx {
   uint32_t wl_max_launches = 0;
   // Estimated modular squares per second of the term kernels, refined by every worklist launch.
   double wl_rate = 5e12;
-  int replay_resident[5] = {-1, -1, -1, -1, -1};  // resident k_replay workgroups per CU, per kind
+  int replay_resident[kNumKinds] = {-1, -1, -1, -1, -1};  // resident k_replay workgroups per CU, per kind
   // 106-bit-mode replay: the call's tables and results, the jobs' group accumulators, and the
   // context's own bigSumMem (65536 double-doubles) for BatchCalc / AddAns at the operator level.
   DevBuf<uint8_t> r106_block;

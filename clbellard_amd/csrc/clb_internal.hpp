@@ -25,8 +25,9 @@ enum Kind : int {
   kSplitW3 = 0,  // fp64 engine, 3-bit windows: m < 2^42
   kSplitW2 = 1,  // fp64 engine, 2-bit windows: m < 2^46
   kSplitW1 = 2,  // fp64 engine, 1-bit windows: m < 2^48
-  kPlainW3 = 3,  // int engine, lazy residues, 3-bit windows: m < 2^55
-  kPlainWide = 4 // int engine, reduced residues, 1-bit windows: m < 2^63
+  kPlainW3 = 3,   // int engine, lazy residues, 3-bit windows: m < 2^55
+  kPlainWide = 4, // int engine, reduced residues, 1-bit windows: m < 2^63
+  kNumKinds = 5
 };
 
 // One head-term job in the flat tile space.  Tiles are aligned to absolute multiples of the tile

@@ -115,27 +115,41 @@ __device__ __forceinline__ void tile_split(const JobDev& jb, uint64_t kbase, int
   add192(acc, 0, 0, u0 << 16);
 }
 
-// int engine: Montgomery form of 2^(e+192), then three exact-division quotient words.
-template <bool kWide, int TPL>
+// int engine: Montgomery form of 2^(e+192), then three exact-division quotient words.  The
+// Montgomery form of 1, 2^64 mod m, is carried along the lane's moduli (m += 64 den0 per term)
+// once m >= 2^37: with 2^64 = qq m + one, 2^64 = qq (m + s) + (one - qq s) and qq s < m, so the
+// next residue is one - qq s, or that plus m + s (and qq - 1) when it is negative; one 64-bit
+// division per lane per tile instead of one per term.  Smaller moduli (the first ~10^10 k) divide
+// per term.
+template <int KIND, int TPL>
 __device__ __forceinline__ void tile_plain(const JobDev& jb, uint64_t kbase, int lane, U192& acc,
                                            uint64_t& nterms) {
+  constexpr bool kWide = KIND == kPlainWide;
   uint64_t k = kbase + (uint64_t)lane;
   uint64_t m = jb.den0 * k + jb.den1;
   uint64_t e = 10ull * (jb.d - k);
   const uint64_t mstep = jb.den0 * (uint64_t)kWave;
+  const bool carry_one = jb.den0 * kbase + jb.den1 >= (1ull << 37);  // uniform per tile
+  uint64_t one = 0;
+  uint32_t qq = 0;  // floor(2^64 / m) < 2^27
+  if (carry_one) {
+    const uint64_t q64 = UINT64_MAX / m;  // m odd > 1, so 2^64 mod m != 0
+    qq = (uint32_t)q64;
+    one = UINT64_MAX - q64 * m + 1;
+  }
   for (int i = 0; i < TPL; ++i) {
     const uint64_t kw = kbase + (uint64_t)kWave * (uint64_t)i;
     if (kw >= jb.k1) break;
     if (k >= jb.k0 && k < jb.k1) {
       const uint64_t mp = neg_inv64(m);
-      const uint64_t one = (0ull - m) % m;  // 2^64 mod m
+      const uint64_t mont1 = carry_one ? one : (0ull - m) % m;  // 2^64 mod m
       const uint64_t E = e + 192;
       uint64_t h;
-      if (kWide) {
-        h = ipow2_mont_wide(E, m, mp, one);
+      if constexpr (kWide) {
+        h = ipow2_mont_wide(E, m, mp, mont1);
       } else {
         const int bl = 64 - __builtin_clzll(10ull * (jb.d - kw) + 192);
-        h = ipow2_mont<3>(E, bl, m, mp, one);
+        h = ipow2_mont<3>(E, bl, m, mp, mont1);
       }
       uint64_t q0, q1, q2;
       iquot192(h, m, mp, q0, q1, q2);
@@ -145,6 +159,15 @@ __device__ __forceinline__ void tile_plain(const JobDev& jb, uint64_t kbase, int
     k += kWave;
     m += mstep;
     e -= 10ull * kWave;
+    if (carry_one) {
+      const uint64_t t = (uint64_t)qq * mstep;  // < m
+      if (one >= t) {
+        one -= t;
+      } else {
+        one = one + m - t;
+        qq -= 1;
+      }
+    }
   }
 }
 
@@ -154,8 +177,8 @@ __device__ __forceinline__ void tile_terms(const JobDev& jb, uint64_t kbase, int
   if constexpr (KIND == kSplitW3) tile_split<3, TPL>(jb, kbase, lane, acc, nterms);
   if constexpr (KIND == kSplitW2) tile_split<2, TPL>(jb, kbase, lane, acc, nterms);
   if constexpr (KIND == kSplitW1) tile_split<1, TPL>(jb, kbase, lane, acc, nterms);
-  if constexpr (KIND == kPlainW3) tile_plain<false, TPL>(jb, kbase, lane, acc, nterms);
-  if constexpr (KIND == kPlainWide) tile_plain<true, TPL>(jb, kbase, lane, acc, nterms);
+  if constexpr (KIND == kPlainW3) tile_plain<kPlainW3, TPL>(jb, kbase, lane, acc, nterms);
+  if constexpr (KIND == kPlainWide) tile_plain<kPlainWide, TPL>(jb, kbase, lane, acc, nterms);
 }
 
 // ------------------------------------------------------------------ tail terms

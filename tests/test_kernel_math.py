@@ -361,3 +361,30 @@ def test_replay_step_numerator_fold():
         assert a == b, (s, x, m, a, b)
         seen.add((s[0] > 0.0, x < 0))
     assert len(seen) == 4
+
+
+# ------------------------------------------------------------------ integer engine (plain kernel)
+
+M64 = (1 << 64) - 1
+
+
+def test_montgomery_one_carried_along_the_stride():
+    """2^64 mod m carried from term to term (m += 64 den0) for m >= 2^37: one - qq s, plus m + s
+    when negative, equals 2^64 mod m at every term of a tile and qq stays floor(2^64 / m)."""
+    rng = random.Random(37)
+    for _ in range(3000):
+        den0 = rng.choice([4, 10])
+        den1 = rng.choice([1, 3, 5, 7, 9]) if den0 == 10 else rng.choice([1, 3])
+        k = rng.randrange(((1 << 37) - den1) // den0 + 1, 10**13)
+        m = den0 * k + den1
+        s = 64 * den0
+        qq, one = (M64 // m), M64 - (M64 // m) * m + 1
+        for _ in range(16):
+            assert one == (1 << 64) % m and qq == (1 << 64) // m and qq < 1 << 27
+            m += s
+            t = qq * s
+            assert t < m
+            if one >= t:
+                one -= t
+            else:
+                one, qq = one + m - t, qq - 1
