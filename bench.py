@@ -224,7 +224,8 @@ def bench_batched(clb, args):
     ctx = clb.Context(0, clb.PREC_U192)
     median = sorted(pos)[len(pos) // 2]
     tpl, bpc = ctx.autotune(median, 3000.0)
-    log(f"autotune at N={median}: {tpl} terms per lane, {bpc} workgroups per CU")
+    sched = ctx.get_schedule()
+    log(f"autotune at N={median}: {tpl} terms per lane, {bpc} workgroups per CU, {sched}")
     ctx.set_graph(not args.no_graph)
     log(f"batched: {len(pos)} positions, {terms:.4g} terms, "
         f"{'eager launches' if args.no_graph else 'captured HIP graph'}")
@@ -251,7 +252,7 @@ def bench_batched(clb, args):
                                f"{args.batches} persistent launch(es)", "positions": len(pos),
                    "terms": terms,
                    "graph": not args.no_graph, "terms_per_lane": tpl,
-                   "workgroups_per_cu": bpc, "head_kernel_ms": head_ms,
+                   "workgroups_per_cu": bpc, "schedule": sched, "head_kernel_ms": head_ms,
                    "checked_single": len(sample)}}), flush=True)
     ctx.close()
     return 0
@@ -369,6 +370,83 @@ def bench_peaks(clb, args):
     return 0
 
 
+def bench_schedule(clb, args):
+    """Sweep of the term kernel's launch knobs (clb_set_schedule, clb_set_tiling, clb_set_launch)
+    on the normal-mode computation at --position (default 10^11 here, 1.8 s per run) and on small
+    positions' call latency; every setting's digits must equal the first's.  Knobs: workgroup size
+    (1, 2, 4, 8 waves, each at full occupancy, i.e. 8 waves per SIMD), the tile queue (fixed
+    chunks of 256 or 64 tiles, or the guided schedule with 1, 2 or 4 chunks per wave per phase),
+    the slot copies (1, 4, 64 at most) and resident workgroups per CU."""
+    n = args.position if args.position != POSITION else 10**11
+    n, d, _ = clb.normalize(n)
+    ctx = clb.Context(0, clb.PREC_U192)
+    ref = ctx.normal(n)[0]
+
+    def timed(label, setup, pos=n, reps=2):
+        setup()
+        best = None
+        for _ in range(reps):
+            frac = ctx.normal(pos)[0]
+            h, t = ctx.last_timing()
+            if pos == n and frac != ref:
+                raise SystemExit(f"{label}: digits differ")
+            best = h if best is None else min(best, h)
+        ctx.set_schedule()
+        ctx.set_launch(0)
+        ctx.set_tiling(0)
+        log(f"{label:40s} head kernel {best:10.3f} ms")
+        return round(best, 3)
+
+    rows = {}
+    for w in (1, 2, 4, 8):
+        rows[f"workgroup {64 * w}"] = timed(f"workgroup {64 * w}",
+                                            lambda w=w: ctx.set_schedule(waves_per_group=w))
+    for label, mc, cpw in (("queue fixed 256 (default)", 256, -1), ("queue fixed 64", 64, -1),
+                           ("queue guided 1/wave, min 64", 64, 1),
+                           ("queue guided 2/wave, min 64", 64, 2),
+                           ("queue guided 4/wave, min 64", 64, 4),
+                           ("queue guided 2/wave, min 16", 16, 2)):
+        rows[label] = timed(label, lambda mc=mc, cpw=cpw: ctx.set_schedule(0, mc, cpw, 0))
+    for bpc in (4, 6):
+        rows[f"{bpc} workgroups per CU"] = timed(f"{bpc} workgroups per CU",
+                                                 lambda bpc=bpc: ctx.set_launch(bpc))
+    small = {}
+    for pos in (100000, 1000000, 10000000):
+        ref_small = ctx.normal(pos)[0]
+        for r in (1, 4, 64):
+            ctx.set_schedule(max_replicas=r)
+            best = None
+            for _ in range(args.calls // 10 + 3):
+                if ctx.normal(pos)[0] != ref_small:
+                    raise SystemExit("slot copies change the digits")
+                t = ctx.last_timing()[1]
+                best = t if best is None else min(best, t)
+            small[f"N={pos} max_replicas={r}"] = round(1e3 * best, 1)
+            log(f"N={pos} max_replicas {r}: call {1e3 * best:.1f} us")
+        for w in (1, 2, 4, 8):
+            ctx.set_schedule(waves_per_group=w)
+            best = None
+            for _ in range(args.calls // 10 + 3):
+                if ctx.normal(pos)[0] != ref_small:
+                    raise SystemExit("workgroup size changes the digits")
+                t = ctx.last_timing()[1]
+                best = t if best is None else min(best, t)
+            small[f"N={pos} workgroup {64 * w}"] = round(1e3 * best, 1)
+            log(f"N={pos} workgroup {64 * w}: call {1e3 * best:.1f} us")
+        ctx.set_schedule()
+    ctx.close()
+    base = rows["queue fixed 256 (default)"]
+    print(json.dumps({
+        "metric": f"term-kernel time at N={n} per launch setting (ms)", "value": base,
+        "unit": "ms", "n_gpus": 1, "steps": 2, "warmup": 1, "ms_per_step": base,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"launch-knob sweep: normal mode at N={n} (best of 2) and call "
+                               "latency at N=1e5, 1e6, 1e7 (us)", "head_kernel_ms": rows,
+                   "small_call_us": small}}), flush=True)
+    return 0
+
+
 def bench_replay(clb, args):
     """Normal mode at N (default 10^12) replaying the reference's kernels of precision 2, 1 and 0
     (clb_set_replay with the reference's 2^24 work-items per launch): the reference's own output
@@ -420,13 +498,14 @@ def main():
     ap.add_argument("--peak-ms", type=float, default=200.0,
                     help="peaks: target duration of each microbenchmark run")
     ap.add_argument("--workload", choices=["normal", "batched", "worklist", "small", "replay",
-                                           "peaks"],
+                                           "peaks", "schedule"],
                     default="normal",
                     help="normal: the metric (N = 10^12); batched: many positions in one launch; "
                          "worklist: uneven term-range jobs written as work files and summed; "
                          "small: call latency of small positions, eager vs captured graph; "
                          "replay: the reference's precision-1/2 output bits at N; "
-                         "peaks: VALU issue-rate microbenchmarks")
+                         "peaks: VALU issue-rate microbenchmarks; "
+                         "schedule: sweep of the launch knobs")
     ap.add_argument("--batches", type=int, default=8,
                     help="batched: split the positions over this many launches (each launch "
                          "prints when it ends; 4096 positions in one launch run ~10 minutes "
@@ -469,6 +548,8 @@ def main():
         return bench_replay(clb, args)
     if args.workload == "peaks":
         return bench_peaks(clb, args)
+    if args.workload == "schedule":
+        return bench_schedule(clb, args)
 
     n_pos, d, _ = clb.normalize(args.position)
     ctx = clb.Context(0, clb.PREC_U192, kernel=clb.KERNEL_SPLIT)

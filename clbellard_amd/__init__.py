@@ -81,6 +81,9 @@ _SIGNATURES = {
     "clb_set_graph": ([_CTX, ctypes.c_int], ctypes.c_int),
     "clb_set_replay": ([_CTX, _U64], ctypes.c_int),
     "clb_set_tiling": ([_CTX, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "clb_set_schedule": ([_CTX, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+                         ctypes.c_int),
+    "clb_get_schedule": ([_CTX] + [ctypes.POINTER(ctypes.c_int)] * 6, ctypes.c_int),
     "clb_autotune": ([_CTX, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(ctypes.c_int),
                       ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "clb_normalize": ([_U64, _P64, _P64, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
@@ -292,6 +295,21 @@ class Context:
         """Terms per lane per tile (16, 4, 1; 0 = chosen per launch: the longest giving every
         resident wave min_tiles_per_wave tiles, or with 0 one tile to every 8th wave)."""
         _check(lib.clb_set_tiling(self._ctx, terms_per_lane, min_tiles_per_wave))
+
+    def set_schedule(self, waves_per_group: int = 0, min_chunk: int = 0, chunks_per_wave: int = 0,
+                     max_replicas: int = 0) -> None:
+        """Workgroup size (waves), queue chunk (the guided queue's last), chunks per wave per phase
+        of a guided queue (0: fixed chunks), most copies per accumulator slot; 0 = default."""
+        _check(lib.clb_set_schedule(self._ctx, waves_per_group, min_chunk, chunks_per_wave,
+                                    max_replicas))
+
+    def get_schedule(self) -> dict:
+        """The launch settings in force (clb_get_schedule)."""
+        v = [ctypes.c_int() for _ in range(6)]
+        _check(lib.clb_get_schedule(self._ctx, *[ctypes.byref(x) for x in v]))
+        keys = ("terms_per_lane", "blocks_per_cu", "waves_per_group", "min_chunk",
+                "chunks_per_wave", "max_replicas")
+        return dict(zip(keys, (x.value for x in v)))
 
     def autotune(self, n: int, budget_ms: float = 2000.0) -> Tuple[int, int]:
         """Time tile lengths and grid sizes on a sample of position n; keep and return the

@@ -132,7 +132,15 @@ struct clb_ctx {
   DevBuf<uint8_t> block;        // the call's block: header, tables, accumulator, results
   HostBuf block_host;           // its pinned staging copy (results land here too)
   float head_ms = 0.f, total_ms = 0.f;
-  int resident[kNumKinds][kNumTpl] = {};  // resident k_head workgroups per CU, per kind and tile length
+  // resident k_head workgroups per CU, per kind, tile length and workgroup size (1, 2, 4, 8 waves)
+  int resident[kNumKinds][kNumTpl][4] = {};
+  // Tile schedule knobs (clb_set_schedule): k_head's workgroup size in waves, the queue chunk
+  // (the last phase's, for the guided schedule), the chunks per wave in each phase of the guided
+  // schedule (0: fixed chunks, the default), the most copies per accumulator slot.
+  int wg_waves = 4;
+  int min_chunk = 256;
+  int chunks_per_wave = 0;
+  int max_replicas = 64;
   int tpl_index = -1;            // index into kTplChoices, or -1: chosen per launch
   int min_tiles_per_wave = 0;    // launch-size threshold of that choice (0: a tile per 8 waves)
   // Graph mode: the kernel sequence of a call, captured once per buffer layout and kernel choice
@@ -186,13 +194,16 @@ int choose_kind(int pref, uint64_t max_m, int* kind) {
 }
 
 // Persistent grid: every workgroup resident at once (one wave of blocks), unless overridden.
+int wg_index(int waves) { return waves == 1 ? 0 : waves == 2 ? 1 : waves == 4 ? 2 : 3; }
+
 int blocks_per_cu(clb_ctx* ctx, int kind, int tpl) {
   if (ctx->blocks_per_cu > 0) return ctx->blocks_per_cu;
-  if (ctx->resident[kind][tpl] == 0) {
-    const int r = head_blocks_per_cu(kind, tpl);
-    ctx->resident[kind][tpl] = r > 0 ? r : 8;
+  int& res = ctx->resident[kind][tpl][wg_index(ctx->wg_waves)];
+  if (res == 0) {
+    const int r = head_blocks_per_cu(kind, tpl, ctx->wg_waves);
+    res = r > 0 ? r : 32 / ctx->wg_waves;
   }
-  return ctx->resident[kind][tpl];
+  return res;
 }
 
 // The captured launch sequence (head with tails, combine) for a kernel choice and block address,
@@ -200,7 +211,8 @@ int blocks_per_cu(clb_ctx* ctx, int kind, int tpl) {
 // while the block is not reallocated; at most 8 are kept.
 int graph_for(clb_ctx* ctx, int kind, int tpl, int grid, hipGraphExec_t* exec) {
   LaunchDev* hdr = reinterpret_cast<LaunchDev*>(ctx->block.p);
-  const uintptr_t key[4] = {(uintptr_t)kind, (uintptr_t)tpl, (uintptr_t)grid, (uintptr_t)hdr};
+  const uintptr_t key[4] = {(uintptr_t)kind | ((uintptr_t)ctx->wg_waves << 8), (uintptr_t)tpl,
+                            (uintptr_t)grid, (uintptr_t)hdr};
   for (const auto& g : ctx->graphs)
     if (memcmp(g.key, key, sizeof key) == 0) {
       *exec = g.exec;
@@ -208,7 +220,7 @@ int graph_for(clb_ctx* ctx, int kind, int tpl, int grid, hipGraphExec_t* exec) {
     }
   hipStream_t s = ctx->stream;
   HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  hipError_t e = launch_head(kind, tpl, hdr, grid, s);
+  hipError_t e = launch_head(kind, tpl, hdr, grid, ctx->wg_waves, s);
   if (e == hipSuccess) e = launch_combine(hdr, s);
   hipGraph_t graph = nullptr;
   const hipError_t e_end = hipStreamEndCapture(s, &graph);
@@ -232,11 +244,13 @@ int graph_for(clb_ctx* ctx, int kind, int tpl, int grid, hipGraphExec_t* exec) {
 // Launch plan of an exact-mode call.
 struct LaunchPlan {
   int tpl = 0;                // index into kTplChoices
+  int wg_waves = 4;           // workgroup size in waves
   uint64_t grid = 0;          // persistent workgroups
   uint64_t nwaves = 0;
   uint64_t flat = 0;          // tiles of the call's head jobs
   uint64_t static_tiles = 0;  // tiles per wave of the static phase (0: none)
-  uint64_t chunk = 1;         // tiles per queue grab of the dynamic phase
+  uint32_t nphases = 0;       // phases of the dynamic schedule (0: all static)
+  QueuePhase phases[kMaxPhases] = {};
   uint32_t replicas = 1;      // copies per accumulator slot
 };
 
@@ -245,36 +259,71 @@ struct LaunchPlan {
 // launches, e.g. N = 10^5, take shorter tiles: 4 terms per lane measured 11% faster there).
 // flat_of(tile_k) gives the call's tile count for a tile length.  Then the tile schedule: up to 64
 // tiles per wave, each wave takes one static range of consecutive tiles (no atomics; atomics on
-// one address serialise at ~35 ns each); larger launches take all tiles from the queue in chunks,
-// at least 8 per wave and at most 256 tiles (~15 ms of one wave's work at N = 10^12, the
-// granularity of the final balance).  And the copies per accumulator slot: enough that the waves'
-// flushes spread over about nwaves / 16 addresses per limb (a power of 2, at most 64), so short
-// launches do not queue on one slot.
+// one address serialise at ~35 ns each); larger launches take all tiles from the queue in fixed
+// chunks, at least 8 per wave and at most min_chunk = 256 tiles (~15 ms of one wave's work at
+// N = 10^12, the granularity of the final balance).  With chunks_per_wave > 0 the queue follows a
+// guided schedule instead: each phase hands out about half of the remaining tiles in
+// chunks_per_wave chunks per wave, so chunks shrink geometrically down to min_chunk, and the queue
+// takes about nwaves chunks_per_wave log2(tiles / nwaves) grabs instead of tiles / min_chunk;
+// measured 2.7-11% slower at N = 10^11 (1, 2, 4 chunks per wave) than fixed chunks, so it is not
+// the default.  And the copies per accumulator slot: enough that the waves' flushes spread over
+// about nwaves / 16 addresses per limb (a power of 2, at most max_replicas), so short launches do
+// not queue on one slot.
 template <class FlatOf>
 void plan_launch(clb_ctx* ctx, int kind, size_t nslots, FlatOf flat_of, LaunchPlan* p) {
+  p->wg_waves = ctx->wg_waves;
   int tpl = ctx->tpl_index;
   for (int t = (tpl >= 0 ? tpl : 0); t < kNumTpl; ++t) {
     p->grid = (uint64_t)ctx->num_cus * (uint64_t)blocks_per_cu(ctx, kind, t);
     p->flat = flat_of(tile_k(kTplChoices[t]));
     tpl = t;
-    const uint64_t waves = p->grid * (uint64_t)(kBlock / kWave);
+    const uint64_t waves = p->grid * (uint64_t)ctx->wg_waves;
     const uint64_t need =
         ctx->min_tiles_per_wave > 0 ? (uint64_t)ctx->min_tiles_per_wave * waves : waves / 8;
     if (ctx->tpl_index >= 0 || p->flat >= need) break;
   }
   p->tpl = tpl;
-  p->nwaves = p->grid * (uint64_t)(kBlock / kWave);
+  p->nwaves = p->grid * (uint64_t)ctx->wg_waves;
   const uint64_t nwaves = p->nwaves, flat = p->flat;
   p->static_tiles = 0;
-  p->chunk = 1;
+  p->nphases = 0;
   if (flat <= 64 * nwaves) {
     p->static_tiles = (flat + nwaves - 1) / nwaves;
   } else {
-    const uint64_t per = flat / (8 * nwaves);
-    p->chunk = per < 1 ? 1 : per > 256 ? 256 : per;
+    const uint64_t g = (uint64_t)ctx->chunks_per_wave;
+    uint64_t start = 0, first = 0, left = flat;
+    while (g > 0 && p->nphases + 1 < kMaxPhases) {
+      const uint64_t chunk = left / (2 * nwaves * g);
+      if (chunk < 2 * (uint64_t)ctx->min_chunk || chunk > 0xffffffffull) break;
+      p->phases[p->nphases++] = {start, (uint32_t)first, (uint32_t)chunk};
+      start += nwaves * g * chunk;
+      first += nwaves * g;
+      left -= nwaves * g * chunk;
+    }
+    // the last phase, to the end of the tile space: chunks of min_chunk (fixed chunks: at most
+    // min_chunk and at least 8 per wave), or larger where more than 2^30 of them would be needed
+    // (chunk indices stay below 2^31 with the waves' last grabs)
+    uint64_t cmin = (uint64_t)ctx->min_chunk;
+    if (g == 0) {
+      const uint64_t per = flat / (8 * nwaves);
+      if (per < cmin) cmin = per < 1 ? 1 : per;
+    }
+    if (left / cmin > (1ull << 30)) cmin = left / (1ull << 30) + 1;
+    p->phases[p->nphases++] = {start, (uint32_t)first, (uint32_t)cmin};
   }
   p->replicas = 1;
-  while (p->replicas < 64 && (uint64_t)p->replicas * 2 * nslots * 16 <= nwaves) p->replicas *= 2;
+  while (p->replicas < (uint32_t)ctx->max_replicas &&
+         (uint64_t)p->replicas * 2 * nslots * 16 <= nwaves)
+    p->replicas *= 2;
+}
+
+// The plan's schedule in a call's header.
+void plan_to_header(const LaunchPlan& p, LaunchDev* h) {
+  h->static_tiles = p.static_tiles;
+  h->waves_per_group = (uint32_t)p.wg_waves;
+  h->nphases = p.nphases;
+  for (uint32_t i = 0; i < p.nphases; ++i) h->phases[i] = p.phases[i];
+  h->replicas = p.replicas;
 }
 
 // Device-side copies a call leaves behind (clb_normal keeps the reference's Allloop state).
@@ -586,12 +635,11 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
   const size_t bytes = off_out + out_words * sizeof(uint64_t);
   if (bytes > 0xffffffffull) return fail(CLB_EINVAL, "too many jobs for one call");
   if ((rc = ctx->block.reserve(bytes)) || (rc = ctx->block_host.reserve(bytes))) return rc;
-  const uint64_t static_tiles = plan.static_tiles, chunk = plan.chunk;
   uint8_t* dev = ctx->block.p;
   LaunchDev h{};
   h.counter = 0;
   h.total_tiles = flat;
-  h.static_tiles = static_tiles;
+  plan_to_header(plan, &h);
   h.dst = (uint64_t)(uintptr_t)(accumulate_dev ? accumulate_dev
                                                : reinterpret_cast<uint64_t*>(dev + off_out));
   h.slot_out = slot_host ? (uint64_t)(uintptr_t)(dev + off_out + 3 * groups.size() * 8) : 0;
@@ -600,10 +648,8 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
   h.mirror_index = mirrors.index;
   h.accumulate = accumulate_dev != nullptr;
   h.njobs = (uint32_t)jobs.size();
-  h.chunk = (uint32_t)chunk;
   h.ntails = (uint32_t)tails.size();
   h.ngroups = (uint32_t)groups.size();
-  h.replicas = replicas;
   h.nslots = (uint32_t)nslots;
   h.off_jobs = (uint32_t)off_jobs;
   h.off_tails = (uint32_t)off_tails;
@@ -657,7 +703,8 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
     HIP_TRY(hipGraphLaunch(exec, s));
     HIP_TRY(hipEventRecord(ctx->ev[1], s));
   } else {
-    if (!replay || !jobs.empty()) HIP_TRY(launch_head(kind, tpl, hdr, (int)grid, s));
+    if (!replay || !jobs.empty())
+      HIP_TRY(launch_head(kind, tpl, hdr, (int)grid, ctx->wg_waves, s));
     if (replay && (!rjobs.empty() || !rtails.empty()))
       HIP_TRY(launch_replay(rkind, hdr, rgrid, s));
     HIP_TRY(hipEventRecord(ctx->ev[1], s));
@@ -1131,6 +1178,8 @@ int clb_autotune(clb_ctx* ctx, uint64_t n, double budget_ms, int* terms_per_lane
   if (!(budget_ms > 0.0)) return fail(CLB_EINVAL, "budget_ms must be positive");
   const int saved_tpl = ctx->tpl_index, saved_bpc = ctx->blocks_per_cu;
   const int saved_min = ctx->min_tiles_per_wave;
+  const int saved_wg = ctx->wg_waves, saved_chunk = ctx->min_chunk;
+  const int saved_cpw = ctx->chunks_per_wave, saved_rep = ctx->max_replicas;
   const bool saved_graph = ctx->use_graph;
   const uint64_t saved_replay = ctx->replay_gws;
   ctx->use_graph = false;
@@ -1154,6 +1203,10 @@ int clb_autotune(clb_ctx* ctx, uint64_t n, double budget_ms, int* terms_per_lane
     ctx->tpl_index = saved_tpl;
     ctx->blocks_per_cu = saved_bpc;
     ctx->min_tiles_per_wave = saved_min;
+    ctx->wg_waves = saved_wg;
+    ctx->min_chunk = saved_chunk;
+    ctx->chunks_per_wave = saved_cpw;
+    ctx->max_replicas = saved_rep;
     ctx->use_graph = saved_graph;
     ctx->replay_gws = saved_replay;
     return rc;
@@ -1161,6 +1214,10 @@ int clb_autotune(clb_ctx* ctx, uint64_t n, double budget_ms, int* terms_per_lane
   // probe at the default settings to size the sample
   ctx->tpl_index = 0;
   ctx->blocks_per_cu = 0;
+  ctx->wg_waves = 4;
+  ctx->min_chunk = 256;
+  ctx->chunks_per_wave = 0;
+  ctx->max_replicas = 64;
   const uint64_t probe = d < (1ull << 20) ? d : (1ull << 20);
   const uint64_t k_mid = d > probe ? (d - probe) / 2 : 0;
   if (int rc = run(ctx, jobs_for(k_mid, probe), shift, groups, res, nullptr, nullptr))
@@ -1168,7 +1225,15 @@ int clb_autotune(clb_ctx* ctx, uint64_t n, double budget_ms, int* terms_per_lane
   const double per_term = (ctx->head_ms > 0.f ? ctx->head_ms : 1e-3) / (double)probe;
   const int resident = blocks_per_cu(ctx, kSplitW3, 0);
   const int bpc_choices[3] = {resident, (3 * resident + 3) / 4, (resident + 1) / 2};
-  const int ncand = kNumTpl * 3, reps = 3;
+  // Stage 2 tries, one at a time on the stage-1 choice, the workgroup size (1, 2, 8 waves), the tile
+  // queue (fixed chunks of 64 or 1024 tiles, a guided queue) and the slot copies (at most 4).
+  struct Sched {
+    int wg, chunk, cpw, rep;
+  };
+  const Sched stage2[] = {{1, 256, 0, 64}, {2, 256, 0, 64}, {8, 256, 0, 64}, {4, 64, 0, 64},
+                          {4, 1024, 0, 64}, {4, 64, 4, 64}, {4, 256, 0, 4}};
+  const int nstage2 = (int)(sizeof stage2 / sizeof stage2[0]);
+  const int ncand = kNumTpl * 3 + nstage2, reps = 3;
   uint64_t s = (uint64_t)(budget_ms / (ncand * reps + 1) / per_term);
   if (s < probe) s = probe;
   if (s > d) s = d;
@@ -1176,32 +1241,70 @@ int clb_autotune(clb_ctx* ctx, uint64_t n, double budget_ms, int* terms_per_lane
   const std::vector<HostJob> sample = jobs_for(k0, s);
   double best = -1.0;
   int best_tpl = 0, best_bpc = resident;
+  // Times the context's current settings; a candidate must beat the best so far by 2% to
+  // replace it (the first is the default: 16 terms per lane, full residency, 256 work-items per
+  // workgroup, fixed chunks of 256 tiles), so timing noise does not move the choice.
+  auto trial = [&](bool* better) -> int {
+    double ms = -1.0;
+    for (int r = 0; r < reps; ++r) {
+      if (int rc = run(ctx, sample, shift, groups, res, nullptr, nullptr)) return rc;
+      if (ms < 0.0 || ctx->head_ms < ms) ms = ctx->head_ms;
+    }
+    if (best >= 0.0 && memcmp(res, best_res, sizeof res) != 0)
+      return fail(CLB_EDEVICE, "autotune: results differ between launch shapes");
+    memcpy(best_res, res, sizeof res);
+    *better = best < 0.0 || ms < best * 0.98;
+    if (*better) best = ms;
+    return CLB_OK;
+  };
   for (int t = 0; t < kNumTpl; ++t)
     for (int b = 0; b < 3; ++b) {
       if (b > 0 && bpc_choices[b] == bpc_choices[b - 1]) continue;
       ctx->tpl_index = t;
       ctx->blocks_per_cu = bpc_choices[b];
-      double ms = -1.0;
-      for (int r = 0; r < reps; ++r) {
-        if (int rc = run(ctx, sample, shift, groups, res, nullptr, nullptr)) return restore(rc);
-        if (ms < 0.0 || ctx->head_ms < ms) ms = ctx->head_ms;
-      }
-      if (best >= 0.0 && memcmp(res, best_res, sizeof res) != 0)
-        return restore(fail(CLB_EDEVICE, "autotune: results differ between launch shapes"));
-      memcpy(best_res, res, sizeof res);
-      // The first candidate is the default (16 terms per lane, full residency); another must
-      // beat the best so far by 2% to replace it, so timing noise does not move the choice.
-      if (best < 0.0 || ms < best * 0.98) {
-        best = ms;
+      bool better = false;
+      if (int rc = trial(&better)) return restore(rc);
+      if (better) {
         best_tpl = t;
         best_bpc = bpc_choices[b];
       }
     }
+  Sched best_s = {4, 256, 0, 64};
+  ctx->tpl_index = best_tpl;
+  for (int i = 0; i < nstage2; ++i) {
+    const Sched& c = stage2[i];
+    ctx->wg_waves = c.wg;
+    // the same resident waves per CU: the stage-1 workgroups per CU rescaled to this size
+    ctx->blocks_per_cu = best_bpc == resident ? 0 : (best_bpc * 4 + c.wg - 1) / c.wg;
+    ctx->min_chunk = c.chunk;
+    ctx->chunks_per_wave = c.cpw;
+    ctx->max_replicas = c.rep;
+    bool better = false;
+    if (int rc = trial(&better)) return restore(rc);
+    if (better) best_s = c;
+  }
   restore(CLB_OK);
   ctx->tpl_index = best_tpl;
-  ctx->blocks_per_cu = best_bpc == resident ? 0 : best_bpc;
+  ctx->wg_waves = best_s.wg;
+  ctx->blocks_per_cu = best_bpc == resident ? 0 : (best_bpc * 4 + best_s.wg - 1) / best_s.wg;
+  ctx->min_chunk = best_s.chunk;
+  ctx->chunks_per_wave = best_s.cpw;
+  ctx->max_replicas = best_s.rep;
   if (terms_per_lane) *terms_per_lane = kTplChoices[best_tpl];
   if (blocks_per_cu_out) *blocks_per_cu_out = best_bpc;
+  return CLB_OK;
+}
+
+int clb_get_schedule(clb_ctx* ctx, int* terms_per_lane, int* blocks_per_cu_out,
+                     int* waves_per_group, int* min_chunk, int* chunks_per_wave,
+                     int* max_replicas) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (terms_per_lane) *terms_per_lane = ctx->tpl_index < 0 ? 0 : kTplChoices[ctx->tpl_index];
+  if (blocks_per_cu_out) *blocks_per_cu_out = ctx->blocks_per_cu;
+  if (waves_per_group) *waves_per_group = ctx->wg_waves;
+  if (min_chunk) *min_chunk = ctx->min_chunk;
+  if (chunks_per_wave) *chunks_per_wave = ctx->chunks_per_wave;
+  if (max_replicas) *max_replicas = ctx->max_replicas;
   return CLB_OK;
 }
 
@@ -1229,6 +1332,24 @@ int clb_set_launch(clb_ctx* ctx, int blocks_per_cu) {
   if (int rc = check_ctx(ctx)) return rc;
   if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(CLB_EINVAL, "blocks_per_cu 0..64");
   ctx->blocks_per_cu = blocks_per_cu;
+  return CLB_OK;
+}
+
+int clb_set_schedule(clb_ctx* ctx, int waves_per_group, int min_chunk, int chunks_per_wave,
+                     int max_replicas) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (waves_per_group != 0 && waves_per_group != 1 && waves_per_group != 2 &&
+      waves_per_group != 4 && waves_per_group != 8)
+    return fail(CLB_EINVAL, "waves_per_group must be 0, 1, 2, 4 or 8");
+  if (min_chunk < 0 || min_chunk > 65536) return fail(CLB_EINVAL, "min_chunk 0..65536");
+  if (chunks_per_wave < -1 || chunks_per_wave > 64)
+    return fail(CLB_EINVAL, "chunks_per_wave -1..64");
+  if (max_replicas < 0 || max_replicas > 64 || (max_replicas & (max_replicas - 1)))
+    return fail(CLB_EINVAL, "max_replicas 0 or a power of 2 up to 64");
+  ctx->wg_waves = waves_per_group ? waves_per_group : 4;
+  ctx->min_chunk = min_chunk ? min_chunk : 256;
+  ctx->chunks_per_wave = chunks_per_wave < 0 ? 0 : chunks_per_wave;
+  ctx->max_replicas = max_replicas ? max_replicas : 64;
   return CLB_OK;
 }
 
@@ -1501,13 +1622,11 @@ int clb_run_jobs_noalloc(clb_ctx* ctx, const clb_job* jobs_in, size_t count, uin
   std::lock_guard<std::mutex> lock(*call_lock);  // one call at a time uses this device's area
   LaunchDev h{};
   h.total_tiles = plan.flat;
-  h.static_tiles = plan.static_tiles;
+  plan_to_header(plan, &h);
   h.dst = (uint64_t)(uintptr_t)(dev + off_out);
   h.njobs = (uint32_t)nhead;
-  h.chunk = (uint32_t)plan.chunk;
   h.ntails = (uint32_t)ntails;
   h.ngroups = (uint32_t)count;
-  h.replicas = plan.replicas;
   h.nslots = (uint32_t)count;
   h.off_jobs = (uint32_t)off_jobs;
   h.off_tails = (uint32_t)off_tails;
@@ -1546,7 +1665,7 @@ int clb_run_jobs_noalloc(clb_ctx* ctx, const clb_job* jobs_in, size_t count, uin
   hipStream_t s = ctx->stream;
   HIP_TRY(hipMemcpyAsync(dev, stage, off_out, hipMemcpyHostToDevice, s));
   HIP_TRY(hipEventRecord(ctx->ev[0], s));
-  HIP_TRY(launch_head(kind, plan.tpl, hdr, (int)plan.grid, s));
+  HIP_TRY(launch_head(kind, plan.tpl, hdr, (int)plan.grid, plan.wg_waves, s));
   HIP_TRY(hipEventRecord(ctx->ev[1], s));
   HIP_TRY(launch_combine(hdr, s));
   HIP_TRY(hipMemcpyAsync(stage + off_out, dev + off_out, 3 * count * sizeof(uint64_t),

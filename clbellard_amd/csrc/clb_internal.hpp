@@ -16,7 +16,9 @@ constexpr int kMaxTermsPerLane = 16;
 constexpr int kNumTpl = 3;
 constexpr int kTplChoices[kNumTpl] = {16, 4, 1};
 __host__ __device__ constexpr uint64_t tile_k(int tpl) { return (uint64_t)kWave * (uint64_t)tpl; }
-constexpr int kBlock = 256;                                // threads per workgroup
+constexpr int kBlock = 256;     // threads per workgroup (k_head's default; the other kernels')
+constexpr int kMaxBlock = 512;  // k_head's largest workgroup (8 waves)
+constexpr int kMaxPhases = 24;  // phases of the tile queue's guided schedule
 constexpr int kAccWords = 8;                               // u64 words per accumulator slot
 constexpr int kTailTerms = 20;                             // k = d .. d+19 (Bellard.cl:1164-1239)
 
@@ -70,6 +72,16 @@ struct GroupDev {
   uint32_t count;
 };
 
+// One phase of the dynamic tile schedule: chunks first_chunk, first_chunk + 1, ... of `chunk`
+// consecutive tiles each, starting at tile `start` (relative to the end of the static phase); the
+// last phase runs to the end of the tile space.  Chunk indices stay below 2^31 (the host sizes the
+// chunks so), so the kernel compares and multiplies them on the scalar unit in 32 bits.
+struct QueuePhase {
+  uint64_t start;
+  uint32_t first_chunk;
+  uint32_t chunk;
+};
+
 // Per-call header at offset 0 of the call's block.  The block holds, at the byte offsets below,
 // the job table, the tail table, the groups, the shifts, the accumulator (zeroed by the copy that
 // brings the block in) and room for the results.  The kernels read everything from the block, so
@@ -77,7 +89,7 @@ struct GroupDev {
 // block is not reallocated.  The host copies the block before each launch sequence, which also
 // resets the tile queue.
 struct LaunchDev {
-  unsigned long long counter;  // tile queue: tiles handed out in the dynamic phase
+  unsigned long long counter;  // tile queue: chunks handed out in the dynamic phase
   uint64_t total_tiles;        // flat tiles of all jobs
   uint64_t static_tiles;       // static phase: wave w takes tiles [w T, (w + 1) T), T = this
   uint64_t dst;                // device address of the group results (3 words each, MSW first)
@@ -87,7 +99,7 @@ struct LaunchDev {
   uint32_t mirror_index;
   uint32_t accumulate;         // 1: dst += result (the AddAns-style accumulation)
   uint32_t njobs;              // entries of the job table
-  uint32_t chunk;              // consecutive tiles per queue grab in the dynamic phase
+  uint32_t waves_per_group;    // k_head's workgroup size in waves (1, 2, 4 or 8)
   uint32_t ntails;             // entries of the tail table
   uint32_t ngroups;            // results (groups of slots)
   uint32_t replicas;           // copies per accumulator slot (a power of 2, at most 64)
@@ -99,6 +111,8 @@ struct LaunchDev {
   uint64_t rtotal_tiles;
   uint32_t nrjobs, nrtails, rchunk;
   uint32_t off_rjobs, off_rtails;
+  uint32_t nphases;                   // phases of the dynamic tile schedule (0: all static)
+  QueuePhase phases[kMaxPhases];
 };
 
 __host__ __device__ inline const uint8_t* hdr_at(const LaunchDev* h, uint32_t off) {
@@ -129,11 +143,12 @@ __host__ __device__ inline unsigned long long* hdr_acc(LaunchDev* h) {
 constexpr int kCombineBlocks = 256;  // k_combine grid (one block per group, grid-stride)
 constexpr int kCombineWaves = 8;     // k_combine block: waves sharing a group's slots
 
-int head_blocks_per_cu(int kind, int tpl_index);
+int head_blocks_per_cu(int kind, int tpl_index, int waves_per_group);
 
 // The launch sequence of one call: the head and tail terms (persistent grid of `grid`
 // workgroups), then the combine (slot replicas folded, shifted, summed per group).
-hipError_t launch_head(int kind, int tpl_index, LaunchDev* hdr, int grid, hipStream_t stream);
+hipError_t launch_head(int kind, int tpl_index, LaunchDev* hdr, int grid, int waves_per_group,
+                       hipStream_t stream);
 hipError_t launch_combine(LaunchDev* hdr, hipStream_t stream);
 hipError_t launch_zero(unsigned long long* p, uint64_t nwords, hipStream_t stream);
 hipError_t launch_sum7(uint64_t* ans, uint64_t* partial, int shift, hipStream_t stream);

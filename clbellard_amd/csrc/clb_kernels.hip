@@ -250,33 +250,39 @@ __device__ __forceinline__ void tail_job(const TailDev tj, int lane, uint32_t re
 // Persistent head-term kernel; it also takes the tail jobs (wave w the tails w, w + nwaves, ...,
 // before its tiles).  Flat tile f belongs to the last job with flat_begin <= f.  Wave w first
 // takes the static range [w T, (w + 1) T) of T = static_tiles consecutive tiles (no atomics;
-// small launches are all static), then chunks of `chunk` consecutive tiles after the nwaves T
-// static ones from a global counter (one atomic per chunk), so the launch balances itself
-// whatever a tile costs.  A wave's tiles only increase, so its job index only moves forward.
+// small launches are all static), then chunks of consecutive tiles after the nwaves T static
+// ones: chunk c (one atomic on a global counter per chunk) lies in the schedule's phase p with
+// first_chunk[p] <= c < first_chunk[p + 1], whose chunks shrink from phase to phase (a guided
+// schedule: few grabs while much work remains, small chunks for the final balance), so the launch
+// balances itself whatever a tile costs.  A wave's tiles and chunks only increase, so its job
+// index and phase only move forward.  Workgroups are waves_per_group waves (the grid's waves
+// share no LDS and no barrier, so the size only affects how the hardware places them).
 template <int KIND, int TPL>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8)))
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(8)))
 k_head(LaunchDev* __restrict__ hdr) {
   unsigned long long* __restrict__ acc = hdr_acc(hdr);
   const JobDev* __restrict__ jobs = hdr_jobs(hdr);
   const int njobs = (int)hdr->njobs;
   const uint64_t total_tiles = hdr->total_tiles;
-  const uint64_t chunk = hdr->chunk;
-  const uint64_t nwaves = (uint64_t)grid_dim_x() * (kBlock / kWave);
+  const uint32_t wpg = hdr->waves_per_group;
+  const uint32_t nphases = hdr->nphases;
+  // Wave counts and the static phase fit 32 bits (at most 64 static tiles per wave), which keeps
+  // this bookkeeping in scalar registers.
+  const uint32_t nwaves = grid_dim_x() * wpg;
   // readfirstlane: the compiler does not know the work-item index / 64 is uniform, and a
   // divergent tile index would move the job table loads and all per-tile values into vector
   // registers.
-  const uint64_t wave_id = (uint64_t)block_x() * (kBlock / kWave) +
-                           (uint64_t)__builtin_amdgcn_readfirstlane(thread_x() / kWave);
-  const uint64_t static_tiles = hdr->static_tiles;
+  const uint32_t wave_id = block_x() * wpg + __builtin_amdgcn_readfirstlane(thread_x() / kWave);
+  const uint32_t static_tiles = (uint32_t)hdr->static_tiles;
   const uint64_t dyn0 = static_tiles * nwaves;  // first tile of the dynamic phase
   const int lane = (int)(thread_x() & (kWave - 1));
   // Slot s has `replicas` copies; wave w adds into copy w mod replicas, which spreads the atomics
   // of many short flushes over more addresses (atomics on one address serialise).
   const uint32_t replicas = hdr->replicas;
-  const uint32_t rep = (uint32_t)wave_id & (replicas - 1);
+  const uint32_t rep = wave_id & (replicas - 1);
   {
     const TailDev* __restrict__ tails = hdr_tails(hdr);
-    for (uint64_t t = wave_id; t < hdr->ntails; t += nwaves)
+    for (uint32_t t = wave_id; t < hdr->ntails; t += nwaves)
       tail_job(tails[t], lane, replicas, rep, acc);
   }
   int cur = -1;
@@ -306,15 +312,19 @@ k_head(LaunchDev* __restrict__ hdr) {
   };
   // One loop and one call site of tile(), so the term code is emitted once (two copies cost
   // registers and occupancy).  All branches here are wave-uniform.
-  uint64_t f = wave_id * static_tiles;
+  uint64_t f = wave_id * static_tiles;  // < 2^32: 64 static tiles per wave at most
   uint64_t f_end = f + static_tiles < total_tiles ? f + static_tiles : total_tiles;
+  uint32_t phase = 0;
   for (;;) {
     if (f >= f_end) {
-      if (dyn0 >= total_tiles) break;
-      unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(&hdr->counter, (unsigned long long)chunk);
-      f = dyn0 + (((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32) |
-                  (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)base));
+      if (dyn0 >= total_tiles || nphases == 0) break;
+      unsigned long long got = 0;
+      if (lane == 0) got = atomicAdd(&hdr->counter, 1ull);
+      const uint32_t c = __builtin_amdgcn_readfirstlane((uint32_t)got);
+      while (phase + 1 < nphases && c >= hdr->phases[phase + 1].first_chunk) ++phase;
+      const uint32_t chunk = hdr->phases[phase].chunk;
+      f = dyn0 + hdr->phases[phase].start +
+          (uint64_t)(c - hdr->phases[phase].first_chunk) * (uint64_t)chunk;
       if (f >= total_tiles) break;
       f_end = f + chunk < total_tiles ? f + chunk : total_tiles;
     }
@@ -439,58 +449,63 @@ __global__ void k_sum7(uint64_t* ans, uint64_t* partial, int shift) {
 // ------------------------------------------------------------------ launch wrappers
 
 template <int KIND, int TPL>
-static int resident_blocks() {
+static int resident_blocks(int block) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_head<KIND, TPL>, kBlock, 0) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_head<KIND, TPL>, block, 0) != hipSuccess)
     n = 0;
   return n;
 }
 
 template <int KIND>
-static int resident_blocks_tpl(int tpl_index) {
+static int resident_blocks_tpl(int tpl_index, int block) {
   switch (tpl_index) {
-    case 0: return resident_blocks<KIND, kTplChoices[0]>();
-    case 1: return resident_blocks<KIND, kTplChoices[1]>();
-    case 2: return resident_blocks<KIND, kTplChoices[2]>();
+    case 0: return resident_blocks<KIND, kTplChoices[0]>(block);
+    case 1: return resident_blocks<KIND, kTplChoices[1]>(block);
+    case 2: return resident_blocks<KIND, kTplChoices[2]>(block);
     default: return 0;
   }
 }
 
-int head_blocks_per_cu(int kind, int tpl_index) {
+int head_blocks_per_cu(int kind, int tpl_index, int waves_per_group) {
+  const int block = kWave * waves_per_group;
   switch (kind) {
-    case kSplitW3: return resident_blocks_tpl<kSplitW3>(tpl_index);
-    case kSplitW2: return resident_blocks_tpl<kSplitW2>(tpl_index);
-    case kSplitW1: return resident_blocks_tpl<kSplitW1>(tpl_index);
-    case kPlainW3: return resident_blocks_tpl<kPlainW3>(tpl_index);
-    case kPlainWide: return resident_blocks_tpl<kPlainWide>(tpl_index);
+    case kSplitW3: return resident_blocks_tpl<kSplitW3>(tpl_index, block);
+    case kSplitW2: return resident_blocks_tpl<kSplitW2>(tpl_index, block);
+    case kSplitW1: return resident_blocks_tpl<kSplitW1>(tpl_index, block);
+    case kPlainW3: return resident_blocks_tpl<kPlainW3>(tpl_index, block);
+    case kPlainWide: return resident_blocks_tpl<kPlainWide>(tpl_index, block);
     default: return 0;
   }
 }
 
 template <int KIND>
-static hipError_t launch_head_tpl(int tpl_index, LaunchDev* hdr, int grid, hipStream_t stream) {
+static hipError_t launch_head_tpl(int tpl_index, LaunchDev* hdr, int grid, int block,
+                                  hipStream_t stream) {
   switch (tpl_index) {
     case 0:
-      hipLaunchKernelGGL((k_head<KIND, kTplChoices[0]>), dim3(grid), dim3(kBlock), 0, stream, hdr);
+      hipLaunchKernelGGL((k_head<KIND, kTplChoices[0]>), dim3(grid), dim3(block), 0, stream, hdr);
       break;
     case 1:
-      hipLaunchKernelGGL((k_head<KIND, kTplChoices[1]>), dim3(grid), dim3(kBlock), 0, stream, hdr);
+      hipLaunchKernelGGL((k_head<KIND, kTplChoices[1]>), dim3(grid), dim3(block), 0, stream, hdr);
       break;
     case 2:
-      hipLaunchKernelGGL((k_head<KIND, kTplChoices[2]>), dim3(grid), dim3(kBlock), 0, stream, hdr);
+      hipLaunchKernelGGL((k_head<KIND, kTplChoices[2]>), dim3(grid), dim3(block), 0, stream, hdr);
       break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_head(int kind, int tpl_index, LaunchDev* hdr, int grid, hipStream_t stream) {
+hipError_t launch_head(int kind, int tpl_index, LaunchDev* hdr, int grid, int waves_per_group,
+                       hipStream_t stream) {
+  if (waves_per_group < 1 || waves_per_group > kMaxBlock / kWave) return hipErrorInvalidValue;
+  const int block = kWave * waves_per_group;
   switch (kind) {
-    case kSplitW3: return launch_head_tpl<kSplitW3>(tpl_index, hdr, grid, stream);
-    case kSplitW2: return launch_head_tpl<kSplitW2>(tpl_index, hdr, grid, stream);
-    case kSplitW1: return launch_head_tpl<kSplitW1>(tpl_index, hdr, grid, stream);
-    case kPlainW3: return launch_head_tpl<kPlainW3>(tpl_index, hdr, grid, stream);
-    case kPlainWide: return launch_head_tpl<kPlainWide>(tpl_index, hdr, grid, stream);
+    case kSplitW3: return launch_head_tpl<kSplitW3>(tpl_index, hdr, grid, block, stream);
+    case kSplitW2: return launch_head_tpl<kSplitW2>(tpl_index, hdr, grid, block, stream);
+    case kSplitW1: return launch_head_tpl<kSplitW1>(tpl_index, hdr, grid, block, stream);
+    case kPlainW3: return launch_head_tpl<kPlainW3>(tpl_index, hdr, grid, block, stream);
+    case kPlainWide: return launch_head_tpl<kPlainWide>(tpl_index, hdr, grid, block, stream);
     default: return hipErrorInvalidValue;
   }
 }

@@ -89,16 +89,33 @@ int clb_set_launch(clb_ctx *ctx, int blocks_per_cu);
  * (0, the default: one tile to every 8th wave, so 16 except for the smallest launches).  Results
  * are identical for every choice. */
 int clb_set_tiling(clb_ctx *ctx, int terms_per_lane, int min_tiles_per_wave);
+/* Tile schedule of the term kernel (no reference counterpart; the reference's launches have a
+ * fixed shape, 2^24 work-items in workgroups of 256, Bellard_192.hsp:5-7).  waves_per_group: the
+ * workgroup size in waves, 1, 2, 4 or 8 (0: 4, i.e. 256 work-items); min_chunk: tiles per grab of
+ * the tile queue, at most (0: 256); chunks_per_wave: > 0 for a guided queue whose chunks halve from
+ * phase to phase, that many per wave per phase, down to min_chunk (0 or -1: fixed chunks, the
+ * default); max_replicas: the most copies of an accumulator slot, a power of 2 (0: 64).  Results
+ * are identical for every choice. */
+int clb_set_schedule(clb_ctx *ctx, int waves_per_group, int min_chunk, int chunks_per_wave,
+                     int max_replicas);
 /* Autotuner (no reference counterpart): times the tile lengths 16, 4, 1 against 1, 3/4 and 1/2
  * of the resident workgroups per CU on a sample of the normal-mode computation at position n
  * (the seven formulas' head terms around d/2, sized to spend about budget_ms; best of 3 runs
- * each), keeps the fastest on the context (as clb_set_tiling / clb_set_launch would; a candidate
- * must beat the default, 16 terms and full residency, by 2%) and returns it (outputs may be
- * NULL).
+ * each), then, one at a time on that choice, the workgroup size (64, 128, 512 work-items at the
+ * same waves per CU), the tile queue (fixed chunks of 64 or 1024 tiles, a guided queue) and at
+ * most 4 slot copies; keeps the fastest on the context (as clb_set_tiling / clb_set_launch /
+ * clb_set_schedule would; a candidate must beat the best so far, starting from the defaults, by
+ * 2%) and returns the tile length and workgroups per CU (outputs may be NULL; clb_get_schedule
+ * reads the rest).
  * Every candidate's sum is compared; a difference returns CLB_EDEVICE.  The exact kernels are
  * timed even on a context in replay mode (clb_set_replay), whose setting is kept. */
 int clb_autotune(clb_ctx *ctx, uint64_t n, double budget_ms, int *terms_per_lane,
                  int *blocks_per_cu);
+/* The context's launch settings (no reference counterpart): terms per lane (0: chosen per
+ * launch), workgroups per CU (0: full residency), then clb_set_schedule's four values.  Outputs
+ * may be NULL. */
+int clb_get_schedule(clb_ctx *ctx, int *terms_per_lane, int *blocks_per_cu, int *waves_per_group,
+                     int *min_chunk, int *chunks_per_wave, int *max_replicas);
 /* Launch mode (no reference counterpart; the reference enqueues every kernel from the HSP host,
  * KernelFuncs.as:25-74): 1 = run each call's kernel sequence (head and tail terms, combine) as a
  * HIP graph, captured on the first call with a given buffer layout and kernel choice and replayed

@@ -337,8 +337,11 @@ def test_autotune_keeps_bits(oracle):
     with clb.Context(0, clb.PREC_U192) as c:
         for n, budget in ((250000, 50.0), (10**9, 300.0)):
             tpl, bpc = c.autotune(n, budget)
-            note(f"autotune N={n}: {tpl} terms per lane, {bpc} workgroups per CU")
-            assert tpl in (16, 4, 1) and bpc >= 1
+            sched = c.get_schedule()
+            note(f"autotune N={n}: {tpl} terms per lane, {bpc} workgroups per CU, {sched}")
+            assert tpl in (16, 4, 1) and bpc >= 1 and sched["terms_per_lane"] == tpl
+            assert sched["waves_per_group"] in (1, 2, 4, 8)
+            assert sched["min_chunk"] in (64, 256, 1024) and sched["max_replicas"] in (4, 64)
         assert w2i(c.normal(250000)[0]) == oracle.normal(250000)
         with clb.Context(0, clb.PREC_U192) as ref:
             assert c.normal(10**8) == ref.normal(10**8)
@@ -407,3 +410,23 @@ def test_issue_rate_microbenchmark():
         assert r["per_s"] > 0 and 0.5 < r["sclk_ghz"] < 3.5 and r["cus"] > 0
         if op == 0:
             assert r["per_s"] / (r["cus"] * r["sclk_ghz"] * 1e9) <= 64 * 1.05
+
+
+def test_schedule_knobs_give_identical_bits(oracle):
+    """Every workgroup size, tile-queue schedule and slot-copy limit (clb_set_schedule) gives the
+    oracle's bits, on small launches (static tiles) and large ones (the guided queue); bad values
+    are refused."""
+    with clb.Context(0, clb.PREC_U192) as c:
+        _, d = py_normalize(10**9)
+        jobs = [(d, 0, d + 1, j) for j in range(7)]
+        want = c.run_jobs(jobs)
+        for w, mc, cpw, r in ((1, 0, 0, 0), (2, 0, 0, 0), (8, 0, 0, 0), (0, 256, -1, 0),
+                              (0, 1, -1, 0), (0, 16, 4, 0), (0, 64, 1, 1), (8, 7, 3, 4)):
+            note(f"schedule waves={w} min_chunk={mc} chunks_per_wave={cpw} replicas={r}")
+            c.set_schedule(w, mc, cpw, r)
+            for n in (5, 65540, 250000):
+                assert w2i(c.normal(n)[0]) == oracle.normal(n), (w, mc, cpw, r, n)
+            assert c.run_jobs(jobs) == want, (w, mc, cpw, r)
+        for bad in ((3, 0, 0, 0), (0, -1, 0, 0), (0, 0, -2, 0), (0, 0, 0, 3), (16, 0, 0, 0)):
+            with pytest.raises(clb.ClbError):
+                c.set_schedule(*bad)

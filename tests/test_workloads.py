@@ -66,7 +66,7 @@ def tuned():
     median = sorted(POSITIONS)[len(POSITIONS) // 2]
     tpl, bpc = ctx.autotune(median, 3000.0)
     note(f"batched positions: autotuned at N={median}: {tpl} terms per lane, {bpc} workgroups "
-         "per CU; captured HIP graph")
+         f"per CU, {ctx.get_schedule()}; captured HIP graph")
     ctx.set_graph(True)
     yield ctx
     ctx.close()
