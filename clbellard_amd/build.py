@@ -59,6 +59,7 @@ def _run(cmd) -> None:
 
 def build_library() -> str:
     units = [("clb_kernels.hip", HIP_FLAGS), ("clb_host.cpp", HIP_FLAGS),
+             ("clb_format.cpp", HIP_FLAGS),
              ("clb_replay.hip", HIP_FLAGS), ("clb_noalloc.hip", HIP_FLAGS),
              ("clb_peaks.hip", HIP_FLAGS)]
     tmp = LIB + ".tmp"

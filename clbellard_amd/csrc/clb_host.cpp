@@ -1,9 +1,10 @@
 // clb_host.cpp -- host side of libclbellard.so: the C ABI declared in include/clbellard.h.
 //
-// This replaces the reference's HSP host (Bellard_192.hsp, KernelFuncs.as, hexsademical.as,
-// workmodeSum0_6.hsp) and its OpenCL plugin calls with one C++ library over the HIP runtime.
-// Every computation of terms happens in the kernels of clb_kernels.hip; this file only builds
-// job tables, moves a few hundred bytes, and formats results.
+// This replaces the reference's HSP host (Bellard_192.hsp, KernelFuncs.as) and its OpenCL plugin
+// calls with one C++ library over the HIP runtime.  Every computation of terms happens in the
+// kernels of clb_kernels.hip; this file builds job tables, moves a few hundred bytes and runs work
+// mode.  The host-only entry points (formats, worklists, work files, the summer) are in
+// clb_format.cpp.
 #include <hip/hip_runtime.h>
 #include <sys/stat.h>
 #include <sys/types.h>
@@ -19,23 +20,12 @@
 #include <vector>
 
 #include "../../include/clbellard.h"
+#include "clb_hostlib.hpp"
 #include "clb_internal.hpp"
 
 using namespace clb;
 
 namespace {
-
-thread_local std::string g_last_error;
-
-int fail(int code, const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  g_last_error = buf;
-  return code;
-}
 
 #define HIP_TRY(expr)                                                                    \
   do {                                                                                   \
@@ -44,17 +34,6 @@ int fail(int code, const char* fmt, ...) {
       return fail(CLB_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),    \
                   __FILE__, __LINE__);                                                   \
   } while (0)
-
-// KernelFuncs.as:9-12: numerator, sign flag (1 = minus), denominator den0*k + den1.
-constexpr uint64_t kNumer[7] = {32, 1, 256, 64, 4, 4, 1};
-constexpr int64_t kNumeSign[7] = {1, 1, 0, 1, 1, 1, 0};
-constexpr uint64_t kDen0[7] = {4, 4, 10, 10, 10, 10, 10};
-constexpr uint64_t kDen1[7] = {1, 3, 1, 3, 5, 7, 9};
-// Sum7_192 (Bellard.cl:1364-1368): multiplication by c_j is a left shift by log2(c_j).
-constexpr uint8_t kShift[7] = {5, 0, 8, 6, 2, 2, 0};
-
-constexpr uint64_t kMaxDigits = 999999999999999999ull;  // Bellard_192.hsp:450
-constexpr uint64_t kBatchSize = 137438953472ull;        // Bellard_192.hsp:19, 2^37 terms
 
 // Head-term job before tiling.
 struct HostJob {
@@ -765,158 +744,6 @@ int out_format(const clb_ctx* ctx, int precision) {
   return ctx->replay_gws && precision == CLB_PREC_DD106 ? 3 : precision;
 }
 
-int normalize(uint64_t n_in, uint64_t* n_out, uint64_t* d_out, int* overdouble) {
-  if (n_in <= 4 || n_in > kMaxDigits)
-    return fail(CLB_EINVAL, "invalid digits %llu (need 5 <= N <= %llu)",
-                (unsigned long long)n_in, (unsigned long long)kMaxDigits);
-  uint64_t d = ((n_in - 1) * 2 - 3) / 5;  // Bellard_192.hsp:468
-  if (d % 2 == 0) d--;                    // :469
-  if (n_out) *n_out = (d * 5 + 3) / 2 + 1;  // :470, a multiple of 5
-  if (d_out) *d_out = d;
-  if (overdouble) *overdouble = d * 10 + 9 >= 9007199254740992ull;  // :472
-  return CLB_OK;
-}
-
-void hex48(const uint64_t w[3], char out[49]) {
-  static const char hx[] = "0123456789abcdef";
-  for (int i = 0; i < 3; ++i)
-    for (int b = 0; b < 16; ++b) out[16 * i + b] = hx[(w[i] >> (60 - 4 * b)) & 15];
-  out[48] = 0;
-}
-
-void twosum(double a, double b, double* x, double* y) {
-  double s = a + b;
-  double t = s - a;
-  *y = (a - (s - t)) + (b - t);
-  *x = s;
-}
-
-// addcab of hexsademical.as:12-19.
-void dd_add(double* h, double* l, double bh, double bl) {
-  double x, y;
-  twosum(*h, bh, &x, &y);
-  y = y + *l + bl;
-  twosum(x, y, h, l);
-}
-
-// hexadecimal_dd, hexsademical.as:21-101: 14 base-256 digits of a double-double in [-1, 1),
-// with carries and borrows pushed into the already written (more significant) bytes of a
-// 24-byte little-endian buffer, printed as 48 hex digits (the last 20 are zero).
-void hex_dd(const double dd[2], char out[49]) {
-  double hh = dd[0], ll = dd[1];
-  uint32_t words[6] = {0, 0, 0, 0, 0, 0};
-  uint8_t* bytes = reinterpret_cast<uint8_t*>(words);
-  if (hh < 0.0) dd_add(&hh, &ll, 1.0, 0.0);
-  auto carry = [&](int idx, int dir) {
-    if (idx >= 24) return;
-    int b = idx / 4;
-    uint32_t add = 1u << ((idx % 4) * 8);
-    words[b] += dir > 0 ? add : (uint32_t)0 - add;
-    for (int rep = 5 - b; rep > 0; --rep) {
-      const int32_t v = (int32_t)words[b];
-      const bool more = dir > 0 ? (v >= 0 && v < (int32_t)add) : (v < 0 && v >= -(int32_t)add);
-      if (!more) break;
-      ++b;
-      add = 1;
-      words[b] += dir > 0 ? 1u : 0xffffffffu;
-    }
-  };
-  for (int cnt = 0; cnt < 14; ++cnt) {
-    hh *= 256.0;
-    ll *= 256.0;
-    twosum(hh, ll, &hh, &ll);
-    int digit = (int)hh;
-    if (digit < 256 && hh >= 0.0) {
-      bytes[23 - cnt] = (uint8_t)digit;
-      hh -= digit;
-      continue;
-    }
-    if (digit == 256) {
-      carry(23 - cnt + 1, +1);
-      bytes[23 - cnt] = 0;
-      hh -= 256.0;
-    }
-    if (hh < 0.0) {
-      digit = (int)(hh + 256.0);
-      carry(23 - cnt + 1, -1);
-      bytes[23 - cnt] = (uint8_t)digit;
-      dd_add(&hh, &ll, 256.0, 0.0);
-    }
-  }
-  static const char hx[] = "0123456789abcdef";
-  for (int i = 0; i < 24; ++i) {
-    out[2 * i] = hx[bytes[23 - i] >> 4];
-    out[2 * i + 1] = hx[bytes[23 - i] & 15];
-  }
-  out[48] = 0;
-}
-
-// Round a non-negative integer given as up to 3 words (most significant first) times 2^-192 to
-// the nearest double (ties to even).
-double round_u192(const uint64_t w[3], bool* exact) {
-  int lead = -1;
-  for (int i = 0; i < 3 && lead < 0; ++i)
-    if (w[i]) lead = (2 - i) * 64 + (63 - __builtin_clzll(w[i]));
-  *exact = true;
-  if (lead < 0) return 0.0;
-  auto bit = [&](int p) -> int { return p < 0 ? 0 : (int)((w[2 - p / 64] >> (p % 64)) & 1); };
-  uint64_t mant = 0;
-  const int low = lead - 52;  // position of the last kept bit
-  for (int p = lead; p >= low; --p) mant = (mant << 1) | (uint64_t)bit(p);
-  bool sticky = false;
-  for (int p = low - 2; p >= 0; --p)
-    if (bit(p)) {
-      sticky = true;
-      break;
-    }
-  const int rbit = low - 1 >= 0 ? bit(low - 1) : 0;
-  if (low - 1 >= 0 && (rbit || sticky)) *exact = false;
-  if (rbit && (sticky || (mant & 1))) mant++;
-  return std::ldexp((double)mant, low - 192);
-}
-
-int u192_to_dd(const uint64_t w[3], double dd[2]) {
-  bool exact;
-  const double hi = round_u192(w, &exact);
-  // remainder w - hi * 2^192 as a signed 192-bit integer
-  uint64_t h[3] = {0, 0, 0};
-  int ex;
-  const double fr = std::frexp(hi, &ex);  // hi = fr * 2^ex, fr in [0.5, 1)
-  if (hi != 0.0) {
-    const uint64_t mant = (uint64_t)std::ldexp(fr, 53);
-    const int pos = ex - 53 + 192;  // bit position of mant's lsb
-    for (int b = 0; b < 54; ++b)
-      if ((mant >> b) & 1) {
-        const int p = pos + b;
-        if (p >= 0 && p < 192) h[2 - p / 64] |= 1ull << (p % 64);
-      }
-  }
-  uint64_t r[3];
-  uint64_t borrow = 0;
-  for (int i = 2; i >= 0; --i) {
-    const uint64_t a = w[i], b = h[i];
-    const uint64_t d1 = a - b;
-    const uint64_t b1 = a < b;
-    const uint64_t d2 = d1 - borrow;
-    const uint64_t b2 = d1 < borrow;
-    r[i] = d2;
-    borrow = b1 | b2;
-  }
-  const bool neg = (r[0] >> 63) != 0;
-  if (neg) {
-    uint64_t c = 1;
-    for (int i = 2; i >= 0; --i) {
-      r[i] = ~r[i] + c;
-      c = (c && r[i] == 0) ? 1 : 0;
-    }
-  }
-  bool ex2;
-  const double lo = round_u192(r, &ex2);
-  dd[0] = hi;
-  dd[1] = neg ? -lo : lo;
-  return CLB_OK;
-}
-
 // ------------------------------------------------------------------ work mode helpers
 
 // Sum of bitlen(10 x) over x in [a, b] (a >= 1): the exponent lengths 10 (d - k) of the head
@@ -943,106 +770,11 @@ double job_squares(uint64_t d, uint64_t k_begin, uint64_t k_end) {
   return c;
 }
 
-bool file_exists(const std::string& p) {
-  struct stat st;
-  return stat(p.c_str(), &st) == 0;
-}
-
-int read_text(const std::string& path, std::string* out) {
-  FILE* f = fopen(path.c_str(), "rb");
-  if (!f) return fail(CLB_EIO, "cannot open %s: %s", path.c_str(), strerror(errno));
-  out->clear();
-  char buf[4096];
-  size_t n;
-  while ((n = fread(buf, 1, sizeof buf, f)) > 0) out->append(buf, n);
-  fclose(f);
-  return CLB_OK;
-}
-
-std::vector<std::string> split_lines(const std::string& text) {
-  std::vector<std::string> lines;
-  std::string cur;
-  for (char c : text) {
-    if (c == '\n') {
-      if (!cur.empty() && cur.back() == '\r') cur.pop_back();
-      lines.push_back(cur);
-      cur.clear();
-    } else {
-      cur.push_back(c);
-    }
-  }
-  if (!cur.empty() && cur.back() == '\r') cur.pop_back();
-  if (!cur.empty()) lines.push_back(cur);
-  return lines;
-}
-
-void strrep(std::string* s, const std::string& from) {
-  size_t p;
-  while (!from.empty() && (p = s->find(from)) != std::string::npos) s->erase(p, from.size());
-}
-
-// HSP int() / int64() of a string: optional sign, then leading decimal digits (0 if none).
-int64_t hsp_int(const std::string& s) {
-  size_t i = 0;
-  bool neg = false;
-  if (i < s.size() && (s[i] == '-' || s[i] == '+')) neg = s[i++] == '-';
-  int64_t v = 0;
-  for (; i < s.size() && s[i] >= '0' && s[i] <= '9'; ++i) {
-    if (v > (INT64_MAX - 9) / 10) break;
-    v = v * 10 + (s[i] - '0');
-  }
-  return neg ? -v : v;
-}
-
-int make_dir(const std::string& p) {
-  if (mkdir(p.c_str(), 0755) == 0 || errno == EEXIST) return CLB_OK;
-  return fail(CLB_EIO, "cannot create %s: %s", p.c_str(), strerror(errno));
-}
-
-// Internal format code: the 3 words hold a double-double's bit patterns (dd_words), as the
-// 106-bit-mode replay returns its results.
-constexpr int kDDWords = 3;
-
-void format_result(int precision, const uint64_t w[3], char hex[49]) {
-  if (precision == kDDWords) {
-    double dd[2];
-    memcpy(dd, w, sizeof dd);
-    hex_dd(dd, hex);
-  } else if (precision == CLB_PREC_DD106) {
-    double dd[2];
-    u192_to_dd(w, dd);
-    hex_dd(dd, hex);
-  } else {
-    hex48(w, hex);
-  }
-}
-
-// fsave, Bellard_192.hsp:514-533: hex line, device line, "<ms>ms"; HSP's "\n" is CR LF.
-int write_work_file(const std::string& work_dir, int j, int64_t b, const uint64_t w[3],
-                    int precision, const std::string& label, long long ms) {
-  if (int rc = make_dir(work_dir)) return rc;
-  const std::string dir = work_dir + "/" + std::to_string(j);
-  if (int rc = make_dir(dir)) return rc;
-  const std::string path = dir + "/" + std::to_string(j) + "_" + std::to_string(b) + ".txt";
-  char hex[49];
-  format_result(precision, w, hex);
-  const std::string body = std::string(hex) + "\r\n" + label + "\r\n" + std::to_string(ms) + "ms";
-  FILE* f = fopen(path.c_str(), "wb");
-  if (!f) return fail(CLB_EIO, "cannot write %s: %s", path.c_str(), strerror(errno));
-  const bool ok = fwrite(body.data(), 1, body.size(), f) == body.size();
-  if (fclose(f) != 0 || !ok) return fail(CLB_EIO, "write failed: %s", path.c_str());
-  return CLB_OK;
-}
-
 }  // namespace
 
 // ====================================================================== C ABI
 
 extern "C" {
-
-const char* clb_last_error(void) { return g_last_error.c_str(); }
-
-const char* clb_version(void) { return "clbellard-mi355x 0.1 (gfx950)"; }
 
 int clb_get_dev_count(int* count) {
   if (!count) return fail(CLB_EINVAL, "null count");
@@ -1350,19 +1082,6 @@ int clb_set_schedule(clb_ctx* ctx, int waves_per_group, int min_chunk, int chunk
   ctx->min_chunk = min_chunk ? min_chunk : 256;
   ctx->chunks_per_wave = chunks_per_wave < 0 ? 0 : chunks_per_wave;
   ctx->max_replicas = max_replicas ? max_replicas : 64;
-  return CLB_OK;
-}
-
-int clb_normalize(uint64_t n_in, uint64_t* n_out, uint64_t* d_out, int* overdouble) {
-  return normalize(n_in, n_out, d_out, overdouble);
-}
-
-int clb_formula(int j, uint64_t* numer, int64_t* numesign, uint64_t* den0, uint64_t* den1) {
-  if (j < 0 || j > 6) return fail(CLB_EINVAL, "formula %d (need 0..6)", j);
-  if (numer) *numer = kNumer[j];
-  if (numesign) *numesign = kNumeSign[j];
-  if (den0) *den0 = kDen0[j];
-  if (den1) *den1 = kDen1[j];
   return CLB_OK;
 }
 
@@ -1682,79 +1401,10 @@ int clb_run_jobs_noalloc(clb_ctx* ctx, const clb_job* jobs_in, size_t count, uin
   return CLB_OK;
 }
 
-int clb_hex48(const uint64_t w[3], char hex[49]) {
-  if (!w || !hex) return fail(CLB_EINVAL, "null pointer");
-  hex48(w, hex);
-  return CLB_OK;
-}
-
-int clb_hex_dd(const double dd[2], char hex[49]) {
-  if (!dd || !hex) return fail(CLB_EINVAL, "null pointer");
-  hex_dd(dd, hex);
-  return CLB_OK;
-}
-
-int clb_u192_to_dd(const uint64_t w[3], double dd[2]) {
-  if (!w || !dd) return fail(CLB_EINVAL, "null pointer");
-  return u192_to_dd(w, dd);
-}
-
 int clb_last_timing(clb_ctx* ctx, double* head_kernel_ms, double* total_ms) {
   if (int rc = check_ctx(ctx)) return rc;
   if (head_kernel_ms) *head_kernel_ms = ctx->head_ms;
   if (total_ms) *total_ms = ctx->total_ms;
-  return CLB_OK;
-}
-
-// *chktxtstr (Bellard_192.hsp:176-207), *workStart (:221-227), *workStart2 (:229-247).
-struct Worklist {
-  int64_t digits = 0;
-  int precision = -1;
-  std::vector<std::pair<int, int64_t>> jobs;
-};
-
-static int parse_worklist(const char* path, Worklist* wl) {
-  std::string text;
-  if (int rc = read_text(path, &text)) return rc;
-  const std::vector<std::string> lines = split_lines(text);
-  if (lines.size() < 2) return fail(CLB_EFORMAT, "worklist needs digits= and precision= lines");
-  std::string digits = lines[0];
-  strrep(&digits, "digits");
-  strrep(&digits, "=");
-  strrep(&digits, " ");
-  int64_t nd = hsp_int(digits);
-  if (nd < 0) nd = 0;
-  if ((uint64_t)nd > kMaxDigits) nd = (int64_t)kMaxDigits;
-  std::string prec = lines[1];
-  strrep(&prec, "precision");
-  strrep(&prec, "=");
-  strrep(&prec, " ");
-  wl->digits = nd;
-  wl->precision = (!prec.empty() && prec[0] >= '0' && prec[0] <= '2') ? prec[0] - '0' : -1;
-  wl->jobs.clear();
-  for (size_t li = 2; li < lines.size(); ++li) {
-    const std::string& t = lines[li];
-    if (t.empty()) continue;  // a blank line is not a job
-    const int j = (int)hsp_int(t.substr(0, 1));
-    const int64_t b = hsp_int(t.size() > 2 ? t.substr(2) : std::string());
-    if (j < 0 || j > 6 || b < 0) return fail(CLB_EFORMAT, "bad job line '%s'", t.c_str());
-    wl->jobs.push_back({j, b});
-  }
-  return CLB_OK;
-}
-
-int clb_parse_worklist(const char* worklist_path, uint64_t* digits, int* precision,
-                       int64_t* jobs, size_t cap, size_t* njobs) {
-  if (!worklist_path) return fail(CLB_EINVAL, "null path");
-  Worklist wl;
-  if (int rc = parse_worklist(worklist_path, &wl)) return rc;
-  if (digits) *digits = (uint64_t)wl.digits;
-  if (precision) *precision = wl.precision;
-  if (njobs) *njobs = wl.jobs.size();
-  for (size_t i = 0; jobs && i < wl.jobs.size() && i < cap; ++i) {
-    jobs[2 * i] = wl.jobs[i].first;
-    jobs[2 * i + 1] = wl.jobs[i].second;
-  }
   return CLB_OK;
 }
 
@@ -1847,55 +1497,6 @@ int clb_set_worklist_launch(clb_ctx* ctx, uint32_t max_jobs, double max_seconds,
   ctx->wl_max_jobs = max_jobs;
   ctx->wl_max_seconds = max_seconds;
   ctx->wl_max_launches = max_launches;
-  return CLB_OK;
-}
-
-int clb_write_work_file(const char* work_dir, int j, int64_t b, const uint64_t w[3],
-                        int precision, const char* device_label, int64_t ms) {
-  if (!work_dir || !w) return fail(CLB_EINVAL, "null pointer");
-  if (j < 0 || j > 6 || b < 0) return fail(CLB_EINVAL, "bad job %d_%lld", j, (long long)b);
-  if (precision < 0 || precision > 2) return fail(CLB_EINVAL, "precision must be 0, 1 or 2");
-  return write_work_file(work_dir, j, b, w, precision, device_label ? device_label : "", ms);
-}
-
-int clb_work_sum(const char* work_dir, char hex[49], int counts[7]) {
-  if (!work_dir || !hex) return fail(CLB_EINVAL, "null pointer");
-  int sum[48] = {0};
-  for (int j = 0; j < 7; ++j) {
-    int used = 0;
-    for (long long b = 0;; ++b) {
-      char name[64];
-      snprintf(name, sizeof name, "/%d/%d_%lld.txt", j, j, b);
-      const std::string path = std::string(work_dir) + name;
-      if (!file_exists(path)) break;  // workmodeSum0_6.hsp:14-16: stop at the first gap
-      std::string text;
-      if (int rc = read_text(path, &text)) return rc;
-      const std::vector<std::string> lines = split_lines(text);
-      if (lines.empty() || lines[0].size() < 48)
-        return fail(CLB_EFORMAT, "%s: first line is not 48 hex digits", path.c_str());
-      const std::string& line = lines[0];
-      for (int c = 0; c < 48; ++c) {  // :21-28, least significant digit first
-        const char ch = line[47 - c];
-        int v;
-        if (ch >= '0' && ch <= '9')
-          v = ch - '0';
-        else if (ch >= 'a' && ch <= 'f')
-          v = ch - 'a' + 10;
-        else
-          return fail(CLB_EFORMAT, "%s: '%c' is not a lowercase hex digit", path.c_str(), ch);
-        sum[47 - c] += v;
-        if (sum[47 - c] >= 16) {
-          sum[47 - c] -= 16;
-          if (c != 47) sum[46 - c]++;
-        }
-      }
-      ++used;
-    }
-    if (counts) counts[j] = used;
-  }
-  static const char hx[] = "0123456789abcdef";
-  for (int i = 0; i < 48; ++i) hex[i] = hx[sum[i]];
-  hex[48] = 0;
   return CLB_OK;
 }
 

@@ -200,7 +200,8 @@ int clb_run_jobs_noalloc(clb_ctx *ctx, const clb_job *jobs, size_t count, uint64
 
 /* hexadecimal_i32 with the host word swap (Bellard_192.hsp:354-359, hexsademical.as:104-127). */
 int clb_hex48(const uint64_t w[3], char hex[49]);
-/* hexadecimal_dd (hexsademical.as:21-101): double-double -> 28 hex digits + 20 zeros. */
+/* hexadecimal_dd (hexsademical.as:21-101): double-double -> 28 hex digits + 20 zeros.  CLB_EINVAL
+ * for a non-finite part or one outside [-1, 1] (the reference's values lie in [-1, 1)). */
 int clb_hex_dd(const double dd[2], char hex[49]);
 /* The double-double (hi, lo) nearest to the 192-bit fraction w (round to nearest, 106 bits). */
 int clb_u192_to_dd(const uint64_t w[3], double dd[2]);

@@ -997,6 +997,12 @@ static void dd_add(double *h, double *l, double bh, double bl) {
 /* hexsademical.as:21-101.  outdd is six 32-bit HSP integers viewed as 24 little-endian bytes;
  * bytes 23..10 receive the 14 emitted base-256 digits, with carries and borrows propagated
  * into the already emitted (more significant) bytes. */
+/* HSP int() of a double (x86 cvttsd2si): INT_MIN for NaN and out-of-range values. */
+static int hsp_int_of(double v) {
+  if (!(v > -2147483649.0 && v < 2147483648.0)) return INT32_MIN;
+  return (int)v;
+}
+
 void orc_hex_dd(const double dd[2], char out[49]) {
   double h = dd[0], l = dd[1];
   uint32_t outdd[6] = {0, 0, 0, 0, 0, 0};
@@ -1005,7 +1011,7 @@ void orc_hex_dd(const double dd[2], char out[49]) {
     h *= 256.0;
     l *= 256.0;
     dd_twosum(h, l, &h, &l);
-    int moji = (int)h;
+    int moji = hsp_int_of(h);
     if (moji < 256 && h >= 0.0) {
       ((uint8_t *)outdd)[23 - cnt] = (uint8_t)moji;
       h -= moji;
@@ -1030,7 +1036,7 @@ void orc_hex_dd(const double dd[2], char out[49]) {
         h -= 256.0;
       }
       if (h < 0.0) {
-        moji = (int)(h + 256.0);
+        moji = hsp_int_of(h + 256.0);
         int idx = 23 - cnt + 1;
         if (idx < 24) {
           int bidx = idx / 4;
