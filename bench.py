@@ -86,6 +86,34 @@ def int32_word_products(d):
     return 7 * 10 * (squares + 3 * d)
 
 
+# 32x32-bit multiply instructions (v_mad_u64_u32, v_mul_lo_u32, v_mul_hi_u32) of the plain
+# kernel, counted in its gfx950 ISA (k_head<3, 16>): 36 per 3-bit window of three Montgomery
+# squares, 47 per term outside the window loop (the first square, neg_inv64, the three
+# exact-division quotient words, the carried 2^64 mod m).
+PLAIN_MULS_PER_SQUARE = 12
+PLAIN_MULS_PER_TERM = 47
+
+
+def plain_multiplies(d, k0, k1):
+    """Multiply instructions the plain kernel issues for the head terms k in [k0, k1) of one
+    formula: its Montgomery exponent is e + 192 = 10 (d - k) + 192, taken in 3-bit windows from
+    the top (one square for the top window, three for each further one)."""
+    total = 0
+    lo_x, hi_x = d - k1 + 1, d - k0  # x = d - k
+    bl = 1
+    while True:
+        # x with bitlen(10 x + 192) == bl: 10 x + 192 in [2^(bl-1), 2^bl - 1]
+        a = max(lo_x, -(-((1 << (bl - 1)) - 192) // 10))
+        b = min(hi_x, ((1 << bl) - 1 - 192) // 10)
+        if a > hi_x:
+            break
+        if b >= a:
+            squares = 1 + 3 * ((bl + 2) // 3 - 1)
+            total += (b - a + 1) * (PLAIN_MULS_PER_SQUARE * (squares - 1) + PLAIN_MULS_PER_TERM)
+        bl += 1
+    return total
+
+
 def cpu_model() -> str:
     """The host CPU's model name and logical CPU count, as /proc/cpuinfo gives them."""
     try:
@@ -189,7 +217,11 @@ def measure_frac_err(clb, orc, ctx, d, starts, span, threads):
         else:
             modes[f"replay_p{prec}"] = [clb.u192_to_int(w) for w in res]
     terms = 7 * span * len(starts)
-    out = {}
+    out = {"what": "signed error of each mode's sum against the 256-bit oracle, log2 of |error| "
+                   "as a fraction: the largest of the 448 sampled windows, the sum over all of "
+                   "them, the mean per term and formula, and the whole-position error "
+                   "extrapolated linearly from the sample (an upper estimate where rounding "
+                   "errors of both signs cancel, as in the replay modes)"}
     for name, vals in modes.items():
         errs = []
         for v, x in zip(vals, exact256):
@@ -206,7 +238,7 @@ def measure_frac_err(clb, orc, ctx, d, starts, span, threads):
         def lg(u):
             return round(math.log2(abs(u)) - 256, 2) if u else None
         out[name] = {"max_window_log2": lg(worst), "sample_sum_log2": lg(total),
-                     "mean_per_term_units_2^-192": round(total / terms / 2**64, 4),
+                     "mean_per_term_formula_log2": lg(total / terms),
                      "full_position_estimate_log2": lg(full)}
     out["plain"] = "bit-identical to exact"
     return out, cpu_s, terms
@@ -595,7 +627,8 @@ def main():
     split_win_s = ctx.last_timing()[1] / 1e3
     ctx.set_kernel(clb.KERNEL_PLAIN)
     plain_parts = ctx.run_jobs(jobs)
-    plain_win_s = ctx.last_timing()[1] / 1e3
+    plain_head_s, plain_win_s = (x / 1e3 for x in ctx.last_timing())
+    plain_muls = 7 * sum(plain_multiplies(d, s, s + span) for s in windows)
     if plain_parts != split_parts:
         raise SystemExit("plain and split kernels disagree on the seeded windows")
     log(f"windows: split {win_terms / split_win_s:.3e} terms/s, plain "
@@ -685,7 +718,15 @@ def main():
             "modular_squares_per_s": 7 * (sum_bitlen_10x(d) - d) / head_avg_s,
         },
         "plain_kernel": {"value": win_terms / plain_win_s, "unit": "terms/s",
-                         "windows": f"{args.windows} x 2^{args.window_log2} k per formula, seed 2"},
+                         "windows": f"{args.windows} x 2^{args.window_log2} k per formula, seed 2",
+                         "int_mul_issue": {
+                             "what": "32x32-bit multiply instructions per second of the plain "
+                                     "kernel on the windows (12 per Montgomery square, 47 per term "
+                                     "besides, counted in its ISA) against the full-rate multiply "
+                                     "issue peak (256 CUs x 64 lanes x 2.4 GHz)",
+                             "achieved": plain_muls / plain_head_s,
+                             "peak": PEAK_INT32_MUL_PER_S, "unit": "multiplies/s",
+                             "frac": plain_muls / plain_head_s / PEAK_INT32_MUL_PER_S}},
         "split_kernel_windows": {"value": win_terms / split_win_s, "unit": "terms/s"},
         "split_vs_plain": plain_win_s / split_win_s,
         "cpu_baseline": cpu,
