@@ -87,11 +87,12 @@ def int32_word_products(d):
 
 
 # 32x32-bit multiply instructions (v_mad_u64_u32, v_mul_lo_u32, v_mul_hi_u32) of the plain
-# kernel, counted in its gfx950 ISA (k_head<3, 16>): 36 per 3-bit window of three Montgomery
-# squares, 47 per term outside the window loop (the first square, neg_inv64, the three
-# exact-division quotient words, the carried 2^64 mod m).
-PLAIN_MULS_PER_SQUARE = 12
-PLAIN_MULS_PER_TERM = 47
+# kernel, counted in its gfx950 ISA (k_head<3, 16> and its device functions compiled alone): 31
+# per 3-bit window of three Montgomery squares (10 per plain square, 11 with the window digit's
+# shift), 46 per term besides (the first square 11, -m^-1 10, the three exact-division quotient
+# words 24, the carried 2^64 mod m 1).
+PLAIN_MULS_PER_WINDOW = 31
+PLAIN_MULS_PER_TERM = 46
 
 
 def plain_multiplies(d, k0, k1):
@@ -108,8 +109,8 @@ def plain_multiplies(d, k0, k1):
         if a > hi_x:
             break
         if b >= a:
-            squares = 1 + 3 * ((bl + 2) // 3 - 1)
-            total += (b - a + 1) * (PLAIN_MULS_PER_SQUARE * (squares - 1) + PLAIN_MULS_PER_TERM)
+            windows = (bl + 2) // 3
+            total += (b - a + 1) * (PLAIN_MULS_PER_WINDOW * (windows - 1) + PLAIN_MULS_PER_TERM)
         bl += 1
     return total
 
@@ -721,9 +722,10 @@ def main():
                          "windows": f"{args.windows} x 2^{args.window_log2} k per formula, seed 2",
                          "int_mul_issue": {
                              "what": "32x32-bit multiply instructions per second of the plain "
-                                     "kernel on the windows (12 per Montgomery square, 47 per term "
-                                     "besides, counted in its ISA) against the full-rate multiply "
-                                     "issue peak (256 CUs x 64 lanes x 2.4 GHz)",
+                                     "kernel on the windows (31 per 3-bit window of Montgomery "
+                                     "squares, 46 per term besides, counted in its ISA) against "
+                                     "the full-rate multiply issue peak (256 CUs x 64 lanes x "
+                                     "2.4 GHz)",
                              "achieved": plain_muls / plain_head_s,
                              "peak": PEAK_INT32_MUL_PER_S, "unit": "multiplies/s",
                              "frac": plain_muls / plain_head_s / PEAK_INT32_MUL_PER_S}},

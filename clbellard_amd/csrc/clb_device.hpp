@@ -245,6 +245,46 @@ __device__ __forceinline__ uint64_t isq(uint64_t h, uint64_t m, uint64_t mp) {
   return redc(mulhi64(h, h), h * h, m, mp);
 }
 
+// The same two squares for m < 2^55 and lazy h < 2m, written as chained 32x32+64 multiply-adds
+// (v_mad_u64_u32) on 32-bit words: h < 2^56 has an upper word below 2^24 and m one below 2^23, so
+// no column carries out of 64 bits and the REDC needs no 128-bit intermediate.  The product
+// T = h (h 2^c) (four words t0..t3, c <= 7) takes 4 products, 3 for the plain square
+// (h0 (2 h1) folds the doubled cross term); q = (t1:t0) mp mod 2^64 three; the high half of q m
+// four.  REDC(T) = (t3:t2) + hi(q m) + [(t1:t0) != 0], since (q m) mod 2^64 = 2^64 - (t1:t0)
+// whenever (t1:t0) != 0.  Ten or eleven multiplies instead of twelve, and few 64-bit adds.
+__device__ __forceinline__ uint64_t redc_words(uint32_t t0, uint32_t t1, uint32_t t2, uint32_t t3,
+                                              uint64_t m, uint64_t mp) {
+  const uint32_t m0 = (uint32_t)m, m1 = (uint32_t)(m >> 32);
+  const uint32_t mp0 = (uint32_t)mp, mp1 = (uint32_t)(mp >> 32);
+  const uint64_t qa = (uint64_t)t0 * mp0;
+  const uint32_t q0 = (uint32_t)qa;
+  const uint32_t q1 = (uint32_t)(qa >> 32) + t0 * mp1 + t1 * mp0;
+  const uint64_t e = (uint64_t)q0 * m0;
+  const uint64_t f = (uint64_t)q1 * m0 + (e >> 32);
+  const uint64_t g = (uint64_t)q0 * m1 + (uint32_t)f;        // m1 < 2^23
+  const uint64_t hi = (uint64_t)q1 * m1 + (f >> 32) + (g >> 32);
+  return (((uint64_t)t3 << 32) | t2) + hi + ((t0 | t1) != 0);
+}
+
+__device__ __forceinline__ uint64_t isq55(uint64_t h, uint64_t m, uint64_t mp) {
+  const uint32_t h0 = (uint32_t)h, h1 = (uint32_t)(h >> 32);  // h1 < 2^24
+  const uint64_t a = (uint64_t)h0 * h0;
+  const uint64_t x = (uint64_t)h0 * (h1 + h1) + (a >> 32);
+  const uint64_t y = (uint64_t)h1 * h1 + (x >> 32);
+  return redc_words((uint32_t)a, (uint32_t)x, (uint32_t)y, (uint32_t)(y >> 32), m, mp);
+}
+
+__device__ __forceinline__ uint64_t isq55_shift(uint64_t h, int c, uint64_t m, uint64_t mp) {
+  const uint64_t gw = h << c;  // < 2^63
+  const uint32_t h0 = (uint32_t)h, h1 = (uint32_t)(h >> 32);
+  const uint32_t g0 = (uint32_t)gw, g1 = (uint32_t)(gw >> 32);
+  const uint64_t a = (uint64_t)h0 * g0;
+  const uint64_t x = (uint64_t)h0 * g1 + (a >> 32);
+  const uint64_t x2 = (uint64_t)h1 * g0 + (uint32_t)x;
+  const uint64_t y = (uint64_t)h1 * g1 + (x >> 32) + (x2 >> 32);
+  return redc_words((uint32_t)a, (uint32_t)x2, (uint32_t)y, (uint32_t)(y >> 32), m, mp);
+}
+
 // Montgomery form of 2^E (= 2^E * 2^64 mod m) in [0, 2m) for m < 2^55: W-bit windows over the
 // low bl bits of E (bl uniform per wave, E < 2^bl), starting from 2^64 mod m, the Montgomery form
 // of 1; leading zero windows leave it unchanged.
@@ -252,11 +292,11 @@ template <int W>
 __device__ __forceinline__ uint64_t ipow2_mont(uint64_t E, int bl, uint64_t m, uint64_t mp,
                                                uint64_t one) {
   int pos = ((bl + W - 1) / W) * W - W;  // start of the top (possibly partial) window
-  uint64_t h = isq_shift(one, (int)(E >> pos), m, mp);
+  uint64_t h = isq55_shift(one, (int)(E >> pos), m, mp);
   for (pos -= W; pos >= 0; pos -= W) {
 #pragma unroll
-    for (int i = 0; i < W - 1; ++i) h = isq(h, m, mp);
-    h = isq_shift(h, (int)((E >> pos) & ((1u << W) - 1)), m, mp);
+    for (int i = 0; i < W - 1; ++i) h = isq55(h, m, mp);
+    h = isq55_shift(h, (int)((E >> pos) & ((1u << W) - 1)), m, mp);
   }
   return h;
 }

@@ -194,7 +194,7 @@ k_peak_isq(uint64_t iters, uint64_t* clk, uint64_t* sink) {
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u)
 #pragma unroll
-      for (int c = 0; c < kSqChains; ++c) h[c] = isq(h[c], m, mp);
+      for (int c = 0; c < kSqChains; ++c) h[c] = isq55(h[c], m, mp);
   }
   uint64_t s = 0;
 #pragma unroll

@@ -388,3 +388,56 @@ def test_montgomery_one_carried_along_the_stride():
                 one -= t
             else:
                 one, qq = one + m - t, qq - 1
+
+
+def _redc_words(t0, t1, t2, t3, m, mp):
+    """redc_words of clb_device.hpp, word by word with the kernel's 32/64-bit wraps."""
+    m0, m1, mp0, mp1 = m & 0xFFFFFFFF, m >> 32, mp & 0xFFFFFFFF, mp >> 32
+    qa = t0 * mp0
+    q0 = qa & 0xFFFFFFFF
+    q1 = ((qa >> 32) + t0 * mp1 + t1 * mp0) & 0xFFFFFFFF
+    e = q0 * m0
+    f = q1 * m0 + (e >> 32)
+    g = q0 * m1 + (f & 0xFFFFFFFF)
+    hi = q1 * m1 + (f >> 32) + (g >> 32)
+    assert max(qa, e, f, g, hi) <= M64
+    return (((t3 << 32) | t2) + hi + (1 if (t0 | t1) else 0)) & M64
+
+
+def _isq55_shift(h, c, m, mp):
+    gw = (h << c) & M64
+    h0, h1, g0, g1 = h & 0xFFFFFFFF, h >> 32, gw & 0xFFFFFFFF, gw >> 32
+    a = h0 * g0
+    x = h0 * g1 + (a >> 32)
+    x2 = h1 * g0 + (x & 0xFFFFFFFF)
+    y = h1 * g1 + (x >> 32) + (x2 >> 32)
+    assert max(a, x, x2, y) <= M64
+    return _redc_words(a & 0xFFFFFFFF, x2 & 0xFFFFFFFF, y & 0xFFFFFFFF, y >> 32, m, mp)
+
+
+def _isq55(h, m, mp):
+    h0, h1 = h & 0xFFFFFFFF, h >> 32
+    a = h0 * h0
+    x = h0 * (h1 + h1) + (a >> 32)
+    y = h1 * h1 + (x >> 32)
+    assert max(a, x, y) <= M64
+    return _redc_words(a & 0xFFFFFFFF, x & 0xFFFFFFFF, y & 0xFFFFFFFF, y >> 32, m, mp)
+
+
+def test_montgomery_square_words_is_redc():
+    """The integer engine's squares for m < 2^55 (chained 32x32+64 multiply-adds): with lazy
+    residues h < 2m and window digits c <= 7, REDC(h^2 2^c) = h^2 2^c 2^-64 mod m (up to one m),
+    the result stays below 2m, and no multiply-add leaves 64 bits."""
+    rng = random.Random(55)
+    for _ in range(20000):
+        bits = rng.choice([3, 20, 33, 41, 42, 48, 54, 55])
+        m = rng.randrange(1 << (bits - 1), 1 << bits) | 1
+        if m >= 1 << 55:
+            m -= 2
+        mp = (-pow(m, -1, 1 << 64)) & M64
+        h = rng.choice([rng.randrange(0, 2 * m), 2 * m - 1, 0, 1, m, m - 1])
+        c = rng.randrange(0, 8)
+        r = _isq55_shift(h, c, m, mp)
+        assert r < 2 * m and r % m == (h * h << c) * pow(2, -64, m) % m, (m, h, c)
+        r0 = _isq55(h, m, mp)
+        assert r0 < 2 * m and r0 % m == h * h * pow(2, -64, m) % m, (m, h)
