@@ -8,7 +8,8 @@ Python mirror of the C ABI in include/clbellard.h (libclbellard.so, built in thi
 * ``Context.set0mem / batch_calc /
   add_ans / read_partial / read_ans``  -- Set0mem*, BatchCalc, AddAns, HCLReadBuffer
                                           (KernelFuncs.as:15-74, Bellard_192.hsp:264-296)
-* ``Context.sglobal / sglobal_after``  -- the Sglobal*_192* / Sglobal_after_192 kernels
+* ``Context.sglobal / sglobal_after /
+  ssum``                               -- the Sglobal* / Sglobal_after* / Ssum kernels
 * ``Context.normal``                   -- Allloop (Bellard_192.hsp:264-296)
 * ``Context.run_worklist``, ``work_sum`` -- work mode (Bellard_192.hsp:88-340, 514-533) and
                                           workmodeSum0_6.hsp
@@ -97,6 +98,7 @@ _SIGNATURES = {
     "clb_clear_ans": ([_CTX], ctypes.c_int),
     "clb_sglobal": ([_CTX, _U64, _U64, _U64, ctypes.c_int64, _U64, _U64], ctypes.c_int),
     "clb_sglobal_after": ([_CTX, _U64, ctypes.c_int64, _U64, _U64], ctypes.c_int),
+    "clb_ssum": ([_CTX], ctypes.c_int),
     "clb_normal": ([_CTX, _U64, _P64, ctypes.c_char_p, _P64], ctypes.c_int),
     "clb_batch_positions": ([_CTX, _P64, ctypes.c_size_t, _P64], ctypes.c_int),
     "clb_run_jobs": ([_CTX, ctypes.POINTER(_Job), ctypes.c_size_t, _P64], ctypes.c_int),
@@ -357,6 +359,9 @@ class Context:
 
     def sglobal_after(self, d: int, numesign: int, den0: int, den1: int) -> None:
         _check(lib.clb_sglobal_after(self._ctx, d, numesign, den0, den1))
+
+    def ssum(self) -> None:
+        _check(lib.clb_ssum(self._ctx))
 
     # --- whole computations
     def normal(self, n: int) -> Tuple[Tuple[int, int, int], str, int]:

@@ -43,6 +43,7 @@ struct HostJob {
   uint32_t slot;
   bool tail;
   bool single = false;  // one kernel launch over [k0, k1) (clb_sglobal), not BatchCalc's chunks
+  bool ssum_op = false;  // clb_ssum in the 106-bit-mode replay: only Ssum's two trees
 };
 
 // Device buffer that grows geometrically, so that its address (which a captured graph records)
@@ -403,9 +404,12 @@ void dd_words(const double* dd, uint64_t w[3]) {
 
 // Reference replay of the 106-bit mode (precision 0): each BatchCalc job on its own bigSumMem
 // (cleared, as Set0memdd before every BatchCalc of Allloop and work mode, Bellard_192.hsp:270,
-// :311), or, with accumulate_dev (clb_batch_calc), the one job on the context's bigSumMem with
+// :311), or, with accumulate_dev (the operators), the one job on the context's bigSumMem with
 // bigSum[0] copied to accumulate_dev; then AddAns per group (Sum7 in formula order).  Results are
-// double-doubles, in the 3-word outputs as dd_words.
+// double-doubles, in the 3-word outputs as dd_words.  Operator jobs: BatchCalc (clb_batch_calc:
+// its launches, the tail where the range reaches d, Ssum), one Sglobal32mtg / Sglobal64mtg launch
+// (clb_sglobal: single), Sglobal_after (clb_sglobal_after: single with tail and no head terms),
+// Ssum alone (clb_ssum: ssum_op).
 int run_replay106(clb_ctx* ctx, const std::vector<HostJob>& hjobs,
                   const std::vector<uint8_t>& shift, const std::vector<GroupDev>& groups,
                   uint64_t* out_host, uint64_t* accumulate_dev, uint64_t* slot_host,
@@ -421,9 +425,11 @@ int run_replay106(clb_ctx* ctx, const std::vector<HostJob>& hjobs,
     if (h.d > ((1ull << 53) - 10) / 10)
       return fail(CLB_ERANGE, "replay: d = %llu needs the reference's OVERDOUBLE build, which "
                   "is not replayed", (unsigned long long)h.d);
-    if (h.single)
-      return fail(CLB_EINVAL, "replay of the 106-bit mode: kernel-level operators are not "
-                  "replayed (use clb_batch_calc)");
+    const uint64_t x32 = ((1ull << 32) - h.den1 - 1) / h.den0 + 1;  // KernelFuncs.as:31-33
+    if (h.single && h.k0 < x32 && x32 < h.k1)
+      return fail(CLB_EINVAL, "replay: one Sglobal32mtg launch over [%llu, %llu) passes "
+                  "loopmax32 = %llu (the reference's 32-bit kernel truncates larger moduli)",
+                  (unsigned long long)h.k0, (unsigned long long)h.k1, (unsigned long long)x32);
     if (h.k0 < h.k1 && h.den0 * (h.k1 - 1) + h.den1 > max_m) max_m = h.den0 * (h.k1 - 1) + h.den1;
   }
   if (op_level && nslots != 1) return fail(CLB_EINVAL, "replay: one job per operator call");
@@ -461,6 +467,8 @@ int run_replay106(clb_ctx* ctx, const std::vector<HostJob>& hjobs,
     j.part = (uint64_t)(uintptr_t)(dev + off_parts + 16 * i);
     j.numesign = h.numesign;
     j.tail = h.tail;
+    j.single = h.single;
+    j.ssum = !h.single || h.ssum_op;
   }
   memcpy(stage + off_jobs, jobs.data(), nslots * sizeof(R106JobDev));
   for (size_t g = 0; g < ngr; ++g) {
@@ -478,13 +486,18 @@ int run_replay106(clb_ctx* ctx, const std::vector<HostJob>& hjobs,
     if (!op_level)
       HIP_TRY(launch_zero(reinterpret_cast<unsigned long long*>(ctx->r106_bigsum.p),
                           n * kBigSumBytes / 8, s));
-    uint64_t most = 0;  // work-items of the chunk's largest launch
+    uint64_t most = 0;  // work-groups of the chunk's largest launch
     for (size_t i = c; i < c + n; ++i) {
-      const uint64_t span = hjobs[i].k1 > hjobs[i].k0 ? hjobs[i].k1 - hjobs[i].k0 : 0;
+      const HostJob& h = hjobs[i];
+      const uint64_t span = h.k1 > h.k0 ? h.k1 - h.k0 : 0;
       const uint64_t items = span < gws ? span : gws;
-      if (items > most) most = items;
+      // an empty clb_sglobal still runs group 0 (its sumfix2); Sglobal_after and Ssum run none
+      const uint64_t groups = h.ssum_op || (h.single && h.tail) ? 0
+                              : h.single && span == 0      ? 1
+                                                           : (items + 255) / 256;
+      if (groups > most) most = groups;
     }
-    HIP_TRY(launch_replay106(kind, djobs + c, (uint32_t)n, gws, (uint32_t)((most + 255) / 256), s));
+    HIP_TRY(launch_replay106(kind, djobs + c, (uint32_t)n, gws, (uint32_t)most, s));
   }
   HIP_TRY(hipEventRecord(ctx->ev[1], s));
   const double* dparts = reinterpret_cast<const double*>(dev + off_parts);
@@ -1156,6 +1169,21 @@ int clb_sglobal_after(clb_ctx* ctx, uint64_t d, int64_t numesign, uint64_t den0,
   h.numesign = (uint32_t)numesign;
   h.tail = true;
   h.single = true;
+  return run_into_partial(ctx, h);
+}
+
+int clb_ssum(clb_ctx* ctx) {
+  if (int rc = check_ctx(ctx)) return rc;
+  // Outside the 106-bit-mode replay the operators add straight into the partial (Ssum_192's
+  // 192-bit sums are exact, so their order does not matter): nothing to do.
+  if (!(ctx->replay_gws && ctx->precision == CLB_PREC_DD106)) return CLB_OK;
+  HostJob h{};
+  h.d = 1;
+  h.den0 = 2;
+  h.den1 = 1;
+  h.tail = false;
+  h.single = true;
+  h.ssum_op = true;
   return run_into_partial(ctx, h);
 }
 

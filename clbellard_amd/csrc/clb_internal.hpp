@@ -166,8 +166,12 @@ struct R106JobDev {
   uint64_t bigsum;       // device address of the job's kRefGroups accumulators (hi, lo doubles)
   uint64_t part;         // device address receiving bigSum[0] after Ssum (hi, lo)
   uint32_t numesign, tail;
+  uint32_t single;       // 1: [k0, k1) is one launch of one kernel (Sglobal32mtg if m(k0) < 2^32,
+                         // else Sglobal64mtg), not BatchCalc's chunks: the operator clb_sglobal
+  uint32_t ssum;         // 1: the two Ssum trees after the tail (BatchCalc; clb_ssum)
 };
 // kind: kSplitW3/W2/W1 (fp64 residues) or kPlainW3 (integer residues), by the largest modulus.
+// finish: 1 to run the tail / Ssum kernel after the group accumulators (jobs with tail or ssum).
 int replay_blocks_per_cu(int kind);
 hipError_t launch_replay(int kind, LaunchDev* hdr, int grid, hipStream_t stream);
 // 106-bit mode: the group accumulators of jobs[0..njobs) (kind as above), then per job the tail

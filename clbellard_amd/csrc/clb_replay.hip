@@ -444,8 +444,9 @@ __device__ DD tail106(const R106JobDev& t) {
 // Group g (blockIdx.x) of job blockIdx.y: the job's launches in the reference's order (GWS * 16
 // terms below loopmax32, GWS * 32 above, KernelFuncs.as:31-53), each reduced over the group's 256
 // work-items and added into the group's accumulator (:760-764, :979-983).  A group beyond every
-// launch's work-items keeps its accumulator (adding the zero tree of its idle work-items leaves
-// a double-double unchanged).
+// launch's work-items keeps its accumulator: adding the zero tree of its idle work-items, then
+// sumfix2, leaves a double-double already in (-1, 1) unchanged, and only bigSum[0] (after the
+// tail) can lie outside, whose group always has work-items.
 template <int KIND>
 __global__ void __launch_bounds__(256) k_r106_groups(const R106JobDev* __restrict__ jobs,
                                                     uint64_t gsize) {
@@ -453,12 +454,30 @@ __global__ void __launch_bounds__(256) k_r106_groups(const R106JobDev* __restric
   const R106JobDev jb = jobs[block_y()];
   const uint32_t g = block_x(), tid = thread_x();
   const uint64_t span = jb.k1 > jb.k0 ? jb.k1 - jb.k0 : 0;
-  if ((uint64_t)g * 256 >= (span < gsize ? span : gsize)) return;
+  const uint64_t first_items = span < gsize ? span : gsize;
+  if ((uint64_t)g * 256 >= first_items && !(jb.single && g == 0)) return;
   double* acc_ptr = reinterpret_cast<double*>(jb.bigsum) + 2 * (size_t)g;
   DD acc = {acc_ptr[0], acc_ptr[1]};
   uint64_t offset = jb.k0, k_max = jb.k0;
   const uint64_t x32 = ((1ull << 32) - jb.den1 - 1) / jb.den0 + 1;
   const uint64_t lm32 = x32 < jb.k1 ? x32 : jb.k1;
+  if (jb.single) {
+    // One launch over [k0, k1) of the kernel for m(k0) (clb_sglobal).  Group 0 runs even when the
+    // range is empty: sumfix2 of its sum then brings a tail-raised bigSum[0] back into (-1, 1),
+    // as the reference's launch of idle work-items does.
+    p[tid] = item106<KIND>(jb, jb.k0, jb.k1, gsize, g, tid, jb.k0 >= x32);
+    tree256(p, tid);
+    if (tid == 0) {
+      DD t = dd_add(acc, p[0]);
+      fix_unit(t);
+      acc = t;
+    }
+    if (tid == 0) {
+      acc_ptr[0] = acc.hi;
+      acc_ptr[1] = acc.lo;
+    }
+    return;
+  }
   for (int phase = 0; phase < 2; ++phase) {
     const uint64_t lim = phase == 0 ? lm32 : jb.k1;
     const uint64_t step = gsize * (phase == 0 ? 16 : 32);
@@ -484,8 +503,8 @@ __global__ void __launch_bounds__(256) k_r106_groups(const R106JobDev* __restric
   }
 }
 
-// Job blockIdx.x: the tail into bigSum[0] without sumfix2 (:1149), then Ssum 65536 -> 256 -> 1
-// (KernelFuncs.as:62-65); bigSum[0] and the job's part receive the total.
+// Job blockIdx.x: the tail into bigSum[0] without sumfix2 (:1149), then (with ssum) Ssum
+// 65536 -> 256 -> 1 (KernelFuncs.as:62-65); bigSum[0] and the job's part receive the total.
 __global__ void __launch_bounds__(256) k_r106_finish(const R106JobDev* __restrict__ jobs) {
   __shared__ DD p[256];
   __shared__ DD small[256];
@@ -496,6 +515,14 @@ __global__ void __launch_bounds__(256) k_r106_finish(const R106JobDev* __restric
     const DD b0 = dd_add({bs[0], bs[1]}, tail106(jb));
     bs[0] = b0.hi;
     bs[1] = b0.lo;
+  }
+  double* part = reinterpret_cast<double*>(jb.part);
+  if (!jb.ssum) {  // uniform per block: bigSum[0] as it stands
+    if (tid == 0) {
+      part[0] = bs[0];
+      part[1] = bs[1];
+    }
+    return;
   }
   __syncthreads();
   for (uint32_t b = 0; b < 256; ++b) {
@@ -508,7 +535,6 @@ __global__ void __launch_bounds__(256) k_r106_finish(const R106JobDev* __restric
   p[tid] = small[tid];
   tree256(p, tid);
   if (tid == 0) {
-    double* part = reinterpret_cast<double*>(jb.part);
     bs[0] = part[0] = p[0].hi;
     bs[1] = part[1] = p[0].lo;
   }

@@ -134,9 +134,14 @@ int clb_set_graph(clb_ctx *ctx, int enable);
  *   256-lane trees and the 65536 per-group accumulators, Sglobal_after (:1118-1150), Ssum and
  *   Sum7 (:1274-1356).  Every result (frac, partials, dev_ans, work files) is then a
  *   double-double, returned in the 3-word outputs as the bit patterns of hi (word 0) and lo
- *   (word 1) with word 2 = 0, and printed with hexadecimal_dd; clb_batch_calc and clb_add_ans act
- *   on the context's own bigSumMem (cleared by clb_set0mem), and the kernel-level operators
- *   clb_sglobal / clb_sglobal_after return CLB_EINVAL.
+ *   (word 1) with word 2 = 0, and printed with hexadecimal_dd.  The operators act on the
+ *   context's own bigSumMem of 65536 double-doubles (cleared by clb_set0mem), as the reference's
+ *   kernels do: clb_sglobal is one launch of the given global work size over [offset, k_max)
+ *   (Sglobal32mtg when m(offset) < 2^32, else Sglobal64mtg; CLB_EINVAL for a range that crosses
+ *   loopmax32, which the reference's 32-bit kernel would truncate), clb_sglobal_after adds the
+ *   tail into bigSum[0], clb_ssum runs Ssum's two passes (bigSum[0] = the total), clb_batch_calc
+ *   all three in BatchCalc's order, and clb_add_ans Sum7; the partial (clb_read_partial) is
+ *   bigSum[0] after each call.
  * CLB_EINVAL for a size that is not a multiple of 256 up to 2^24; calls with 10 d + 9 >= 2^53 (the
  * reference's OVERDOUBLE build) then fail with CLB_ERANGE.  Replayed calls never use graph mode. */
 #define CLB_REF_GLOBAL_WORK_SIZE 16777216ull
@@ -169,6 +174,12 @@ int clb_sglobal(clb_ctx *ctx, uint64_t offset, uint64_t d, uint64_t k_max, int64
                 uint64_t den0, uint64_t den1);
 /* Sglobal_after_192 kernel operator (Bellard.cl:1153-1253): adds the tail terms k = d..d+19. */
 int clb_sglobal_after(clb_ctx *ctx, uint64_t d, int64_t numesign, uint64_t den0, uint64_t den1);
+/* Ssum kernel operator, both passes (KernelFuncs.as:62-65, Bellard.cl:1274-1297 / :1300-1321):
+ * reduces the partial's per-group accumulators into the partial.  Only the 106-bit-mode replay
+ * keeps per-group accumulators (clb_set_replay); everywhere else the operators add straight into
+ * the partial with exact 192-bit sums, whose order does not change the result, and this call
+ * does nothing. */
+int clb_ssum(clb_ctx *ctx);
 
 /* Allloop (Bellard_192.hsp:264-296): the full normal-mode computation at hex position N.
  * frac = frac(16^(N'-1) pi) (192 bits), hex = the display string (precision dependent),

@@ -176,6 +176,30 @@ static int set_sglobal(cl_kernel k, int p, cl_ulong offset, cl_ulong d, cl_ulong
 
 static int set0mem(int p) { return fill0(g_big[p], BIGSUM_SIZE * (p == 0 ? 2u : 3u), 64); }
 
+/* Sglobal_after* on one work-item (KernelFuncs.as:56-59). */
+static int after(int p, uint64_t d, int j) {
+  const cl_long sign = kNumeSign[j];
+  const cl_ulong dd = d, den0 = kDen0[j], den1 = kDen1[j];
+  cl_kernel k = g_kafter[p];
+  clSetKernelArg(k, 0, sizeof(cl_mem), &g_big[p]);
+  clSetKernelArg(k, 1, sizeof(cl_ulong), &dd);
+  clSetKernelArg(k, 2, sizeof(cl_long), &sign);
+  clSetKernelArg(k, 3, sizeof(cl_ulong), &den0);
+  clSetKernelArg(k, 4, sizeof(cl_ulong), &den1);
+  return launch(k, 1, 1);
+}
+
+/* Ssum 65536 -> 256, then 256 -> 1 (KernelFuncs.as:61-65). */
+static int ssum(int p) {
+  cl_kernel s = g_ksum[p];
+  clSetKernelArg(s, 0, sizeof(cl_mem), &g_big[p]);
+  clSetKernelArg(s, 1, sizeof(cl_mem), &g_small[p]);
+  if (launch(s, BIGSUM_SIZE, LWS)) return -1;
+  clSetKernelArg(s, 0, sizeof(cl_mem), &g_small[p]);
+  clSetKernelArg(s, 1, sizeof(cl_mem), &g_big[p]);
+  return launch(s, BIGSUM_SIZE / 256, LWS);
+}
+
 /* KernelFuncs.as:25-66 */
 static int batch_calc(int p, uint64_t d, uint64_t k_start, uint64_t k_end, int j) {
   uint64_t k_max = k_start, offset = k_start;
@@ -193,24 +217,8 @@ static int batch_calc(int p, uint64_t d, uint64_t k_start, uint64_t k_end, int j
     if (set_sglobal(g_k64[p], p, offset, d, k_max, j) || launch(g_k64[p], g_gws, LWS)) return -1;
     offset = k_max;
   }
-  if (offset == d) {
-    const cl_long sign = kNumeSign[j];
-    const cl_ulong dd = d, den0 = kDen0[j], den1 = kDen1[j];
-    cl_kernel k = g_kafter[p];
-    clSetKernelArg(k, 0, sizeof(cl_mem), &g_big[p]);
-    clSetKernelArg(k, 1, sizeof(cl_ulong), &dd);
-    clSetKernelArg(k, 2, sizeof(cl_long), &sign);
-    clSetKernelArg(k, 3, sizeof(cl_ulong), &den0);
-    clSetKernelArg(k, 4, sizeof(cl_ulong), &den1);
-    if (launch(k, 1, 1)) return -1;
-  }
-  cl_kernel s = g_ksum[p];
-  clSetKernelArg(s, 0, sizeof(cl_mem), &g_big[p]);
-  clSetKernelArg(s, 1, sizeof(cl_mem), &g_small[p]);
-  if (launch(s, BIGSUM_SIZE, LWS)) return -1;
-  clSetKernelArg(s, 0, sizeof(cl_mem), &g_small[p]);
-  clSetKernelArg(s, 1, sizeof(cl_mem), &g_big[p]);
-  return launch(s, BIGSUM_SIZE / 256, LWS);
+  if (offset == d && after(p, d, j)) return -1;
+  return ssum(p);
 }
 
 /* KernelFuncs.as:68-74 */
@@ -268,6 +276,33 @@ int ref_job(uint64_t d, uint64_t k_start, uint64_t k_end, int j, int precision, 
   if (read_words(g_big[precision], precision, part)) return -1;
   g_last_ms = now_ms() - t0;
   return 0;
+}
+
+/* The reference's kernels one at a time on its bigSumMem, for the operator-level tests:
+ * op 0 Set0mem, 1 Sglobal32mtg*(offset = a, d = b, k_max = c), 2 Sglobal64mtg*(a, b, c),
+ * 3 Sglobal_after*(d = b), 4 Ssum's two passes, then (every op) bigSum[0..] into out (2 words
+ * for precision 0, 3 otherwise).  Launches use the work-item count of ref_set_gws. */
+int ref_op(int op, int precision, uint64_t a, uint64_t b, uint64_t c, int j, uint64_t *out) {
+  const int p = precision;
+  if (p < 0 || p > 2 || j < 0 || j > 6 || op < 0 || op > 4) {
+    snprintf(g_err, sizeof g_err, "ref_op: bad argument");
+    return -1;
+  }
+  if ((op == 1 || op == 2 || op == 3) && check_d(b)) return -1;
+  int rc = 0;
+  switch (op) {
+    case 0: rc = set0mem(p); break;
+    case 1:
+    case 2: {
+      cl_kernel k = op == 1 ? g_k32[p] : g_k64[p];
+      rc = set_sglobal(k, p, a, b, c, j) || launch(k, g_gws, LWS);
+      break;
+    }
+    case 3: rc = after(p, b, j); break;
+    case 4: rc = ssum(p); break;
+  }
+  if (rc) return -1;
+  return read_words(g_big[p], p, out);
 }
 
 void ref_shutdown(void) {

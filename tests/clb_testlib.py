@@ -411,6 +411,8 @@ class RefKernels:
         lib.ref_last_ms.restype = ctypes.c_double
         lib.ref_shutdown.restype = None
         lib.ref_set_gws.argtypes, lib.ref_set_gws.restype = [u64], c_int
+        lib.ref_op.argtypes = [c_int, c_int, u64, u64, u64, c_int, p64]
+        lib.ref_op.restype = c_int
         self.lib = lib
         self._check(lib.ref_init(co_dir.encode()))
 
@@ -433,6 +435,13 @@ class RefKernels:
     def job(self, d: int, k0: int, k1: int, j: int, precision: int):
         w = U3()
         self._check(self.lib.ref_job(d, k0, k1, j, precision, w))
+        return self._value(w, precision)
+
+    def op(self, op: int, precision: int, a: int = 0, b: int = 0, c: int = 0, j: int = 0):
+        """One reference kernel on bigSumMem, then bigSum[0]: op 0 Set0mem, 1 Sglobal32mtg*(offset
+        a, d b, k_max c), 2 Sglobal64mtg*(a, b, c), 3 Sglobal_after*(d b), 4 Ssum's two passes."""
+        w = U3()
+        self._check(self.lib.ref_op(op, precision, a, b, c, j, w))
         return self._value(w, precision)
 
     def set_gws(self, gws: int = 1 << 24) -> None:

@@ -214,9 +214,119 @@ def test_hip_106bit_replay_operators(ref, ctx106):
         ctx106.batch_calc(d, 0, d, j)
         ctx106.add_ans(j)
     assert dd_of_words(ctx106.read_ans()) == ref.normal(d, 0)
-    with pytest.raises(clb.ClbError) as e:
-        ctx106.sglobal(0, d, 100, 1, 4, 1)
+
+
+def loopmax32(j):
+    _, _, den0, den1 = clb.formula(j)
+    return ((1 << 32) - den1 - 1) // den0 + 1
+
+
+class OpPair:
+    """The library's operators and the reference's kernels side by side on their own bigSumMem:
+    every call is made on both and bigSum[0] (the library's partial) compared bit for bit."""
+
+    def __init__(self, ctx, ref, j):
+        self.ctx, self.ref, self.j = ctx, ref, j
+        _, self.sign, self.den0, self.den1 = clb.formula(j)
+        self.calls = 0
+
+    def check(self, want, what):
+        got = dd_of_words(self.ctx.read_partial())
+        assert got == want, (what, self.j, got, want)
+        self.calls += 1
+
+    def set0mem(self):
+        self.ctx.set0mem()
+        self.check(self.ref.op(0, 0), "Set0memdd")
+
+    def sglobal(self, offset, d, k_max):
+        self.ctx.sglobal(offset, d, k_max, self.sign, self.den0, self.den1)
+        op = 1 if offset < loopmax32(self.j) else 2
+        self.check(self.ref.op(op, 0, offset, d, k_max, self.j), ("Sglobal", op, offset, k_max))
+
+    def after(self, d):
+        self.ctx.sglobal_after(d, self.sign, self.den0, self.den1)
+        self.check(self.ref.op(3, 0, 0, d, 0, self.j), ("Sglobal_after", d))
+
+    def ssum(self):
+        self.ctx.ssum()
+        self.check(self.ref.op(4, 0), "Ssum")
+
+    def batch_calc(self, d, k_start, k_end, gws):
+        """BatchCalc (KernelFuncs.as:25-66) spelled out as single kernel calls."""
+        offset = k_max = k_start
+        lm32 = min(loopmax32(self.j), d, k_end)
+        while offset < lm32:
+            k_max = min(k_max + gws * 16, lm32)
+            self.sglobal(offset, d, k_max)
+            offset = k_max
+        lm64 = min(k_end, d)
+        while offset < lm64:
+            k_max = min(k_max + gws * 32, lm64)
+            self.sglobal(offset, d, k_max)
+            offset = k_max
+        if offset == d:
+            self.after(d)
+        self.ssum()
+
+
+def op_cases():
+    x = loopmax32(2)
+    g = 4096
+    return [  # d, k_start, k_end, j, work-items per launch
+        (1234567, 1000000, 1234567, 0, g),               # several launches, then the tail
+        (999999999, x - 3 * g * 16 - 5, x + 2 * g * 32 + 7, 2, g),  # across loopmax32
+        (500001, 0, 500001, 6, clb.REF_GLOBAL_WORK_SIZE),  # Allloop's single launch
+    ]
+
+
+@pytest.mark.parametrize("case", op_cases(), ids=lambda c: "d{}_k{}-{}_j{}_g{}".format(*c))
+def test_hip_106bit_replay_kernel_operators(ref, ctx106, case):
+    """Sglobal32mtg / Sglobal64mtg / Sglobal_after / Ssum one call at a time on the context's
+    65536 group accumulators (clb_sglobal, clb_sglobal_after, clb_ssum) equal the reference's
+    kernels after every call, and their BatchCalc sequence equals clb_batch_calc."""
+    d, k0, k1, j, gws = case
+    note(f"106-bit kernel operators d={d} k=[{k0},{k1}) j={j} gws={gws}")
+    ctx106.set_replay(gws)
+    ref.set_gws(gws)
+    pair = OpPair(ctx106, ref, j)
+    pair.set0mem()
+    pair.batch_calc(d, k0, k1, gws)
+    by_ops = ctx106.read_partial()
+    ctx106.set0mem()
+    ctx106.batch_calc(d, k0, k1, j)
+    assert ctx106.read_partial() == by_ops
+    note(f"  {pair.calls} calls equal the reference kernels; BatchCalc equal")
+
+
+def test_hip_106bit_replay_operator_orders(ref, ctx106):
+    """Orders BatchCalc never makes, still the reference kernels' results after every call:
+    tails before head terms and twice in a row, an empty launch (its work-group 0 still applies
+    sumfix2 to bigSum[0]), Ssum twice, launches after Ssum on the accumulators it leaves, and two
+    BatchCalcs without Set0mem between them."""
+    gws = 8192
+    ctx106.set_replay(gws)
+    ref.set_gws(gws)
+    d, j = 2000001, 3
+    p = OpPair(ctx106, ref, j)
+    p.set0mem()
+    p.after(d)
+    p.after(d)
+    p.sglobal(500, d, 500)
+    p.sglobal(0, d, 300000)
+    p.after(d)
+    p.sglobal(10, d, 5)
+    p.ssum()
+    p.ssum()
+    p.sglobal(300000, d, 700000)
+    p.ssum()
+    p.batch_calc(d, 700000, d, gws)
+    p.batch_calc(d, 0, 123457, gws)
+    x = loopmax32(j)
+    with pytest.raises(clb.ClbError) as e:  # one 32-bit launch past loopmax32
+        ctx106.sglobal(x - 10, 10**9 + 1, x + 10, p.sign, p.den0, p.den1)
     assert e.value.code == clb.EINVAL
+    note(f"106-bit operator orders: {p.calls} calls equal the reference kernels")
 
 
 def test_hip_106bit_replay_work_files(ref, ctx106, tmp_path):
