@@ -326,9 +326,9 @@ int expand_replay(const clb_ctx* ctx, const std::vector<HostJob>& in, std::vecto
     return fail(CLB_EINVAL, "replay: the 106-bit mode (precision 0) is not replayed");
   uint64_t flat = 0;
   for (const HostJob& h : in) {
-    if (h.d > ((1ull << 53) - 10) / 10)  // Bellard_192.hsp:472: 10 d + 9 >= 2^53
-      return fail(CLB_ERANGE, "replay: d = %llu needs the reference's OVERDOUBLE build, which "
-                  "is not replayed", (unsigned long long)h.d);
+    if (h.d > (INT64_MAX - 9) / 10)  // the -D OVERDOUBLE kernels need moduli below 2^63
+      return fail(CLB_ERANGE, "replay: d = %llu puts moduli at 2^63 or above",
+                  (unsigned long long)h.d);
     uint64_t offset = h.k0, k_max = h.k0;
     auto launch = [&](uint64_t end) {
       ReplayJobDev r{};
@@ -422,9 +422,9 @@ int run_replay106(clb_ctx* ctx, const std::vector<HostJob>& hjobs,
   const size_t nslots = hjobs.size();
   uint64_t max_m = 1;
   for (const HostJob& h : hjobs) {
-    if (h.d > ((1ull << 53) - 10) / 10)
-      return fail(CLB_ERANGE, "replay: d = %llu needs the reference's OVERDOUBLE build, which "
-                  "is not replayed", (unsigned long long)h.d);
+    if (h.d > (INT64_MAX - 9) / 10)
+      return fail(CLB_ERANGE, "replay: d = %llu puts moduli at 2^63 or above",
+                  (unsigned long long)h.d);
     const uint64_t x32 = ((1ull << 32) - h.den1 - 1) / h.den0 + 1;  // KernelFuncs.as:31-33
     if (h.single && h.k0 < x32 && x32 < h.k1)
       return fail(CLB_EINVAL, "replay: one Sglobal32mtg launch over [%llu, %llu) passes "
@@ -435,7 +435,6 @@ int run_replay106(clb_ctx* ctx, const std::vector<HostJob>& hjobs,
   if (op_level && nslots != 1) return fail(CLB_EINVAL, "replay: one job per operator call");
   int kind = kPlainW3;
   if ((rc = choose_kind(ctx->kernel, max_m, &kind))) return rc;
-  if (kind == kPlainWide) kind = kPlainW3;  // replayed moduli are below 2^53
   const size_t ngr = groups.size();
   auto align64 = [](size_t x) { return (x + 63) & ~(size_t)63; };
   const size_t off_jobs = 0;
@@ -657,7 +656,6 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
       if (m > rmax_m) rmax_m = m;
     }
     if ((rc = choose_kind(ctx->kernel, rmax_m, &rkind))) return rc;
-    if (rkind == kPlainWide) rkind = kPlainW3;  // replayed moduli are below 2^53
     int& res = ctx->replay_resident[rkind];
     if (res < 0) res = replay_blocks_per_cu(rkind);
     if (res <= 0) return fail(CLB_EDEVICE, "replay kernel: no occupancy");

@@ -172,6 +172,28 @@ __device__ __forceinline__ void dd_step(DD& s, double x, double m, double minv) 
   s = dd_add(s, div_nn_m(__builtin_fma(c, m, x), m, minv));
 }
 
+__device__ __forceinline__ DD dd_of(uint64_t v) {  // (double)v and (double)((long)v - (long)x)
+  const double x = (double)v;
+  return {x, (double)((int64_t)v - (int64_t)x)};
+}
+
+// The same update from an integer residue r in [0, m), m < 2^63.  Moduli of 2^53 and above occur
+// only when 10 d + 9 >= 2^53, where the reference host builds its kernels with -D OVERDOUBLE
+// (Bellard_192.hsp:472-474): the steered numerator and m then go into dd_div split into
+// double-doubles (Bellard.cl:938-939, :957-962, :1040-1046).  Below 2^53 both parts' low halves
+// are 0, and dd_div of such operands returns nn_div's doubles (its extra terms, z1 0 and + 0,
+// change no value), so the build's choice only matters from 2^53 on.
+__device__ __forceinline__ void dd_step_u(DD& s, uint64_t r, uint64_t m, double mf, double minv) {
+  if (m >= (1ull << 53)) {
+    const bool pos = s.hi > 0.0;
+    DD q = div_dd(dd_of(pos ? m - r : r), dd_of(m));
+    if (pos) q = dd_neg(q);
+    s = dd_add(s, q);
+  } else {
+    dd_step(s, (double)r, mf, minv);
+  }
+}
+
 // Window width of the fp64 engine for a kernel kind (0 for the integer engine).
 template <int KIND>
 constexpr int kWindow = KIND == kSplitW3 ? 3 : KIND == kSplitW2 ? 2 : KIND == kSplitW1 ? 1 : 0;
@@ -207,38 +229,50 @@ __device__ __forceinline__ double pow2_lane(uint64_t E0, uint32_t lane, uint64_t
   return fpow2<W>(E0 - 10ull * lane, start_shift<W>(E0, m0), m, minv);
 }
 
-// x = 2^e mod m and b = 2^(e + 96) mod m = x 2^96 mod m (:798-799, :1052-1053), exact, for
-// e = E0 - 10 lane (E0 and m0 = lane 0's exponent and modulus, uniform), as integer-valued
-// doubles of magnitude below m (m < 2^53; mf = m as a double, minv = recip(mf)).
-//  * fp64 engine (KIND kSplitW3/W2/W1, m < 2^42 / 2^46 / 2^48): balanced residues, pow2_lane
-//    (|x| < 0.92 m), then two exact shift-and-reduce steps of 48 bits (|x 2^48 / m - q| <=
-//    0.5 + 2^48 1.5 2^-52 < 0.6, so |b| < 0.6 m and every intermediate is an exact integer below
-//    2^53);
-//  * int engine (kPlainW3, odd m < 2^55): residues in [0, m), Montgomery form h = 2^(e + 64) mod m,
-//    x = REDC(h), and b = REDC(h (2^96 mod m)) with 2^96 mod m the Montgomery form of 2^32.
 template <int KIND>
-__device__ __forceinline__ void residues(uint64_t E0, uint32_t lane, uint64_t m0, uint64_t m,
-                                         double mf, double minv, double& x, double& b) {
+constexpr bool kIntEngine = KIND == kPlainW3 || KIND == kPlainWide;
+
+// The int engine's residues (kPlainW3: odd m < 2^55, 3-bit windows, lazy residues; kPlainWide:
+// m < 2^63, one bit per square, reduced), in [0, m): x = 2^e mod m and, with b, x 2^96 mod m
+// (:798-799, :1052-1053), e = E0 - 10 lane.  From the Montgomery form h = 2^(e + 64) mod m:
+// x = REDC(h), b = REDC(h (2^96 mod m)), 2^96 mod m being the Montgomery form of 2^32.
+template <int KIND>
+__device__ __forceinline__ void int_residues(uint64_t E0, uint32_t lane, uint64_t m, uint64_t& x,
+                                             uint64_t* b) {
+  const uint64_t mp = neg_inv64(m);
+  const uint64_t one = (0ull - m) % m;
+  uint64_t h, c = 0;
   if constexpr (KIND == kPlainW3) {
-    const uint64_t mp = neg_inv64(m);
-    const uint64_t one = (0ull - m) % m;
     const int bl = 64 - __builtin_clzll(E0);  // every lane's e < 2^bl
-    const uint64_t h = ipow2_mont<3>(E0 - 10ull * lane, bl, m, mp, one);
-    const uint64_t c = ipow2_mont<3>(32, 6, m, mp, one);
-    uint64_t ri = redc(0, h, m, mp);
-    if (ri >= m) ri -= m;
-    uint64_t bi = redc(mulhi64(h, c), h * c, m, mp);
-    if (bi >= m) bi -= m;
-    x = (double)ri;
-    b = (double)bi;
+    h = ipow2_mont<3>(E0 - 10ull * lane, bl, m, mp, one);
+    if (b) c = ipow2_mont<3>(32, 6, m, mp, one);
   } else {
-    x = pow2_lane<kWindow<KIND>>(E0, lane, m0, mf, minv);
-    b = x;
+    h = ipow2_mont_wide(E0 - 10ull * lane, m, mp, one);
+    if (b) c = ipow2_mont_wide(32, m, mp, one);
+  }
+  x = redc(0, h, m, mp);
+  if (x >= m) x -= m;
+  if (b) {
+    uint64_t v = redc(mulhi64(h, c), h * c, m, mp);
+    if (v >= m) v -= m;
+    *b = v;
+  }
+}
+
+// The fp64 engine's residues (KIND kSplitW3/W2/W1, m < 2^42 / 2^46 / 2^48; E0 and m0 = lane 0's
+// exponent and modulus, uniform; mf = m, minv = recip(mf)): x = 2^e mod m balanced by pow2_lane
+// (|x| < 0.92 m), and b = x 2^96 mod m by two exact shift-and-reduce steps of 48 bits
+// (|x 2^48 / m - q| <= 0.5 + 2^48 1.5 2^-52 < 0.6, so |b| < 0.6 m and every intermediate is an
+// exact integer below 2^53).
+template <int KIND>
+__device__ __forceinline__ void fp_residues(uint64_t E0, uint32_t lane, uint64_t m0, double mf,
+                                            double minv, double& x, double& b) {
+  x = pow2_lane<kWindow<KIND>>(E0, lane, m0, mf, minv);
+  b = x;
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const double z = __builtin_amdgcn_ldexp(b, 48);
-      b = __builtin_fma(-__builtin_rint(z * minv), mf, z);
-    }
+  for (int i = 0; i < 2; ++i) {
+    const double z = __builtin_amdgcn_ldexp(b, 48);
+    b = __builtin_fma(-__builtin_rint(z * minv), mf, z);
   }
 }
 
@@ -251,7 +285,7 @@ __device__ U192 replay_item(const ReplayJobDev& jb, uint64_t tile, int lane) {
   const uint64_t d = jb.d, k_max = jb.k_max, gsize = jb.gsize, den0 = jb.den0, den1 = jb.den1;
   const uint64_t k_lane0 = jb.offset + tile * kWave;
   // The lane's modulus advances by den0 gsize per term: for the fp64 engine one exact add of
-  // doubles (every m used is below 2^53), for the int engine a 64-bit add and a conversion.
+  // doubles (every m used is below 2^48), for the int engine a 64-bit add and a conversion.
   uint64_t m = den0 * (k_lane0 + (uint64_t)lane) + den1;
   double mf = (double)m;
   const uint64_t mstep = den0 * gsize;
@@ -259,14 +293,21 @@ __device__ U192 replay_item(const ReplayJobDev& jb, uint64_t tile, int lane) {
   for (uint64_t kw = k_lane0; kw < k_max; kw += gsize) {
     const uint64_t k = kw + (uint64_t)lane;
     if (k < k_max) {
-      if constexpr (KIND == kPlainW3) mf = (double)m;
+      if constexpr (kIntEngine<KIND>) mf = (double)m;
       const double minv = recip(mf);  // the exponent engine's and the divisions'
-      double x, b;
-      residues<KIND>(10ull * (d - kw), (uint32_t)lane, den0 * kw + den1, m, mf, minv, x, b);
-      dd_step(s0, x, mf, minv);
-      dd_step(s1, b, mf, minv);
+      if constexpr (kIntEngine<KIND>) {
+        uint64_t x, b;
+        int_residues<KIND>(10ull * (d - kw), (uint32_t)lane, m, x, &b);
+        dd_step_u(s0, x, m, mf, minv);
+        dd_step_u(s1, b, m, mf, minv);
+      } else {
+        double x, b;
+        fp_residues<KIND>(10ull * (d - kw), (uint32_t)lane, den0 * kw + den1, mf, minv, x, b);
+        dd_step(s0, x, mf, minv);
+        dd_step(s1, b, mf, minv);
+      }
     }
-    if constexpr (KIND == kPlainW3)
+    if constexpr (kIntEngine<KIND>)
       m += mstep;
     else
       mf += mstepf;
@@ -286,11 +327,6 @@ __device__ uint64_t pow2_mod_small(int e, uint64_t m) {  // 2^e mod m, m < 2^63,
     if (x >= m) x -= m;
   }
   return x;
-}
-
-__device__ __forceinline__ DD dd_of(uint64_t v) {  // (double)v and (double)((long)v - (long)x)
-  const double x = (double)v;
-  return {x, (double)((int64_t)v - (int64_t)x)};
 }
 
 // Sglobal_after_192, :1153-1253: k = d + i, i = 19..0, into s0 with numerators 2^(-10 i);
@@ -354,22 +390,6 @@ __device__ __forceinline__ void fix_one(DD& s) {  // sumfix, :491-501
   }
 }
 
-// x = 2^e mod m alone, exact, as an integer-valued double: balanced for the fp64 engine, in
-// [0, m) for the int engine (see residues).
-template <int KIND>
-__device__ __forceinline__ double residue(uint64_t E0, uint32_t lane, uint64_t m0, uint64_t m,
-                                          double mf, double minv) {
-  if constexpr (KIND == kPlainW3) {
-    const uint64_t mp = neg_inv64(m);
-    const uint64_t one = (0ull - m) % m;
-    const int bl = 64 - __builtin_clzll(E0);
-    uint64_t r = redc(0, ipow2_mont<3>(E0 - 10ull * lane, bl, m, mp, one), m, mp);
-    return (double)(r >= m ? r - m : r);
-  } else {
-    return pow2_lane<kWindow<KIND>>(E0, lane, m0, mf, minv);
-  }
-}
-
 // The 256 work-items of one work-group reduced in LDS the reference's way (shared_reduction_dd,
 // :583-597, and Ssum's tree, :1274-1297): p[l] = sumfix2(p[l] + p[l + i]), i = 128 .. 1.
 __device__ __forceinline__ void tree256(DD* p, uint32_t tid) {
@@ -404,18 +424,30 @@ __device__ DD item106(const R106JobDev& jb, uint64_t offset, uint64_t k_max, uin
   for (uint64_t kw = offset + wave0; kw < k_max; kw += gsize) {  // kw: the wave's lane 0
     const uint64_t k = kw + lane;
     if (k < k_max) {
-      if constexpr (KIND == kPlainW3) mf = (double)m;
+      if constexpr (kIntEngine<KIND>) mf = (double)m;
       const double minv = recip(mf);
-      const double x =
-          residue<KIND>(10ull * (jb.d - kw), (uint32_t)lane, jb.den0 * kw + jb.den1, m, mf, minv);
-      if (is64) {
-        dd_step(s, x, mf, minv);
+      if constexpr (kIntEngine<KIND>) {
+        uint64_t r;
+        int_residues<KIND>(10ull * (jb.d - kw), (uint32_t)lane, m, r, nullptr);
+        if (is64) {
+          dd_step_u(s, r, m, mf, minv);
+        } else {  // m < 2^32
+          s = dd_add(s, div_nn_m((double)r, mf, minv));
+          fix_one(s);
+        }
       } else {
-        s = dd_add(s, div_nn_m(x < 0.0 ? x + mf : x, mf, minv));
-        fix_one(s);
+        // x = 2^e mod m alone, balanced (the fp64 engine)
+        const double x = pow2_lane<kWindow<KIND>>(10ull * (jb.d - kw), (uint32_t)lane,
+                                                   jb.den0 * kw + jb.den1, mf, minv);
+        if (is64) {
+          dd_step(s, x, mf, minv);
+        } else {
+          s = dd_add(s, div_nn_m(x < 0.0 ? x + mf : x, mf, minv));
+          fix_one(s);
+        }
       }
     }
-    if constexpr (KIND == kPlainW3)
+    if constexpr (kIntEngine<KIND>)
       m += mstep;
     else
       mf += mstepf;
@@ -638,6 +670,7 @@ int replay_blocks_per_cu(int kind) {
     case kSplitW2: return replay_blocks<kSplitW2>();
     case kSplitW1: return replay_blocks<kSplitW1>();
     case kPlainW3: return replay_blocks<kPlainW3>();
+    case kPlainWide: return replay_blocks<kPlainWide>();
     default: return 0;
   }
 }
@@ -653,6 +686,9 @@ hipError_t launch_replay106(int kind, const R106JobDev* jobs, uint32_t njobs, ui
       case kSplitW2: hipLaunchKernelGGL(k_r106_groups<kSplitW2>, g, b, 0, stream, jobs, gs); break;
       case kSplitW1: hipLaunchKernelGGL(k_r106_groups<kSplitW1>, g, b, 0, stream, jobs, gs); break;
       case kPlainW3: hipLaunchKernelGGL(k_r106_groups<kPlainW3>, g, b, 0, stream, jobs, gs); break;
+      case kPlainWide:
+        hipLaunchKernelGGL(k_r106_groups<kPlainWide>, g, b, 0, stream, jobs, gs);
+        break;
       default: return hipErrorInvalidValue;
     }
     const hipError_t e = hipGetLastError();
@@ -694,6 +730,7 @@ hipError_t launch_replay(int kind, LaunchDev* hdr, int grid, hipStream_t stream)
     case kSplitW2: hipLaunchKernelGGL(k_replay<kSplitW2>, g, b, 0, stream, hdr); break;
     case kSplitW1: hipLaunchKernelGGL(k_replay<kSplitW1>, g, b, 0, stream, hdr); break;
     case kPlainW3: hipLaunchKernelGGL(k_replay<kPlainW3>, g, b, 0, stream, hdr); break;
+    case kPlainWide: hipLaunchKernelGGL(k_replay<kPlainWide>, g, b, 0, stream, hdr); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -142,8 +142,12 @@ int clb_set_graph(clb_ctx *ctx, int enable);
  *   tail into bigSum[0], clb_ssum runs Ssum's two passes (bigSum[0] = the total), clb_batch_calc
  *   all three in BatchCalc's order, and clb_add_ans Sum7; the partial (clb_read_partial) is
  *   bigSum[0] after each call.
- * CLB_EINVAL for a size that is not a multiple of 256 up to 2^24; calls with 10 d + 9 >= 2^53 (the
- * reference's OVERDOUBLE build) then fail with CLB_ERANGE.  Replayed calls never use graph mode. */
+ *   Loop counts with 10 d + 9 >= 2^53 replay the reference's -D OVERDOUBLE build
+ *   (Bellard_192.hsp:472-474): from m = 2^53 on, numerator and modulus split into double-doubles
+ *   and divided with dd_div (Bellard.cl:937-962, :1017-1046); moduli must stay below 2^63
+ *   (CLB_ERANGE otherwise).
+ * CLB_EINVAL for a size that is not a multiple of 256 up to 2^24.  Replayed calls never use graph
+ * mode. */
 #define CLB_REF_GLOBAL_WORK_SIZE 16777216ull
 int clb_set_replay(clb_ctx *ctx, uint64_t global_work_size);
 

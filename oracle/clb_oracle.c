@@ -605,11 +605,27 @@ static void rp_dd2_to_ulong3(rdd s0, rdd s1, uint64_t w[3]) {
   w[2] = u2;
 }
 
+/* The -D OVERDOUBLE build (Bellard_192.hsp:472-474): every kernel of the position's program is
+ * built with it once 10 d + 9 >= 2^53. */
+static int rp_overdouble(uint64_t d) { return d * 10 + 9 >= 9007199254740992ull; }
+
+/* (double)v and the remainder, (double)((long)v - (long)(double)v) (Bellard.cl:938-939,
+ * :1041-1042; v < 2^63). */
+static rdd rp_u64dd(uint64_t v) {
+  rdd r;
+  r.hi = (double)v;
+  r.lo = (double)((int64_t)v - (int64_t)r.hi);
+  return r;
+}
+
 /* One term of Bellard.cl:783-805 / :1012-1069: the numerator is steered so that the sum stays
- * in (-1, 1) (m - r and a negated quotient while the sum is positive). */
-static void rp_ddstep(rdd *s, uint64_t r, uint64_t m) {
+ * in (-1, 1) (m - r and a negated quotient while the sum is positive).  od: the -D OVERDOUBLE
+ * build's dd_div of the split numerator and denominator (:957-962, :1040-1046) instead of
+ * nn_div. */
+static void rp_ddstep(rdd *s, uint64_t r, uint64_t m, int od) {
   const int pos = s->hi > 0.0;
-  rdd q = rp_nn_div((double)(pos ? m - r : r), (double)m);
+  const uint64_t n2 = pos ? m - r : r;
+  rdd q = od ? rp_dd_div(rp_u64dd(n2), rp_u64dd(m)) : rp_nn_div((double)n2, (double)m);
   if (pos) q = rp_neg(q);
   *s = rp_addcab(*s, q);
 }
@@ -618,13 +634,15 @@ void orc_ref_item192(uint64_t d, uint64_t offset, uint64_t k_max, uint64_t idx, 
                      int j, uint64_t out[3]) {
   rdd s0 = {0.0, 0.0}, s1 = {0.0, 0.0};
   uint64_t k = offset + idx;
+  const int od = rp_overdouble(d);
   for (; k < k_max; k += gsize) {
     const uint64_t m = kDen0[j] * k + kDen1[j];
     const uint64_t r = orc_pow1024_mod(d - k, m);
     uint64_t b = (uint64_t)(((u128)r << 64) % m); /* r 2^96 mod m, Bellard.cl:798-799, :1052-1053 */
     b = (uint64_t)(((u128)b << 32) % m);
-    rp_ddstep(&s0, r, m);
-    rp_ddstep(&s1, b, m);
+    /* the 32-bit kernel (m < 2^32) has no OVERDOUBLE variant */
+    rp_ddstep(&s0, r, m, od && m >= (1ull << 32));
+    rp_ddstep(&s1, b, m, od && m >= (1ull << 32));
   }
   if (((uint64_t)kNumeSign[j] + k % 2) % 2 == 1) { /* Bellard.cl:807-810, :1075-1078 */
     s0 = rp_neg(s0);
@@ -738,7 +756,7 @@ int orc_ref_batch192(uint64_t d, uint64_t k_start, uint64_t k_end, int j, int pr
                      uint64_t gsize, int nthreads, uint64_t out[3]) {
   out[0] = out[1] = out[2] = 0;
   if (j < 0 || j > 6 || (precision != 1 && precision != 2) || gsize == 0) return -1;
-  if (d * 10 + 9 >= 9007199254740992ull) return -1; /* -D OVERDOUBLE builds differ */
+  if (d > (9223372036854775807ull - 9) / 10) return -1; /* moduli below 2^63, Bellard.cl:1041 */
   uint64_t k_max = k_start, offset = k_start;
   uint64_t lm32 = ((1ull << 32) - kDen1[j] - 1) / kDen0[j] + 1; /* KernelFuncs.as:31-33 */
   if (lm32 > d) lm32 = d;
@@ -814,7 +832,7 @@ static rdd rp_item106(uint64_t d, uint64_t offset, uint64_t k_max, uint64_t idx,
     const uint64_t m = kDen0[j] * k + kDen1[j];
     const uint64_t r = orc_pow1024_mod(d - k, m);
     if (is64) {
-      rp_ddstep(&s, r, m);
+      rp_ddstep(&s, r, m, rp_overdouble(d));
     } else {
       s = rp_addcab(s, rp_nn_div((double)r, (double)m));
       rp_sumfix(&s);
@@ -939,7 +957,7 @@ static void rp_sum7_106(rdd *part, rdd *ans, uint64_t numer) {
 int orc_ref_job106(uint64_t d, uint64_t k_start, uint64_t k_end, int j, uint64_t gsize,
                    int nthreads, double out[2]) {
   if (j < 0 || j > 6 || gsize == 0 || gsize % 256 || gsize > 256ull * REF_GROUPS) return -1;
-  if (d * 10 + 9 >= 9007199254740992ull) return -1;
+  if (d > (9223372036854775807ull - 9) / 10) return -1;
   rdd *bigsum = calloc(REF_GROUPS, sizeof(rdd));
   if (!bigsum) return -1;
   rdd ans = {0.0, 0.0};
@@ -955,7 +973,6 @@ int orc_ref_normal106(uint64_t n, uint64_t gsize, int nthreads, double ans_out[2
   uint64_t d;
   if (orc_normalize(n, NULL, &d, NULL)) return -1;
   if (gsize == 0 || gsize % 256 || gsize > 256ull * REF_GROUPS) return -1;
-  if (d * 10 + 9 >= 9007199254740992ull) return -1;
   rdd *bigsum = malloc(REF_GROUPS * sizeof(rdd));
   if (!bigsum) return -1;
   rdd ans = {0.0, 0.0};

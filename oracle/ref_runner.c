@@ -15,8 +15,9 @@
  *   KernelFuncs.as:25-66       BatchCalc: GWS*16-term launches below loopmax32, GWS*32 above,
  *                              the tail iff offset == d, Ssum 65536 -> 256 -> 1
  *   KernelFuncs.as:68-74       AddAns -> Sum7
- * The programs are the default build (no -cl-denorms-are-zero, no -cl-fast-relaxed-math, no
- * -D OVERDOUBLE), so loop counts with 10 d + 9 >= 2^53 are refused.
+ * The programs are the default build (no -cl-denorms-are-zero, no -cl-fast-relaxed-math); as the
+ * reference host does (Bellard_192.hsp:472-474), a call whose loop count has 10 d + 9 >= 2^53
+ * runs the kernels of Bellard.cl built with -D OVERDOUBLE (Bellard_od.gfx950.co).
  */
 #define CL_TARGET_OPENCL_VERSION 120
 #include <CL/cl.h>
@@ -37,8 +38,10 @@ static const uint64_t kDen1[7] = {1, 3, 1, 3, 5, 7, 9};
 
 static cl_context g_ctx;
 static cl_command_queue g_q;
-static cl_program g_prg, g_prg2;
-static cl_kernel g_k32[3], g_k64[3], g_kafter[3], g_ksum[3], g_ksum7[3], g_kfill;
+static cl_program g_prg, g_prg2, g_prg_od;
+/* [build][precision]: build 0 the default program, 1 Bellard.cl with -D OVERDOUBLE */
+static cl_kernel g_k32[2][3], g_k64[2][3], g_kafter[2][3], g_ksum[2][3], g_ksum7[2][3], g_kfill;
+static int g_od; /* the build of the current call */
 static cl_mem g_big[3], g_small[3], g_ans[3];
 static char g_err[512];
 static double g_last_ms;
@@ -115,6 +118,9 @@ int ref_init(const char *co_dir) {
   snprintf(path, sizeof path, "%s/Bellard_div2by1.gfx950.co", co_dir);
   g_prg2 = load_program(dev, path, &e);
   if (e != CL_SUCCESS) return failf("Bellard_div2by1 program", e);
+  snprintf(path, sizeof path, "%s/Bellard_od.gfx950.co", co_dir);
+  g_prg_od = load_program(dev, path, &e);
+  if (e != CL_SUCCESS) return failf("Bellard -D OVERDOUBLE program", e);
   /* Bellard_192.hsp:388-405 */
   const char *k32[3] = {"Sglobal32mtg", "Sglobal32mtg_192", "Sglobal32mtg_192"};
   const char *k64[3] = {"Sglobal64mtg", "Sglobal64mtg_192", "Sglobal64mtg_192_Refresh"};
@@ -122,16 +128,19 @@ int ref_init(const char *co_dir) {
   const char *ksm[3] = {"Ssum", "Ssum_192", "Ssum_192"};
   const char *ks7[3] = {"Sum7", "Sum7_192", "Sum7_192"};
   for (int p = 0; p < 3; ++p) {
-    g_k32[p] = clCreateKernel(g_prg, k32[p], &e);
-    if (e != CL_SUCCESS) return failf(k32[p], e);
-    g_k64[p] = clCreateKernel(p == 2 ? g_prg2 : g_prg, k64[p], &e);
-    if (e != CL_SUCCESS) return failf(k64[p], e);
-    g_kafter[p] = clCreateKernel(g_prg, kaf[p], &e);
-    if (e != CL_SUCCESS) return failf(kaf[p], e);
-    g_ksum[p] = clCreateKernel(g_prg, ksm[p], &e);
-    if (e != CL_SUCCESS) return failf(ksm[p], e);
-    g_ksum7[p] = clCreateKernel(g_prg, ks7[p], &e);
-    if (e != CL_SUCCESS) return failf(ks7[p], e);
+    for (int b = 0; b < 2; ++b) {
+      cl_program prg = b ? g_prg_od : g_prg;
+      g_k32[b][p] = clCreateKernel(prg, k32[p], &e);
+      if (e != CL_SUCCESS) return failf(k32[p], e);
+      g_k64[b][p] = clCreateKernel(p == 2 ? g_prg2 : prg, k64[p], &e);
+      if (e != CL_SUCCESS) return failf(k64[p], e);
+      g_kafter[b][p] = clCreateKernel(prg, kaf[p], &e);
+      if (e != CL_SUCCESS) return failf(kaf[p], e);
+      g_ksum[b][p] = clCreateKernel(prg, ksm[p], &e);
+      if (e != CL_SUCCESS) return failf(ksm[p], e);
+      g_ksum7[b][p] = clCreateKernel(prg, ks7[p], &e);
+      if (e != CL_SUCCESS) return failf(ks7[p], e);
+    }
     /* Bellard_192.hsp:409-411: 8 * (2 + (precision != 0)) bytes per slot */
     const size_t w = 8u * (p == 0 ? 2u : 3u);
     g_big[p] = clCreateBuffer(g_ctx, CL_MEM_READ_WRITE, BIGSUM_SIZE * w, NULL, &e);
@@ -180,7 +189,7 @@ static int set0mem(int p) { return fill0(g_big[p], BIGSUM_SIZE * (p == 0 ? 2u : 
 static int after(int p, uint64_t d, int j) {
   const cl_long sign = kNumeSign[j];
   const cl_ulong dd = d, den0 = kDen0[j], den1 = kDen1[j];
-  cl_kernel k = g_kafter[p];
+  cl_kernel k = g_kafter[g_od][p];
   clSetKernelArg(k, 0, sizeof(cl_mem), &g_big[p]);
   clSetKernelArg(k, 1, sizeof(cl_ulong), &dd);
   clSetKernelArg(k, 2, sizeof(cl_long), &sign);
@@ -191,7 +200,7 @@ static int after(int p, uint64_t d, int j) {
 
 /* Ssum 65536 -> 256, then 256 -> 1 (KernelFuncs.as:61-65). */
 static int ssum(int p) {
-  cl_kernel s = g_ksum[p];
+  cl_kernel s = g_ksum[g_od][p];
   clSetKernelArg(s, 0, sizeof(cl_mem), &g_big[p]);
   clSetKernelArg(s, 1, sizeof(cl_mem), &g_small[p]);
   if (launch(s, BIGSUM_SIZE, LWS)) return -1;
@@ -208,13 +217,15 @@ static int batch_calc(int p, uint64_t d, uint64_t k_start, uint64_t k_end, int j
   if (loopmax32 > k_end) loopmax32 = k_end;
   while (offset < loopmax32) {
     k_max = k_max + g_gws * 16 < loopmax32 ? k_max + g_gws * 16 : loopmax32;
-    if (set_sglobal(g_k32[p], p, offset, d, k_max, j) || launch(g_k32[p], g_gws, LWS)) return -1;
+    cl_kernel k = g_k32[g_od][p];
+    if (set_sglobal(k, p, offset, d, k_max, j) || launch(k, g_gws, LWS)) return -1;
     offset = k_max;
   }
   const uint64_t loopmax64 = k_end < d ? k_end : d;
   while (offset < loopmax64) {
     k_max = k_max + g_gws * 32 < loopmax64 ? k_max + g_gws * 32 : loopmax64;
-    if (set_sglobal(g_k64[p], p, offset, d, k_max, j) || launch(g_k64[p], g_gws, LWS)) return -1;
+    cl_kernel k = g_k64[g_od][p];
+    if (set_sglobal(k, p, offset, d, k_max, j) || launch(k, g_gws, LWS)) return -1;
     offset = k_max;
   }
   if (offset == d && after(p, d, j)) return -1;
@@ -224,7 +235,7 @@ static int batch_calc(int p, uint64_t d, uint64_t k_start, uint64_t k_end, int j
 /* KernelFuncs.as:68-74 */
 static int add_ans(int p, int j) {
   const cl_ulong numer = kNumer[j];
-  cl_kernel k = g_ksum7[p];
+  cl_kernel k = g_ksum7[g_od][p];
   clSetKernelArg(k, 0, sizeof(cl_mem), &g_ans[p]);
   clSetKernelArg(k, 1, sizeof(cl_mem), &g_big[p]);
   clSetKernelArg(k, 2, sizeof(cl_ulong), &numer);
@@ -242,11 +253,14 @@ static double now_ms(void) {
   return ts.tv_sec * 1e3 + ts.tv_nsec / 1e6;
 }
 
+/* Picks the program build for loop count d (Bellard_192.hsp:472-474); moduli up to 10 d + 9 must
+ * stay below 2^63 (the -D OVERDOUBLE kernels' condition, Bellard.cl:1041). */
 static int check_d(uint64_t d) {
-  if (d * 10 + 9 >= 9007199254740992ull) { /* Bellard_192.hsp:472: needs the OVERDOUBLE build */
-    snprintf(g_err, sizeof g_err, "d = %llu needs the -D OVERDOUBLE build", (unsigned long long)d);
+  if (d > (9223372036854775807ull - 9) / 10) {
+    snprintf(g_err, sizeof g_err, "d = %llu: moduli reach 2^63", (unsigned long long)d);
     return -1;
   }
+  g_od = d * 10 + 9 >= 9007199254740992ull;
   return 0;
 }
 
@@ -294,7 +308,7 @@ int ref_op(int op, int precision, uint64_t a, uint64_t b, uint64_t c, int j, uin
     case 0: rc = set0mem(p); break;
     case 1:
     case 2: {
-      cl_kernel k = op == 1 ? g_k32[p] : g_k64[p];
+      cl_kernel k = op == 1 ? g_k32[g_od][p] : g_k64[g_od][p];
       rc = set_sglobal(k, p, a, b, c, j) || launch(k, g_gws, LWS);
       break;
     }
@@ -308,11 +322,13 @@ int ref_op(int op, int precision, uint64_t a, uint64_t b, uint64_t c, int j, uin
 void ref_shutdown(void) {
   if (g_q) clFinish(g_q);
   for (int p = 0; p < 3; ++p) {
-    if (g_k32[p]) clReleaseKernel(g_k32[p]);
-    if (g_k64[p]) clReleaseKernel(g_k64[p]);
-    if (g_kafter[p]) clReleaseKernel(g_kafter[p]);
-    if (g_ksum[p]) clReleaseKernel(g_ksum[p]);
-    if (g_ksum7[p]) clReleaseKernel(g_ksum7[p]);
+    for (int b = 0; b < 2; ++b) {
+      if (g_k32[b][p]) clReleaseKernel(g_k32[b][p]);
+      if (g_k64[b][p]) clReleaseKernel(g_k64[b][p]);
+      if (g_kafter[b][p]) clReleaseKernel(g_kafter[b][p]);
+      if (g_ksum[b][p]) clReleaseKernel(g_ksum[b][p]);
+      if (g_ksum7[b][p]) clReleaseKernel(g_ksum7[b][p]);
+    }
     if (g_big[p]) clReleaseMemObject(g_big[p]);
     if (g_small[p]) clReleaseMemObject(g_small[p]);
     if (g_ans[p]) clReleaseMemObject(g_ans[p]);
@@ -320,9 +336,19 @@ void ref_shutdown(void) {
   if (g_kfill) clReleaseKernel(g_kfill);
   if (g_prg) clReleaseProgram(g_prg);
   if (g_prg2) clReleaseProgram(g_prg2);
+  if (g_prg_od) clReleaseProgram(g_prg_od);
   if (g_q) clReleaseCommandQueue(g_q);
   if (g_ctx) clReleaseContext(g_ctx);
   memset(g_k32, 0, sizeof g_k32);
+  memset(g_k64, 0, sizeof g_k64);
+  memset(g_kafter, 0, sizeof g_kafter);
+  memset(g_ksum, 0, sizeof g_ksum);
+  memset(g_ksum7, 0, sizeof g_ksum7);
+  memset(g_big, 0, sizeof g_big);
+  memset(g_small, 0, sizeof g_small);
+  memset(g_ans, 0, sizeof g_ans);
+  g_kfill = NULL;
+  g_prg = g_prg2 = g_prg_od = NULL;
   g_q = NULL;
   g_ctx = NULL;
 }

@@ -348,6 +348,7 @@ def loopmax32(j: int) -> int:
 # full 16 or 32 terms on ranges the CPU restatement finishes in seconds.
 REF_DD_GOLDEN = os.path.join(GOLDEN, "ref_dd192.json")
 REF_DD106_GOLDEN = os.path.join(GOLDEN, "ref_dd106.json")
+REF_OD_GOLDEN = os.path.join(GOLDEN, "ref_dd_overdouble.json")
 
 
 def ref_dd_cases():
@@ -390,6 +391,30 @@ def ref_dd106_cases():
         cases.append((d10, lo - 3001, lo + 9000, j, 256))     # Sglobal32mtg, then Sglobal64mtg
         cases.append((d10, d10 - 5000, d10 + 3, j, 512))      # Sglobal64mtg and the tail
         cases.append((d12, d12 // 2 | 1, d12 // 2 + 8192, j, 256))
+    return cases
+
+
+def ref_od_cases():
+    """Jobs (d, k_start, k_end, j, precision, work-items per launch) at loop counts with
+    10 d + 9 >= 2^53, where the reference host builds its kernels with -D OVERDOUBLE
+    (Bellard_192.hsp:472-474), in all three precisions: moduli below 2^53 (the smallest such d),
+    across 2^53 inside one launch, above it (den0 4 and 10), and above 2^55 (the wide engine)."""
+    d_a = 900719925474099           # the smallest: 10 d + 9 >= 2^53, every modulus below 2^53
+    d_b = 1000000000000001
+    d_c = 3000000000000001
+    d_d = 5000000000000001          # moduli up to 5e16 > 2^55
+    k53 = (1 << 53) // 10           # den0 = 10: m crosses 2^53 at k = k53
+    cases = []
+    for p in (0, 1, 2):
+        cases += [
+            (d_a, d_a - 4000, d_a, 2, p, 256),              # the tail, moduli below 2^53
+            (d_b, 0, 3000, 0, p, 256),                      # Sglobal32mtg alone
+            (d_b, loopmax32(3) - 1500, loopmax32(3) + 2500, 3, p, 256),  # both head kernels
+            (d_b, k53 - 1000, k53 + 2000, 4, p, 256),       # across 2^53
+            (d_b, d_b - 4000, d_b, 6, p, 256),              # above 2^53, and the tail
+            (d_c, d_c - 3000, d_c, 0, p, 512),              # den0 4 above 2^53
+            (d_d, d_d - 2500, d_d + 5, 5, p, 256),          # above 2^55, k_end past d
+        ]
     return cases
 
 

@@ -1,14 +1,16 @@
 """CPU: the double-double replay of the reference's kernels (orc_ref_batch192 for precision 1/2,
 orc_ref_job106 / orc_ref_normal106 for the 106-bit mode) against the reference kernels' own
 results in tests/golden/ref_dd192.json and ref_dd106.json (written by
-tests/golden/make_ref_dd_fixtures.py on an MI355X with oracle/_ref), bit for bit, and the
-replay's distance from the exact sums within the double-double error the reference carries."""
+tests/golden/make_ref_dd_fixtures.py on an MI355X with oracle/_ref) and, for loop counts with
+10 d + 9 >= 2^53 (the -D OVERDOUBLE build), ref_dd_overdouble.json (make_ref_od_fixtures.py), bit
+for bit, and the replay's distance from the exact sums within the double-double error the
+reference carries."""
 import json
 
 import pytest
 
-from clb_testlib import (M192, REF_DD106_GOLDEN, REF_DD_GOLDEN, SHIFT, load_oracle, py_batch_calc,
-                         py_normalize, ref_dd106_cases, ref_dd_cases)
+from clb_testlib import (M192, REF_DD106_GOLDEN, REF_DD_GOLDEN, REF_OD_GOLDEN, SHIFT, load_oracle,
+                         py_batch_calc, py_normalize, ref_dd106_cases, ref_dd_cases, ref_od_cases)
 
 
 @pytest.fixture(scope="module")
@@ -71,3 +73,38 @@ def test_106bit_positions_equal_reference_kernels(orc, row):
 def test_106bit_fixture_covers_the_cases():
     have = {(c["d"], c["k0"], c["k1"], c["j"], c["gws"]) for c in golden106()["cases"]}
     assert have == set(ref_dd106_cases())
+
+
+def golden_od():
+    with open(REF_OD_GOLDEN) as f:
+        return json.load(f)["cases"]
+
+
+def test_overdouble_fixture_covers_the_cases():
+    have = {(c["d"], c["k0"], c["k1"], c["j"], c["precision"], c["gws"]) for c in golden_od()}
+    assert have == set(ref_od_cases())
+
+
+@pytest.mark.parametrize("case", golden_od(), ids=case_id)
+def test_overdouble_replay_equals_reference_kernels(orc, case):
+    """The -D OVERDOUBLE build's kernels (dd_div of split numerators and moduli) replayed."""
+    d, k0, k1, j, p, gws = (case[k] for k in ("d", "k0", "k1", "j", "precision", "gws"))
+    if p == 0:
+        want = (float.fromhex(case["hi"]), float.fromhex(case["lo"]))
+        assert orc.ref_job106(d, k0, k1, j, gws) == want
+    else:
+        got = (orc.ref_batch192(d, k0, k1, j, p, gws) << SHIFT[j]) & M192
+        assert got == int(case["part"], 16)
+
+
+@pytest.mark.parametrize("case", [c for c in ref_od_cases() if c[4] == 1],
+                         ids=lambda c: "d{}_k{}-{}_j{}".format(*c[:4]))
+def test_overdouble_replay_close_to_exact(orc, case):
+    """The -D OVERDOUBLE kernels stay within the same 2 units of 2^-192 per term of the exact sum
+    (the split numerator is exact, so dd_div adds no error of its own past 2^53)."""
+    d, k0, k1, j, p, gws = case
+    exact = py_batch_calc(d, k0, k1, j)
+    diff = (orc.ref_batch192(d, k0, k1, j, p, gws) - exact) & M192
+    diff = diff - (1 << 192) if diff >> 191 else diff
+    terms = min(k1, d) - k0 + (20 if k1 >= d else 0)
+    assert abs(diff) <= 2 * terms, diff

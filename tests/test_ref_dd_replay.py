@@ -15,7 +15,7 @@ import pytest
 
 import clbellard_amd as clb
 from clb_testlib import (M192, REF_LIB, SHIFT, RefKernels, load_oracle, py_normalize,
-                         ref_dd106_cases, ref_dd_cases, w2i)
+                         ref_dd106_cases, ref_dd_cases, ref_od_cases, w2i)
 
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not os.path.exists(REF_LIB),
@@ -115,12 +115,35 @@ def test_replay_arguments(ctxs):
         assert e.value.code == clb.EINVAL
     ctx.set_replay(clb.REF_GLOBAL_WORK_SIZE)
     try:
-        d_over = (1 << 53) // 10  # 10 d + 9 >= 2^53: the reference's OVERDOUBLE build
+        d_over = ((1 << 63) - 9) // 10 + 1  # moduli reach 2^63 (Bellard.cl:1041 needs less)
         with pytest.raises(clb.ClbError) as e:
             ctx.run_jobs([(d_over, d_over - 100, d_over, 0)])
         assert e.value.code == clb.ERANGE
     finally:
         ctx.set_replay(0)
+
+
+@pytest.mark.parametrize("case", ref_od_cases(), ids=lambda c: "d{}_k{}-{}_j{}_p{}_g{}".format(*c))
+def test_hip_overdouble_replay_equals_reference_kernels(ref, ctxs, case):
+    """Loop counts with 10 d + 9 >= 2^53: the reference's kernels built with -D OVERDOUBLE
+    (dd_div of the split numerator and modulus from 2^53 on), all three precisions, residues
+    from the integer engine (3-bit windows below 2^55, the wide engine above)."""
+    d, k0, k1, j, p, gws = case
+    note(f"OVERDOUBLE replay d={d} k=[{k0},{k1}) j={j} precision={p} gws={gws}")
+    ref.set_gws(gws)
+    if p == 0:
+        with clb.Context(0, clb.PREC_DD106) as c:
+            c.set_replay(gws)
+            assert dd_of_words(c.run_jobs([(d, k0, k1, j)])[0]) == ref.job(d, k0, k1, j, 0)
+        return
+    ctx = ctxs[(p, "plain")]
+    ctx.set_replay(gws)
+    try:
+        got = w2i(ctx.run_jobs([(d, k0, k1, j)])[0])
+    finally:
+        ctx.set_replay(0)
+    want = w2i(ref.job(d, k0, k1, j, p))
+    assert got == want, f"{got:048x} != {want:048x}"
 
 
 @pytest.mark.parametrize("p", [1, 2])
