@@ -13,7 +13,8 @@
   consecutive draws (each launch about 75 s, one test item each, so that progress is reported
   between them); a size-spread sample is then recomputed one position at a time on an untuned,
   eager context and must be bit-identical, and the smallest positions also equal the oracle.
-  About 10 minutes.
+  About 10 minutes.  The 64 smallest positions are also batched in the 106-bit mode with the
+  reference's double-double kernels replayed, against single runs.
 """
 import json
 import os
@@ -96,3 +97,16 @@ def test_batched_positions_equal_single_runs(oracle):
             assert single.normal(POSITIONS[i])[0] == RESULTS[i], (i, POSITIONS[i])
     for i in by_size[:2]:
         assert w2i(RESULTS[i]) == oracle.normal(POSITIONS[i]), (i, POSITIONS[i])
+
+
+def test_batched_positions_106bit_replay_equal_single_runs():
+    """The same draw's 64 smallest positions in the 106-bit mode, the reference's double-double
+    kernels replayed (each position's seven BatchCalcs on their own 65536 group accumulators, then
+    Sum7): the batched double-doubles equal single runs bit for bit."""
+    pos = sorted(POSITIONS)[:64]
+    note(f"106-bit replay: {len(pos)} positions {pos[0]}..{pos[-1]} batched, then one by one")
+    with clb.Context(0, clb.PREC_DD106) as c:
+        c.set_replay(clb.REF_GLOBAL_WORK_SIZE)
+        batched = c.batch_positions(pos)
+        for n, b in zip(pos, batched):
+            assert c.normal(n)[0] == b, n
