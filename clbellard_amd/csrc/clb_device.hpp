@@ -249,9 +249,11 @@ __device__ __forceinline__ uint64_t isq(uint64_t h, uint64_t m, uint64_t mp) {
 // (v_mad_u64_u32) on 32-bit words: h < 2^56 has an upper word below 2^24 and m one below 2^23, so
 // no column carries out of 64 bits and the REDC needs no 128-bit intermediate.  The product
 // T = h (h 2^c) (four words t0..t3, c <= 7) takes 4 products, 3 for the plain square
-// (h0 (2 h1) folds the doubled cross term); q = (t1:t0) mp mod 2^64 three; the high half of q m
-// four.  REDC(T) = (t3:t2) + hi(q m) + [(t1:t0) != 0], since (q m) mod 2^64 = 2^64 - (t1:t0)
-// whenever (t1:t0) != 0.  Ten or eleven multiplies instead of twelve, and few 64-bit adds.
+// (h0 (2 h1) folds the doubled cross term); q = (t1:t0) mp mod 2^64 three; q m + (t1:t0) four,
+// with the low words t0, t1 as addends of the first two columns: that sum is a multiple of 2^64
+// (its low words come out 0), so its high half is exactly (q m + (t1:t0)) / 2^64 and
+// REDC(T) = (t3:t2) + that half, in [0, 2m), with no separate carry.  Every column fits 64 bits:
+// (2^32 - 1)^2 + 2 (2^32 - 1) = 2^64 - 1.  Ten or eleven multiplies, few adds.
 __device__ __forceinline__ uint64_t redc_words(uint32_t t0, uint32_t t1, uint32_t t2, uint32_t t3,
                                               uint64_t m, uint64_t mp) {
   const uint32_t m0 = (uint32_t)m, m1 = (uint32_t)(m >> 32);
@@ -259,11 +261,11 @@ __device__ __forceinline__ uint64_t redc_words(uint32_t t0, uint32_t t1, uint32_
   const uint64_t qa = (uint64_t)t0 * mp0;
   const uint32_t q0 = (uint32_t)qa;
   const uint32_t q1 = (uint32_t)(qa >> 32) + t0 * mp1 + t1 * mp0;
-  const uint64_t e = (uint64_t)q0 * m0;
-  const uint64_t f = (uint64_t)q1 * m0 + (e >> 32);
-  const uint64_t g = (uint64_t)q0 * m1 + (uint32_t)f;        // m1 < 2^23
+  const uint64_t e = (uint64_t)q0 * m0 + t0;                  // low word 0
+  const uint64_t f = (uint64_t)q1 * m0 + ((e >> 32) + t1);
+  const uint64_t g = (uint64_t)q0 * m1 + (uint32_t)f;         // m1 < 2^23; low word 0
   const uint64_t hi = (uint64_t)q1 * m1 + (f >> 32) + (g >> 32);
-  return (((uint64_t)t3 << 32) | t2) + hi + ((t0 | t1) != 0);
+  return (((uint64_t)t3 << 32) | t2) + hi;
 }
 
 __device__ __forceinline__ uint64_t isq55(uint64_t h, uint64_t m, uint64_t mp) {

@@ -396,12 +396,17 @@ def _redc_words(t0, t1, t2, t3, m, mp):
     qa = t0 * mp0
     q0 = qa & 0xFFFFFFFF
     q1 = ((qa >> 32) + t0 * mp1 + t1 * mp0) & 0xFFFFFFFF
-    e = q0 * m0
-    f = q1 * m0 + (e >> 32)
+    e = q0 * m0 + t0
+    f = q1 * m0 + ((e >> 32) + t1)
     g = q0 * m1 + (f & 0xFFFFFFFF)
     hi = q1 * m1 + (f >> 32) + (g >> 32)
     assert max(qa, e, f, g, hi) <= M64
-    return (((t3 << 32) | t2) + hi + (1 if (t0 | t1) else 0)) & M64
+    assert e & 0xFFFFFFFF == 0 and g & 0xFFFFFFFF == 0  # the low words of q m + (t1:t0) vanish
+    q = (q1 << 32) | q0
+    assert hi == (q * m + ((t1 << 32) | t0)) >> 64
+    r = ((t3 << 32) | t2) + hi
+    assert r <= M64
+    return r
 
 
 def _isq55_shift(h, c, m, mp):
