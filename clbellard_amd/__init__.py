@@ -27,7 +27,8 @@ from typing import Iterable, List, Optional, Sequence, Tuple
 __all__ = [
     "ClbError", "lib", "LIB_PATH", "normalize", "formula", "hex48", "hex_dd", "u192_to_dd",
     "work_sum", "parse_worklist", "write_work_file", "device_count", "device_name", "Context",
-    "u192_to_int", "int_to_u192", "issue_rate", "PEAK_OPS",
+    "u192_to_int", "int_to_u192", "issue_rate", "PEAK_OPS", "read_schedule", "write_schedule",
+    "SCHEDULE_KEYS",
     "OK", "EINVAL", "EDEVICE", "ERANGE", "EIO", "ENOMEM", "EFORMAT",
     "PREC_DD106", "PREC_U192", "PREC_U192_BETA", "KERNEL_AUTO", "KERNEL_PLAIN", "KERNEL_SPLIT",
     "NUMER", "NUMESIGN", "DEN0", "DEN1", "REF_GLOBAL_WORK_SIZE", "NOALLOC_MAX_JOBS",
@@ -61,6 +62,17 @@ class ClbError(RuntimeError):
         self.code = code
 
 
+class _Schedule(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_char * 256)] + [
+        (k, ctypes.c_int) for k in ("terms_per_lane", "min_tiles_per_wave", "blocks_per_cu",
+                                    "waves_per_group", "min_chunk", "chunks_per_wave",
+                                    "max_replicas")]
+
+
+SCHEDULE_KEYS = ("terms_per_lane", "min_tiles_per_wave", "blocks_per_cu", "waves_per_group",
+                 "min_chunk", "chunks_per_wave", "max_replicas")
+
+
 class _Job(ctypes.Structure):
     _fields_ = [("d", ctypes.c_uint64), ("k_begin", ctypes.c_uint64), ("k_end", ctypes.c_uint64),
                 ("formula", ctypes.c_int32), ("reserved", ctypes.c_int32)]
@@ -87,6 +99,10 @@ _SIGNATURES = {
     "clb_get_schedule": ([_CTX] + [ctypes.POINTER(ctypes.c_int)] * 6, ctypes.c_int),
     "clb_autotune": ([_CTX, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(ctypes.c_int),
                       ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "clb_read_schedule": ([ctypes.c_char_p, ctypes.POINTER(_Schedule)], ctypes.c_int),
+    "clb_write_schedule": ([ctypes.c_char_p, ctypes.POINTER(_Schedule)], ctypes.c_int),
+    "clb_save_schedule": ([_CTX, ctypes.c_char_p], ctypes.c_int),
+    "clb_load_schedule": ([_CTX, ctypes.c_char_p], ctypes.c_int),
     "clb_normalize": ([_U64, _P64, _P64, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "clb_formula": ([ctypes.c_int, _P64, ctypes.POINTER(ctypes.c_int64), _P64, _P64],
                     ctypes.c_int),
@@ -222,6 +238,24 @@ def write_work_file(work_dir: str, j: int, b: int, w: Sequence[int], precision: 
                                    device_label.encode(), ms))
 
 
+def read_schedule(path: str) -> dict:
+    """A launch-settings file (clb_read_schedule): {"device": name, "terms_per_lane": ..., ...}."""
+    s = _Schedule()
+    _check(lib.clb_read_schedule(os.fsencode(path), ctypes.byref(s)))
+    out = {"device": s.device.decode(errors="replace")}
+    out.update({k: getattr(s, k) for k in SCHEDULE_KEYS})
+    return out
+
+
+def write_schedule(path: str, settings: dict) -> None:
+    """Writes a launch-settings file (clb_write_schedule) from read_schedule's dict form."""
+    s = _Schedule()
+    s.device = settings["device"].encode()
+    for k in SCHEDULE_KEYS:
+        setattr(s, k, settings[k])
+    _check(lib.clb_write_schedule(os.fsencode(path), ctypes.byref(s)))
+
+
 def parse_worklist(path: str) -> Tuple[int, int, List[Tuple[int, int]]]:
     """Worklist -> (digits, precision (-1 if invalid), [(formula, batch), ...])."""
     digits, prec, n = _U64(), ctypes.c_int(), ctypes.c_size_t()
@@ -312,6 +346,14 @@ class Context:
         keys = ("terms_per_lane", "blocks_per_cu", "waves_per_group", "min_chunk",
                 "chunks_per_wave", "max_replicas")
         return dict(zip(keys, (x.value for x in v)))
+
+    def save_schedule(self, path: str) -> None:
+        """Writes the context's launch settings with its device's name (clb_save_schedule)."""
+        _check(lib.clb_save_schedule(self._ctx, os.fsencode(path)))
+
+    def load_schedule(self, path: str) -> None:
+        """Applies a launch-settings file written for this device (clb_load_schedule)."""
+        _check(lib.clb_load_schedule(self._ctx, os.fsencode(path)))
 
     def autotune(self, n: int, budget_ms: float = 2000.0) -> Tuple[int, int]:
         """Time tile lengths and grid sizes on a sample of position n; keep and return the

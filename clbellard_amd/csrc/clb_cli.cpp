@@ -2,20 +2,24 @@
 // (Bellard_192.hsp normal mode and work mode) and workmodeSum0_6.exe.
 //
 //   clbellard normal <digits> [--precision 0|1|2] [--device i] [--kernel auto|plain|split]
-//                    [--replay | --replay-gws GWS]
+//                    [--replay | --replay-gws GWS] [--autotune MS] [--tune-cache FILE]
 //   clbellard work <worklist.txt> [--workdir work] [--batch-size B] [--device i]
 //                  [--launch-seconds S] [--launch-jobs J] [--replay | --replay-gws GWS]
+//                  [--autotune MS] [--tune-cache FILE]
 //   clbellard sum [--workdir work]
 //   clbellard batch <positions.txt | -> [--precision 0|1|2] [--device i] [--graph]
-//                   [--autotune MS] [--replay | --replay-gws GWS]
+//                   [--autotune MS] [--tune-cache FILE] [--replay | --replay-gws GWS]
 // work: the pending jobs run in launches of at most S estimated seconds (default 60) and J jobs
 // (default: any number), each launch's files written before the next (clb_set_worklist_launch).
 // --replay: the reference kernels' own output bits (clb_set_replay) with the reference's
 // 16777216 work-items per launch; --replay-gws: with GWS work-items per launch.
 // batch: one position per line (file, or standard input for "-"), all of them in one persistent
 // launch (clb_batch_positions); prints "N' digits" per position in input order.  --graph replays
-// the captured launch sequence (clb_set_graph); --autotune MS first tunes tile length and grid on
-// the largest position for about MS milliseconds (clb_autotune).
+// the captured launch sequence (clb_set_graph).
+// --autotune MS first tunes the launch settings for about MS milliseconds (clb_autotune) on the
+// position (normal), the worklist's position (work) or the largest position (batch);
+// --tune-cache FILE loads them from FILE when it exists (clb_load_schedule; it must have been
+// written for this device), and otherwise writes the tuned settings there (clb_save_schedule).
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
@@ -29,12 +33,13 @@ static int usage() {
   fprintf(stderr,
           "usage:\n"
           "  clbellard normal <digits> [--precision 0|1|2] [--device i] [--kernel auto|plain|split]\n"
-          "                   [--replay | --replay-gws GWS]\n"
+          "                   [--replay | --replay-gws GWS] [--autotune MS] [--tune-cache FILE]\n"
           "  clbellard work <worklist.txt> [--workdir DIR] [--batch-size B] [--device i]\n"
           "                 [--launch-seconds S] [--launch-jobs J] [--replay | --replay-gws GWS]\n"
+          "                 [--autotune MS] [--tune-cache FILE]\n"
           "  clbellard sum [--workdir DIR]\n"
           "  clbellard batch <positions.txt | -> [--precision 0|1|2] [--device i] [--graph]\n"
-          "                  [--autotune MS] [--replay | --replay-gws GWS]\n");
+          "                  [--autotune MS] [--tune-cache FILE] [--replay | --replay-gws GWS]\n");
   return 2;
 }
 
@@ -111,6 +116,7 @@ int main(int argc, char** argv) {
   double launch_seconds = 60.0;
   bool graph = false;
   double tune_ms = 0.0;
+  std::string tune_cache;
   // Every option is checked here, before any GPU call: a malformed one prints the usage.
   for (int i = 2; i < argc; ++i) {
     const std::string a = argv[i];
@@ -141,6 +147,8 @@ int main(int argc, char** argv) {
       char* end = nullptr;
       tune_ms = strtod(v, &end);
       if (end == v || *end != 0 || !(tune_ms > 0.0 && tune_ms < 1e7)) return usage();
+    } else if (a == "--tune-cache" && (v = next()) && *v) {
+      tune_cache = v;
     } else if (a == "--replay") {
       replay = CLB_REF_GLOBAL_WORK_SIZE;
     } else if (a == "--replay-gws" && parse_u64(v = next(), &u) && u > 0) {
@@ -180,10 +188,44 @@ int main(int argc, char** argv) {
       fprintf(stderr, "position %llu: %s\n", (unsigned long long)n, clb_last_error());
       return usage();
     }
+  // The position the launch settings are tuned on, and a cache file that is there already (read
+  // and checked here; a missing one is written after tuning).
+  uint64_t tune_n = digits;
+  for (uint64_t n : pos) tune_n = n > tune_n ? n : tune_n;
+  if (cmd == "work" && tune_ms > 0.0) {
+    uint64_t wl_digits = 0;
+    size_t njobs = 0;
+    if (clb_parse_worklist(positional.c_str(), &wl_digits, nullptr, nullptr, 0, &njobs) != CLB_OK ||
+        clb_normalize(wl_digits, nullptr, nullptr, nullptr) != CLB_OK) {
+      fprintf(stderr, "%s: %s\n", positional.c_str(), clb_last_error());
+      return usage();
+    }
+    tune_n = wl_digits;
+  }
+  bool cached = false;
+  if (!tune_cache.empty()) {
+    clb_schedule s;
+    const int r = clb_read_schedule(tune_cache.c_str(), &s);
+    if (r != CLB_OK && r != CLB_EIO) {
+      fprintf(stderr, "%s\n", clb_last_error());
+      return usage();
+    }
+    cached = r == CLB_OK;
+  }
   clb_ctx* ctx = nullptr;
   if (int rc = clb_ctx_create(device, precision, &ctx)) return die(rc);
   int rc = clb_set_kernel(ctx, kernel);
   if (!rc && replay) rc = clb_set_replay(ctx, replay);
+  if (!rc && cached) {
+    rc = clb_load_schedule(ctx, tune_cache.c_str());
+    if (!rc) fprintf(stderr, "launch settings from %s\n", tune_cache.c_str());
+  } else if (!rc && tune_ms > 0.0 && tune_n > 0) {
+    rc = clb_autotune(ctx, tune_n, tune_ms, nullptr, nullptr);
+    if (!rc && !tune_cache.empty()) {
+      rc = clb_save_schedule(ctx, tune_cache.c_str());
+      if (!rc) fprintf(stderr, "launch settings written to %s\n", tune_cache.c_str());
+    }
+  }
   if (!rc && cmd == "normal") {
     uint64_t w[3], n_out = 0;
     char hex[49];
@@ -194,10 +236,7 @@ int main(int argc, char** argv) {
       printf("digits=%llu\n%s\nAll calc time %.0fms\n", (unsigned long long)n_out, hex, total);
     }
   } else if (!rc && cmd == "batch") {
-    uint64_t largest = 0;
-    for (uint64_t n : pos) largest = n > largest ? n : largest;
-    if (!pos.empty() && tune_ms > 0.0) rc = clb_autotune(ctx, largest, tune_ms, nullptr, nullptr);
-    if (!rc && graph) rc = clb_set_graph(ctx, 1);
+    if (graph) rc = clb_set_graph(ctx, 1);
     std::vector<uint64_t> frac(3 * pos.size());
     if (!rc && !pos.empty()) rc = clb_batch_positions(ctx, pos.data(), pos.size(), frac.data());
     for (size_t i = 0; !rc && i < pos.size(); ++i) {

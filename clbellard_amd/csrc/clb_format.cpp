@@ -309,6 +309,70 @@ int parse_worklist(const char* path, Worklist* wl) {
   return CLB_OK;
 }
 
+// The launch-settings file: keys, and the ranges their setters accept.
+struct SchedKey {
+  const char* name;
+  int clb_schedule::*field;
+  int lo, hi;
+};
+static const SchedKey kSchedKeys[] = {
+    {"terms_per_lane", &clb_schedule::terms_per_lane, 0, 16},
+    {"min_tiles_per_wave", &clb_schedule::min_tiles_per_wave, 0, 1024},
+    {"blocks_per_cu", &clb_schedule::blocks_per_cu, 0, 64},
+    {"waves_per_group", &clb_schedule::waves_per_group, 0, 8},
+    {"min_chunk", &clb_schedule::min_chunk, 0, 65536},
+    {"chunks_per_wave", &clb_schedule::chunks_per_wave, -1, 64},
+    {"max_replicas", &clb_schedule::max_replicas, 0, 64},
+};
+
+static bool is_pow2_or_zero(int v) { return v >= 0 && (v & (v - 1)) == 0; }
+
+// Values the setters accept (clb_set_tiling, clb_set_launch, clb_set_schedule).
+static int check_schedule(const clb_schedule& s) {
+  const size_t len = strnlen(s.device, sizeof s.device);
+  if (len == 0 || len == sizeof s.device) return fail(CLB_EINVAL, "device name empty or too long");
+  for (size_t i = 0; i < len; ++i)
+    if ((unsigned char)s.device[i] < 0x20 || s.device[i] == 0x7f)
+      return fail(CLB_EINVAL, "device name holds a control character");
+  for (const SchedKey& k : kSchedKeys)
+    if (s.*k.field < k.lo || s.*k.field > k.hi)
+      return fail(CLB_EINVAL, "%s = %d outside [%d, %d]", k.name, s.*k.field, k.lo, k.hi);
+  const int t = s.terms_per_lane;
+  if (t != 0 && t != 1 && t != 4 && t != 16)
+    return fail(CLB_EINVAL, "terms_per_lane must be 0, 1, 4 or 16");
+  const int w = s.waves_per_group;
+  if (w != 0 && w != 1 && w != 2 && w != 4 && w != 8)
+    return fail(CLB_EINVAL, "waves_per_group must be 0, 1, 2, 4 or 8");
+  if (!is_pow2_or_zero(s.max_replicas))
+    return fail(CLB_EINVAL, "max_replicas must be 0 or a power of 2");
+  return CLB_OK;
+}
+
+// A decimal int with an optional minus sign, nothing else.
+static bool parse_int(const std::string& v, int* out) {
+  size_t i = v.size() && v[0] == '-' ? 1 : 0;
+  if (i == v.size() || v.size() - i > 9) return false;
+  long long x = 0;
+  for (; i < v.size(); ++i) {
+    if (v[i] < '0' || v[i] > '9') return false;
+    x = x * 10 + (v[i] - '0');
+  }
+  *out = (int)(v[0] == '-' ? -x : x);
+  return true;
+}
+
+// At most cap bytes of a file; CLB_EFORMAT if it holds more.
+static int read_text_capped(const char* path, size_t cap, std::string* out) {
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(CLB_EIO, "cannot open %s: %s", path, strerror(errno));
+  out->assign(cap + 1, '\0');
+  const size_t n = fread(&(*out)[0], 1, cap + 1, f);
+  fclose(f);
+  if (n > cap) return fail(CLB_EFORMAT, "%s: above %zu bytes", path, cap);
+  out->resize(n);
+  return CLB_OK;
+}
+
 }  // namespace clb
 
 using namespace clb;
@@ -373,6 +437,57 @@ int clb_write_work_file(const char* work_dir, int j, int64_t b, const uint64_t w
   if (j < 0 || j > 6 || b < 0) return fail(CLB_EINVAL, "bad job %d_%lld", j, (long long)b);
   if (precision < 0 || precision > 2) return fail(CLB_EINVAL, "precision must be 0, 1 or 2");
   return write_work_file(work_dir, j, b, w, precision, device_label ? device_label : "", ms);
+}
+
+
+int clb_read_schedule(const char* path, clb_schedule* out) {
+  if (!path || !out) return fail(CLB_EINVAL, "null pointer");
+  std::string text;
+  if (int rc = read_text_capped(path, 4096, &text)) return rc;
+  clb_schedule s;
+  memset(&s, 0, sizeof s);
+  bool seen_device = false;
+  bool seen[sizeof kSchedKeys / sizeof kSchedKeys[0]] = {};
+  for (const std::string& line : split_lines(text)) {
+    if (line.empty()) continue;
+    const size_t eq = line.find('=');
+    if (eq == std::string::npos) return fail(CLB_EFORMAT, "%s: line without '='", path);
+    const std::string key = line.substr(0, eq), val = line.substr(eq + 1);
+    if (key == "device") {
+      if (seen_device) return fail(CLB_EFORMAT, "%s: device given twice", path);
+      if (val.size() >= sizeof s.device) return fail(CLB_EINVAL, "%s: device name too long", path);
+      memcpy(s.device, val.data(), val.size());
+      s.device[val.size()] = 0;
+      seen_device = true;
+      continue;
+    }
+    size_t k = 0;
+    while (k < sizeof kSchedKeys / sizeof kSchedKeys[0] && key != kSchedKeys[k].name) ++k;
+    if (k == sizeof kSchedKeys / sizeof kSchedKeys[0])
+      return fail(CLB_EFORMAT, "%s: unknown key '%.64s'", path, key.c_str());
+    if (seen[k]) return fail(CLB_EFORMAT, "%s: %s given twice", path, kSchedKeys[k].name);
+    if (!parse_int(val, &(s.*kSchedKeys[k].field)))
+      return fail(CLB_EFORMAT, "%s: %s is not an integer", path, kSchedKeys[k].name);
+    seen[k] = true;
+  }
+  if (!seen_device) return fail(CLB_EFORMAT, "%s: no device", path);
+  for (size_t k = 0; k < sizeof kSchedKeys / sizeof kSchedKeys[0]; ++k)
+    if (!seen[k]) return fail(CLB_EFORMAT, "%s: no %s", path, kSchedKeys[k].name);
+  if (int rc = check_schedule(s)) return rc;
+  *out = s;
+  return CLB_OK;
+}
+
+int clb_write_schedule(const char* path, const clb_schedule* s) {
+  if (!path || !s) return fail(CLB_EINVAL, "null pointer");
+  if (int rc = check_schedule(*s)) return rc;
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(CLB_EIO, "cannot create %s: %s", path, strerror(errno));
+  int bad = fprintf(f, "device=%s\n", s->device) < 0;
+  for (const SchedKey& k : kSchedKeys) bad |= fprintf(f, "%s=%d\n", k.name, s->*k.field) < 0;
+  bad |= fclose(f) != 0;
+  if (bad) return fail(CLB_EIO, "cannot write %s", path);
+  return CLB_OK;
 }
 
 int clb_work_sum(const char* work_dir, char hex[49], int counts[7]) {

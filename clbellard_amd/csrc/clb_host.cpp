@@ -1051,6 +1051,36 @@ int clb_get_schedule(clb_ctx* ctx, int* terms_per_lane, int* blocks_per_cu_out,
   return CLB_OK;
 }
 
+int clb_save_schedule(clb_ctx* ctx, const char* path) {
+  if (int rc = check_ctx(ctx)) return rc;
+  clb_schedule s;
+  memset(&s, 0, sizeof s);
+  snprintf(s.device, sizeof s.device, "%s", ctx->device_name);
+  s.terms_per_lane = ctx->tpl_index < 0 ? 0 : kTplChoices[ctx->tpl_index];
+  s.min_tiles_per_wave = ctx->min_tiles_per_wave;
+  s.blocks_per_cu = ctx->blocks_per_cu;
+  s.waves_per_group = ctx->wg_waves;
+  s.min_chunk = ctx->min_chunk;
+  s.chunks_per_wave = ctx->chunks_per_wave;
+  s.max_replicas = ctx->max_replicas;
+  return clb_write_schedule(path, &s);
+}
+
+int clb_load_schedule(clb_ctx* ctx, const char* path) {
+  if (int rc = check_ctx(ctx)) return rc;
+  clb_schedule s;
+  if (int rc = clb_read_schedule(path, &s)) return rc;
+  if (strcmp(s.device, ctx->device_name) != 0)
+    return fail(CLB_EINVAL, "%s holds settings for '%s', not for this context's '%s'", path,
+                s.device, ctx->device_name);
+  // clb_read_schedule checked every value against these setters' ranges: none of them fails
+  int rc = clb_set_tiling(ctx, s.terms_per_lane, s.min_tiles_per_wave);
+  if (!rc) rc = clb_set_launch(ctx, s.blocks_per_cu);
+  if (!rc)
+    rc = clb_set_schedule(ctx, s.waves_per_group, s.min_chunk, s.chunks_per_wave, s.max_replicas);
+  return rc;
+}
+
 int clb_set_replay(clb_ctx* ctx, uint64_t global_work_size) {
   if (int rc = check_ctx(ctx)) return rc;
   if (global_work_size == 0) {

@@ -116,6 +116,28 @@ int clb_autotune(clb_ctx *ctx, uint64_t n, double budget_ms, int *terms_per_lane
  * may be NULL. */
 int clb_get_schedule(clb_ctx *ctx, int *terms_per_lane, int *blocks_per_cu, int *waves_per_group,
                      int *min_chunk, int *chunks_per_wave, int *max_replicas);
+/* A context's launch settings as a file, so that a later run reuses a tuning (no reference
+ * counterpart: the reference's launch shape is compile-time constants, Bellard_192.hsp:5-7).
+ * Text, one key=value per line, in any order, each key once: device (clb_get_device_name's
+ * string), terms_per_lane, min_tiles_per_wave (clb_set_tiling), blocks_per_cu (clb_set_launch),
+ * waves_per_group, min_chunk, chunks_per_wave, max_replicas (clb_set_schedule); integers in
+ * decimal.  Blank lines are skipped.
+ * clb_read_schedule / clb_write_schedule are host only: CLB_EIO if the file cannot be read or
+ * written, CLB_EFORMAT for an unknown, repeated or missing key, a value that is not an integer
+ * or a file above 4096 bytes, CLB_EINVAL for a value outside its setter's range or a device name
+ * that is empty, too long or holds a control character.
+ * clb_save_schedule writes the context's settings with its device's name; clb_load_schedule
+ * reads a file and applies it as the setters would, or returns CLB_EINVAL when the file names
+ * another device (nothing is applied on any error). */
+typedef struct {
+  char device[256];
+  int terms_per_lane, min_tiles_per_wave, blocks_per_cu;
+  int waves_per_group, min_chunk, chunks_per_wave, max_replicas;
+} clb_schedule;
+int clb_read_schedule(const char *path, clb_schedule *out);
+int clb_write_schedule(const char *path, const clb_schedule *s);
+int clb_save_schedule(clb_ctx *ctx, const char *path);
+int clb_load_schedule(clb_ctx *ctx, const char *path);
 /* Launch mode (no reference counterpart; the reference enqueues every kernel from the HSP host,
  * KernelFuncs.as:25-74): 1 = run each call's kernel sequence (head and tail terms, combine) as a
  * HIP graph, captured on the first call with a given buffer layout and kernel choice and replayed

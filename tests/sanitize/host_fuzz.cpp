@@ -1,7 +1,8 @@
 // Sanitizer driver for the host-only entry points of libclbellard.so (clb_format.cpp), built with
 // g++ -fsanitize=address,undefined by tests/test_sanitizers.py and linked with that source file
 // directly (no HIP).  It feeds the entry points that parse or format external data -- worklists,
-// work files, digit positions, 192-bit values, double-doubles -- with random, malformed and
+// work files, launch-settings files, digit positions, 192-bit values, double-doubles -- with
+// random, malformed and
 // boundary inputs; any memory error or undefined behaviour aborts the run.  Results are checked
 // where they are known (round trips, status codes).
 #include <sys/stat.h>
@@ -162,6 +163,42 @@ int main(int argc, char** argv) {
     CHECK(rc == CLB_OK || rc == CLB_EFORMAT);
   }
   CHECK(clb_work_sum(nullptr, hex, counts) == CLB_EINVAL);
+
+  // launch-settings files: a round trip, then random and mutated text
+  clb_schedule s0;
+  memset(&s0, 0, sizeof s0);
+  snprintf(s0.device, sizeof s0.device, "%s", "AMD Instinct MI355X");
+  s0.terms_per_lane = 16;
+  s0.blocks_per_cu = 8;
+  s0.waves_per_group = 4;
+  s0.min_chunk = 256;
+  s0.max_replicas = 64;
+  const std::string sp = dir + "/sched.txt";
+  CHECK(clb_write_schedule(sp.c_str(), &s0) == CLB_OK);
+  clb_schedule s1;
+  CHECK(clb_read_schedule(sp.c_str(), &s1) == CLB_OK && memcmp(&s0, &s1, sizeof s0) == 0);
+  const std::string good = "device=g\nterms_per_lane=4\nmin_tiles_per_wave=0\nblocks_per_cu=0\n"
+                           "waves_per_group=2\nmin_chunk=64\nchunks_per_wave=-1\nmax_replicas=8\n";
+  for (int i = 0; i < iters; ++i) {
+    std::string body;
+    switch (i % 4) {
+      case 0: body = random_bytes(rng, rng() % 300, "=\r\n-0123456789abcdeimnprstuvwx_"); break;
+      case 1: body = random_bytes(rng, rng() % 300, nullptr); break;
+      case 2: {
+        body = good;
+        for (int k = 0; k < 3; ++k) body[rng() % body.size()] = (char)(rng() & 255);
+        break;
+      }
+      default: body = good + random_bytes(rng, rng() % 5000, "=\n9"); break;
+    }
+    write_file(sp, body);
+    clb_schedule s;
+    const int rc = clb_read_schedule(sp.c_str(), &s);
+    CHECK(rc == CLB_OK || rc == CLB_EFORMAT || rc == CLB_EINVAL);
+    if (rc == CLB_OK) CHECK(strlen(s.device) > 0 && strlen(s.device) < sizeof s.device);
+  }
+  CHECK(clb_read_schedule(nullptr, &s1) == CLB_EINVAL);
+  CHECK(clb_write_schedule(sp.c_str(), nullptr) == CLB_EINVAL);
   printf("host_fuzz: %d iterations, %d failed checks, last error '%s'\n", iters, failures,
          clb_last_error());
   return failures ? 1 : 0;

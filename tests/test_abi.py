@@ -280,3 +280,78 @@ def test_write_work_file_round_trip(tmp_path, oracle):
     assert line0 == oracle.hex_dd(hi, lo) and line0.endswith("0" * 20)
     with pytest.raises(clb.ClbError):
         clb.write_work_file(d, 7, 0, i2w(1))
+
+
+SCHED = {"device": "AMD Instinct MI355X", "terms_per_lane": 4, "min_tiles_per_wave": 0,
+         "blocks_per_cu": 6, "waves_per_group": 2, "min_chunk": 1024, "chunks_per_wave": 0,
+         "max_replicas": 4}
+
+
+def test_schedule_file_round_trip(tmp_path):
+    p = str(tmp_path / "tune.txt")
+    clb.write_schedule(p, SCHED)
+    text = open(p).read()
+    assert text.splitlines()[0] == "device=AMD Instinct MI355X" and "min_chunk=1024" in text
+    assert clb.read_schedule(p) == SCHED
+    # any key order, CR LF line ends and blank lines are read the same
+    lines = text.splitlines()
+    (tmp_path / "b.txt").write_bytes(("\r\n".join(lines[::-1] + [""]) + "\r\n").encode())
+    assert clb.read_schedule(str(tmp_path / "b.txt")) == SCHED
+
+
+def test_schedule_file_rejects_malformed(tmp_path):
+    good = [f"{k}={v}" for k, v in SCHED.items()]
+    cases = {
+        "unknown key": (good + ["tile=4"], clb.EFORMAT),
+        "repeated key": (good + ["min_chunk=64"], clb.EFORMAT),
+        "repeated device": (good + ["device=x"], clb.EFORMAT),
+        "missing key": ([x for x in good if not x.startswith("max_replicas")], clb.EFORMAT),
+        "missing device": (good[1:], clb.EFORMAT),
+        "no equals sign": (good + ["max_replicas"], clb.EFORMAT),
+        "not an integer": ([x if not x.startswith("min_chunk") else "min_chunk=1e3"
+                            for x in good], clb.EFORMAT),
+        "blank value": ([x if not x.startswith("blocks_per_cu") else "blocks_per_cu="
+                         for x in good], clb.EFORMAT),
+        "huge value": ([x if not x.startswith("min_chunk") else "min_chunk=99999999999"
+                        for x in good], clb.EFORMAT),
+        "out of range": ([x if not x.startswith("blocks_per_cu") else "blocks_per_cu=65"
+                          for x in good], clb.EINVAL),
+        "bad tile length": ([x if not x.startswith("terms_per_lane") else "terms_per_lane=8"
+                             for x in good], clb.EINVAL),
+        "bad replicas": ([x if not x.startswith("max_replicas") else "max_replicas=6"
+                          for x in good], clb.EINVAL),
+        "empty device": (["device="] + good[1:], clb.EINVAL),
+        "long device": (["device=" + "x" * 300] + good[1:], clb.EINVAL),
+    }
+    for name, (lines, code) in cases.items():
+        p = tmp_path / "t.txt"
+        p.write_text("\n".join(lines) + "\n")
+        with pytest.raises(clb.ClbError) as e:
+            clb.read_schedule(str(p))
+        assert e.value.code == code, (name, e.value)
+    (tmp_path / "big.txt").write_text("\n" * 5000)
+    with pytest.raises(clb.ClbError) as e:
+        clb.read_schedule(str(tmp_path / "big.txt"))
+    assert e.value.code == clb.EFORMAT
+    with pytest.raises(clb.ClbError) as e:
+        clb.read_schedule(str(tmp_path / "missing.txt"))
+    assert e.value.code == clb.EIO
+    bad = dict(SCHED, device="a\tb")
+    with pytest.raises(clb.ClbError) as e:
+        clb.write_schedule(str(tmp_path / "w.txt"), bad)
+    assert e.value.code == clb.EINVAL and not os.path.exists(tmp_path / "w.txt")
+
+
+def test_cli_rejects_malformed_tune_cache(tmp_path):
+    """A --tune-cache file that is there but malformed is reported before any GPU call."""
+    import subprocess
+    cli = os.path.join(ROOT, "clbellard_amd", "clbellard")
+    if not os.path.exists(cli):
+        pytest.skip("CLI not built")
+    bad = tmp_path / "tune.txt"
+    bad.write_text("device=x\nterms_per_lane=3\n")
+    for args in (["normal", "1000000", "--tune-cache", str(bad)],
+                 ["batch", "-", "--tune-cache", str(bad)],
+                 ["normal", "1000000", "--tune-cache"], ["normal", "1000000", "--tune-cache", ""]):
+        out = subprocess.run([cli, *args], input="1000000\n", capture_output=True, text=True)
+        assert out.returncode == 2 and "usage" in out.stderr, (args, out.stderr)

@@ -9,7 +9,8 @@ import sys
 import pytest
 
 import clbellard_amd as clb
-from clb_testlib import (DEN0, DEN1, M192, NUMESIGN, SHIFT, bound_ulps, i2w, py_normalize, w2i)
+from clb_testlib import (DEN0, DEN1, M192, NUMESIGN, ROOT, SHIFT, bound_ulps, i2w, py_normalize,
+                         w2i)
 
 pytestmark = pytest.mark.gpu
 R = random.Random(424242)
@@ -345,6 +346,47 @@ def test_autotune_keeps_bits(oracle):
         assert w2i(c.normal(250000)[0]) == oracle.normal(250000)
         with clb.Context(0, clb.PREC_U192) as ref:
             assert c.normal(10**8) == ref.normal(10**8)
+
+
+def test_schedule_file_carries_a_tuning(tmp_path):
+    """clb_save_schedule writes the context's launch settings under its device's name;
+    clb_load_schedule on a new context restores them (same bits), and refuses a file written for
+    another device without changing anything."""
+    p = str(tmp_path / "tune.txt")
+    with clb.Context(0, clb.PREC_U192) as a:
+        a.set_tiling(4, 3)
+        a.set_launch(6)
+        a.set_schedule(2, 1024, 0, 4)
+        a.save_schedule(p)
+        want_sched, want = a.get_schedule(), a.normal(10**7)
+    saved = clb.read_schedule(p)
+    assert saved["device"] == clb.device_name(0)
+    assert saved["min_tiles_per_wave"] == 3 and saved["terms_per_lane"] == 4
+    with clb.Context(0, clb.PREC_U192) as b:
+        default = b.get_schedule()
+        b.load_schedule(p)
+        assert b.get_schedule() == want_sched != default
+        assert b.normal(10**7) == want
+        other = str(tmp_path / "other.txt")
+        clb.write_schedule(other, dict(saved, device="another GPU", blocks_per_cu=2))
+        with pytest.raises(clb.ClbError) as e:
+            b.load_schedule(other)
+        assert e.value.code == clb.EINVAL and b.get_schedule() == want_sched
+
+
+def test_cli_tune_cache(tmp_path):
+    """`clbellard normal N --autotune MS --tune-cache FILE` tunes and writes FILE; the next run
+    loads it instead of tuning, with the same digits."""
+    import subprocess
+    cli = os.path.join(ROOT, "clbellard_amd", "clbellard")
+    p = str(tmp_path / "tune.txt")
+    args = [cli, "normal", "100000000", "--autotune", "200", "--tune-cache", p]
+    first = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert first.returncode == 0 and "written to" in first.stderr, first.stderr
+    assert clb.read_schedule(p)["device"] == clb.device_name(0)
+    second = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert second.returncode == 0 and "launch settings from" in second.stderr, second.stderr
+    assert first.stdout.splitlines()[:2] == second.stdout.splitlines()[:2]
 
 
 def test_out_of_range_moduli_are_refused(ctx):
