@@ -32,15 +32,21 @@ __device__ __forceinline__ void flush_job(U192 v, uint64_t nterms, uint32_t nume
 // Lane L's exponent is E - 10 L with E the exponent of lane 0 (uniform): for E >= 1664 it is
 // evaluated as 2^(1024 H + v) with H = (E - 640) >> 10 uniform and v = ((E - 640) & 1023) + 640
 // - 10 L in [10, 1663] per lane (fpow2_split); smaller exponents take per-lane windows.
-// kFast: every term of the tile lies in the job and has E >= 1664, and m >= 2^37 from the second
-// term on (nearly all tiles), so the term loop carries no range check, no exponent-form test, no
-// per-term count and no choice of reciprocal update.
-template <int W, int TPL, bool kFast>
+// 1/m of the next term (m + 64 den0) comes from this term's reciprocal by Newton steps
+// r <- r + r (1 - m r), each squaring the relative error: the step 64 den0 <= 640 leaves it below
+// 640 / m, so one step lands within an ulp of 1/m once m >= 2^37 and two steps once m >= 2^23;
+// smaller moduli take the hardware estimate and two steps (recip).  nrecip (0, 1 or 2) is the
+// tile's uniform choice for terms 1..TPL-1, from the modulus of its second term.
+// kFast: every term of the tile lies in the job and has E >= 1664, and m >= 2^23 from the second
+// term on (all tiles but the first ~10^6 k of a job and the last ~170 k before d), so the term
+// loop carries no range check, no exponent-form test, no per-term count and no choice of
+// reciprocal update: kRecip is nrecip, fixed.
+template <int W, int TPL, bool kFast, int kRecip>
 __device__ __forceinline__ void split_terms(const JobDev& jb, uint64_t kbase, int lane, int nt,
-                                            bool newton1, uint64_t E0, int tmax, double (&f)[4],
+                                            int nrecip, uint64_t E0, int tmax, double (&f)[4],
                                             uint32_t& corr, uint64_t& nterms) {
   if (kFast) {
-    newton1 = true;
+    nrecip = kRecip;
     nt = TPL;  // a constant trip count keeps the loop test scalar
   }
   const uint32_t ten_lane = 10u * (uint32_t)lane;
@@ -69,13 +75,15 @@ __device__ __forceinline__ void split_terms(const JobDev& jb, uint64_t kbase, in
       if (!kFast) nterms += (m > 1.0);     // m = 1 gives r = 0 and a zero term of either sign
     }
     m += mstep;
-    // 1/m of the next term: one Newton step from this one's once m >= 2^37 (the step
-    // 64 den0 < 2^-27 m), else the hardware estimate and two steps.
-    if (newton1) {
-      const double e = __builtin_fma(-m, minv, 1.0);
-      minv = __builtin_fma(minv, e, minv);
-    } else {
+    if (nrecip == 0) {
       minv = recip(m);
+    } else {
+      double e = __builtin_fma(-m, minv, 1.0);
+      minv = __builtin_fma(minv, e, minv);
+      if (nrecip == 2) {
+        e = __builtin_fma(-m, minv, 1.0);
+        minv = __builtin_fma(minv, e, minv);
+      }
     }
   }
 }
@@ -88,19 +96,23 @@ __device__ __forceinline__ void tile_split(const JobDev& jb, uint64_t kbase, int
   const uint64_t left = (jb.k1 - kbase + kWave - 1) / kWave;  // terms of lane 0 before k1
   const int nt = left < (uint64_t)TPL ? (int)left : TPL;
   const bool full = kbase >= jb.k0 && kbase + tile_k(TPL) <= jb.k1;  // every lane and term valid
-  const bool newton1 = jb.den0 * (kbase + kWave) + jb.den1 >= (1ull << 37);  // m of terms i >= 1
+  const uint64_t m1 = jb.den0 * (kbase + kWave) + jb.den1;  // smallest m of terms i >= 1
+  const int nrecip = m1 >= (1ull << 37) ? 1 : m1 >= (1ull << 23) ? 2 : 0;
   const uint64_t E0 = 10ull * (jb.d - kbase);                    // exponent of lane 0, term 0
   // every term of the tile in 1024 H + v form
   const bool big = E0 - 10ull * kWave * (uint64_t)(nt - 1) >= 1664;
   const int tmax = 49 + 63 - __builtin_clzll(jb.den0 * kbase + jb.den1);  // smallest m of the tile
   double f[4] = {0.0, 0.0, 0.0, 0.0};
   uint32_t corr = 0;
-  if (full && big && newton1) {
-    split_terms<W, TPL, true>(jb, kbase, lane, nt, newton1, E0, tmax, f, corr, nterms);
+  if (full && big && nrecip != 0) {
+    if (nrecip == 1)
+      split_terms<W, TPL, true, 1>(jb, kbase, lane, nt, nrecip, E0, tmax, f, corr, nterms);
+    else
+      split_terms<W, TPL, true, 2>(jb, kbase, lane, nt, nrecip, E0, tmax, f, corr, nterms);
     nterms += TPL;  // every term counted; k = 0 with den1 = 1 is the only m = 1
     if (kbase == 0 && lane == 0 && jb.den1 == 1) nterms -= 1;
   } else {
-    split_terms<W, TPL, false>(jb, kbase, lane, nt, newton1, E0, tmax, f, corr, nterms);
+    split_terms<W, TPL, false, 0>(jb, kbase, lane, nt, nrecip, E0, tmax, f, corr, nterms);
   }
   // Digit i has weight 2^(144 - 48 i).  f[0] may be negative (the residue is balanced, |x| up to
   // 1.34 m, so |f[0]| < 16 1.34 2^48 < 2^53): only its low 48 bits matter mod 2^192, and its two's

@@ -220,6 +220,57 @@ def test_tile_digit_sums():
         assert tile_sum(xs, ms, rng.choice((-1, 0, 1))) == want, trial
 
 
+def ulps_from_reciprocal(r: float, m: float) -> int:
+    """Signed distance of r from the correctly rounded 1/m, in ulps (capped at 9)."""
+    t = 1.0 / m
+    n, x = 0, t
+    while x != r and n < 9:
+        x = math.nextafter(x, math.inf if r > t else -math.inf)
+        n += 1
+    return n if r > t else -n
+
+
+def newton_update(m: float, r: float, steps: int) -> float:
+    for _ in range(steps):
+        e = fma(-m, r, 1.0)
+        r = fma(r, e, r)
+    return r
+
+
+@pytest.mark.parametrize("steps,lo,hi", [(1, 37, 48), (2, 23, 37)])
+def test_reciprocal_carried_along_the_stride(steps, lo, hi):
+    """clb_kernels.hip split_terms: a lane's terms step m by 64 den0 (den0 4 or 10), and 1/m of the
+    next term is taken from this term's reciprocal by one Newton step once m >= 2^37 and by two
+    once m >= 2^23.  Over a tile's 16 terms, from a first reciprocal within an ulp of 1/m (the
+    hardware estimate and two steps), every reciprocal stays within an ulp of the correctly
+    rounded 1/m -- the accuracy the squares' and digits' bounds assume (recip(m, -1..1) above)."""
+    rng = random.Random(steps)
+    starts = [(1 << lo) + 1, (1 << lo) + 641, (1 << hi) - 64 * 10 * 16 - 1]
+    starts += [rng.randrange(1 << lo, 1 << hi) | 1 for _ in range(600)]
+    for m0 in starts:
+        for den0 in (4, 10):
+            r = recip(float(m0), rng.choice((-1, 0, 1)))
+            m = m0
+            for _ in range(15):
+                m += 64 * den0
+                r = newton_update(float(m), r, steps)
+                assert abs(ulps_from_reciprocal(r, float(m))) <= 1, (m0, den0, m)
+
+
+def test_one_newton_step_is_not_enough_below_2_37():
+    """The bound that sets the switch: from 2^23 to 2^30 a single step leaves the reciprocal of the
+    next modulus several ulps away (relative error (640 / m)^2 > 2^-53), so the tile choice of
+    two steps there is needed, not a margin."""
+    rng = random.Random(3)
+    far = 0
+    for _ in range(300):
+        m0 = rng.randrange(1 << 23, 1 << 30) | 1
+        r = 1.0 / m0
+        m = m0 + 640
+        far += abs(ulps_from_reciprocal(newton_update(float(m), r, 1), float(m))) > 1
+    assert far > 250
+
+
 def test_quotient_digits_reconstruct_the_term():
     rng = random.Random(11)
     for _ in range(3000):
