@@ -32,10 +32,11 @@
  *                                       term192; 256 bits of pi from mpmath at 22 positions
  *   orc_hex48                           printf("%016llx") layout
  *   orc_work_sum_add                    big-integer sum mod 16^48
- *   orc_hex_dd                          exact 112-bit values, hand-traced borrow vectors, and
- *                                       floor(v 2^112) for random non-negative inputs; negative
- *                                       inputs pinned only to within 2^-107 (one dd-add rounding),
- *                                       parity unpinned below that
+ *   orc_hex_dd                          exact 112-bit values, hand-traced borrow vectors,
+ *                                       floor(v 2^112) for random non-negative inputs, and for
+ *                                       negative ones floor((x + s) 2^112), (x, y) = TwoSum(a0, 1),
+ *                                       s = RN(y + a1): the add of 1's one rounding, computed with
+ *                                       Python's IEEE sums and exact rationals
  *   orc_ref_item192, orc_ref_tail192,
  *   orc_ref_batch192, orc_ref_normal192,
  *   orc_ref_job106, orc_ref_normal106   the reference kernels' own results (GPU runs of
