@@ -451,7 +451,10 @@ def test_issue_rate_microbenchmark():
         note(f"{r['op']}: {r['per_s']:.3e}/s at {r['sclk_ghz']:.3f} GHz")
         assert r["per_s"] > 0 and 0.5 < r["sclk_ghz"] < 3.5 and r["cus"] > 0
         if op == 0:
-            assert r["per_s"] / (r["cus"] * r["sclk_ghz"] * 1e9) <= 64 * 1.05
+            # runs read 65.2-65.5 lanes per CU per clock: the clock of the eight sampled waves
+            # differs from the chip's average by a few percent; a miscount (two lane-operations
+            # per instruction, a wrong CU count) would read 128 or more
+            assert r["per_s"] / (r["cus"] * r["sclk_ghz"] * 1e9) <= 64 * 1.2
 
 
 def test_schedule_knobs_give_identical_bits(oracle):
