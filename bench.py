@@ -593,22 +593,25 @@ def main():
         ctx.normal(n_pos)
         log(f"warmup {i}: {time.perf_counter() - t0:.2f}s")
 
-    head_ms, step_s, result = [], [], None
+    head_ms, step_s, sclk, result = [], [], [], None
     for i in range(args.steps):
         t0 = time.perf_counter()
         frac, hexs, _ = ctx.normal(n_pos)
         dt = time.perf_counter() - t0
         h, tot = ctx.last_timing()
+        ghz = ctx.last_clock()[0]  # after the timed call: a 256-byte read-back
         head_ms.append(h)
         step_s.append(dt)
+        sclk.append(ghz)
         if result is not None and result != frac:
             raise SystemExit("nondeterministic result across steps")
         result = frac
-        log(f"step {i}: {dt:.3f}s (head kernel {h / 1e3:.3f}s) digits {hexs}")
+        log(f"step {i}: {dt:.3f}s (head kernel {h / 1e3:.3f}s, {ghz:.3f} GHz) digits {hexs}")
     total_s = sum(step_s)
     ms_per_step = 1e3 * total_s / len(step_s)
     value = d * len(step_s) / total_s
     head_avg_s = sum(head_ms) / len(head_ms) / 1e3
+    sclk_ghz = sum(sclk) / len(sclk)
     slots = algorithmic_fp64_slots(d)
     achieved_slots = slots / head_avg_s  # FP64 VALU lane-slots per second
     flops = algorithmic_fp64_flops(d)
@@ -702,6 +705,16 @@ def main():
                     "lanes x 2.4 GHz); no MFMA is used",
             "achieved": achieved_slots / 1e12, "peak": PEAK_VALU_LANE_SLOTS_PER_S / 1e12,
             "unit": "Tslot/s", "frac": achieved_slots / PEAK_VALU_LANE_SLOTS_PER_S,
+            "sclk_ghz_measured": sclk_ghz,
+            "frac_at_measured_clock": achieved_slots / (256 * 64 * sclk_ghz * 1e9),
+            "clock_what": "the head kernel's shader clock during the timed steps (the first wave "
+                          "on each XCD reads the shader-clock and 100 MHz real-time counters at "
+                          "its start and end; mean over XCDs and steps), and the issue fraction "
+                          "at that clock; the peak and frac above use the nominal 2.4 GHz.  It "
+                          "can exceed 1: the algorithmic count takes bitlen(e) - 1 squares from "
+                          "x = 2, and the kernel starts from 2^t mod m (t up to 49 + log2 m) in "
+                          "one step, about six squares fewer, while it also issues rounding, "
+                          "ldexp and integer instructions that the count leaves out",
             "traffic": load_traffic(),
             "head_kernel_ms": 1e3 * head_avg_s, "algorithmic_fp64_slots": slots,
             "fp64_flops": {"achieved": flops / head_avg_s / 1e12, "peak": PEAK_FP64_TFLOPS,

@@ -38,6 +38,7 @@ OK, EINVAL, EDEVICE, ERANGE, EIO, ENOMEM, EFORMAT = 0, -1, -2, -3, -4, -5, -6
 PREC_DD106, PREC_U192, PREC_U192_BETA = 0, 1, 2
 KERNEL_AUTO, KERNEL_PLAIN, KERNEL_SPLIT = 0, 1, 2
 REF_GLOBAL_WORK_SIZE = 65536 * 256  # the reference's work-items per launch, Bellard_192.hsp:5-7
+CLOCK_XCDS = 8  # CLB_CLOCK_XCDS: head-kernel clock samples, one per XCD
 NOALLOC_MAX_JOBS = 4096  # CLB_NOALLOC_MAX_JOBS
 
 # KernelFuncs.as:9-12
@@ -137,6 +138,8 @@ _SIGNATURES = {
                         ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "clb_last_timing": ([_CTX, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
                         ctypes.c_int),
+    "clb_last_clock": ([_CTX, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+                       ctypes.c_int),
 }
 
 
@@ -467,3 +470,11 @@ class Context:
         h, t = ctypes.c_double(), ctypes.c_double()
         _check(lib.clb_last_timing(self._ctx, ctypes.byref(h), ctypes.byref(t)))
         return h.value, t.value
+
+    def last_clock(self) -> Tuple[float, List[float]]:
+        """(mean GHz, GHz per XCD) of the last call's head-term kernel, from the shader-clock and
+        real-time counters its first wave on each XCD read at the start and at the end."""
+        per = (ctypes.c_double * CLOCK_XCDS)()
+        mean = ctypes.c_double()
+        _check(lib.clb_last_clock(self._ctx, per, ctypes.byref(mean)))
+        return mean.value, list(per)

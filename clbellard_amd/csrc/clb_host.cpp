@@ -112,6 +112,10 @@ struct clb_ctx {
   DevBuf<uint8_t> block;        // the call's block: header, tables, accumulator, results
   HostBuf block_host;           // its pinned staging copy (results land here too)
   float head_ms = 0.f, total_ms = 0.f;
+  // k_head's clock samples (kClockXcds x 4 words), and whether the last call's head kernel wrote
+  // them
+  uint64_t* clk = nullptr;
+  bool clk_valid = false;
   // resident k_head workgroups per CU, per kind, tile length and workgroup size (1, 2, 4, 8 waves)
   int resident[kNumKinds][kNumTpl][4] = {};
   // Tile schedule knobs (clb_set_schedule): k_head's workgroup size in waves, the queue chunk
@@ -534,6 +538,7 @@ int run_replay106(clb_ctx* ctx, const std::vector<HostJob>& hjobs,
       for (size_t i = 0; i < nslots; ++i) dd_words(sv + 2 * i, slot_host + 3 * i);
   }
   ctx->head_ms = head;
+  ctx->clk_valid = false;
   ctx->total_ms = (float)std::chrono::duration<double, std::milli>(
                       std::chrono::steady_clock::now() - t_call)
                       .count();
@@ -637,6 +642,7 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
   h.mirror_total = (uint64_t)(uintptr_t)mirrors.total;
   h.mirror_slot = (uint64_t)(uintptr_t)mirrors.slot;
   h.mirror_index = mirrors.index;
+  h.clk_out = (uint64_t)(uintptr_t)ctx->clk;
   h.accumulate = accumulate_dev != nullptr;
   h.njobs = (uint32_t)jobs.size();
   h.ntails = (uint32_t)tails.size();
@@ -715,6 +721,7 @@ int run(clb_ctx* ctx, const std::vector<HostJob>& hjobs, const std::vector<uint8
   if (slot_host && nslots)
     memcpy(slot_host, res + 3 * groups.size(), 3 * nslots * sizeof(uint64_t));
   ctx->head_ms = head;
+  ctx->clk_valid = !replay || !jobs.empty();  // k_head ran
   ctx->total_ms = (float)std::chrono::duration<double, std::milli>(
                       std::chrono::steady_clock::now() - t_call)
                       .count();
@@ -840,7 +847,8 @@ int clb_ctx_create(int device, int precision, clb_ctx** out) {
     if ((e = hipEventCreate(&ev)) != hipSuccess)
       return cleanup(fail(CLB_EDEVICE, "hipEventCreate: %s", hipGetErrorString(e)));
   if ((e = hipMalloc((void**)&ctx->partial, 3 * sizeof(uint64_t))) != hipSuccess ||
-      (e = hipMalloc((void**)&ctx->ans, 3 * sizeof(uint64_t))) != hipSuccess)
+      (e = hipMalloc((void**)&ctx->ans, 3 * sizeof(uint64_t))) != hipSuccess ||
+      (e = hipMalloc((void**)&ctx->clk, 4 * kClockXcds * sizeof(uint64_t))) != hipSuccess)
     return cleanup(fail(CLB_ENOMEM, "hipMalloc: %s", hipGetErrorString(e)));
   // The doubling squares (term kernels and replay kernels, two code objects) need the FP64
   // output modifier, which the hardware applies only in device code built with the IEEE mode bit
@@ -884,6 +892,7 @@ int clb_ctx_destroy(clb_ctx* ctx) {
   ctx->graphs.clear();
   if (ctx->partial) (void)hipFree(ctx->partial);
   if (ctx->ans) (void)hipFree(ctx->ans);
+  if (ctx->clk) (void)hipFree(ctx->clk);
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1451,6 +1460,7 @@ int clb_run_jobs_noalloc(clb_ctx* ctx, const clb_job* jobs_in, size_t count, uin
   if (e2 != hipSuccess) return fail(CLB_EDEVICE, "term launch failed: %s", hipGetErrorString(e2));
   memcpy(partial, stage + off_out, 3 * count * sizeof(uint64_t));
   ctx->head_ms = head;
+  ctx->clk_valid = false;  // its header carries no clock address
   ctx->total_ms = (float)std::chrono::duration<double, std::milli>(
                       std::chrono::steady_clock::now() - t_call)
                       .count();
@@ -1461,6 +1471,32 @@ int clb_last_timing(clb_ctx* ctx, double* head_kernel_ms, double* total_ms) {
   if (int rc = check_ctx(ctx)) return rc;
   if (head_kernel_ms) *head_kernel_ms = ctx->head_ms;
   if (total_ms) *total_ms = ctx->total_ms;
+  return CLB_OK;
+}
+
+static_assert(CLB_CLOCK_XCDS == kClockXcds, "clock samples: one per XCD");
+
+int clb_last_clock(clb_ctx* ctx, double ghz[CLB_CLOCK_XCDS], double* mean_ghz) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!ctx->clk_valid)
+    return fail(CLB_EINVAL, "the last call ran no head-term kernel with clock samples");
+  if (int rc = set_device(ctx)) return rc;
+  uint64_t w[4 * kClockXcds];
+  HIP_TRY(hipMemcpyAsync(w, ctx->clk, sizeof w, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  double sum = 0.0;
+  int n = 0;
+  for (int x = 0; x < kClockXcds; ++x) {
+    const uint64_t cycles = w[4 * x + 2] - w[4 * x], ticks = w[4 * x + 3] - w[4 * x + 1];
+    // the real-time counter runs at 100 MHz: ticks / 100 microseconds
+    const double g = ticks ? (double)cycles / ((double)ticks * 10.0) : 0.0;
+    if (ghz) ghz[x] = g;
+    if (ticks) {
+      sum += g;
+      ++n;
+    }
+  }
+  if (mean_ghz) *mean_ghz = n ? sum / n : 0.0;
   return CLB_OK;
 }
 

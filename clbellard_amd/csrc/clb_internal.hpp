@@ -96,6 +96,8 @@ struct LaunchDev {
   uint64_t slot_out;           // 0, or the device address of every slot's shifted value
   uint64_t mirror_total;       // 0, or a device address that also receives group 0's result
   uint64_t mirror_slot;        // 0, or a device address that also receives slot mirror_index
+  uint64_t clk_out;            // 0, or the device address of kClockXcds x 4 words: k_head's clock
+                               // samples (below)
   uint32_t mirror_index;
   uint32_t accumulate;         // 1: dst += result (the AddAns-style accumulation)
   uint32_t njobs;              // entries of the job table
@@ -139,6 +141,12 @@ __host__ __device__ inline const TailDev* hdr_rtails(const LaunchDev* h) {
 __host__ __device__ inline unsigned long long* hdr_acc(LaunchDev* h) {
   return reinterpret_cast<unsigned long long*>(reinterpret_cast<uint8_t*>(h) + h->off_acc);
 }
+
+// k_head's clock samples: the first wave of workgroup x < kClockXcds (workgroups are dispatched
+// round-robin over the 8 XCDs, so one per XCD) writes, at clk_out[4 x ..], the shader-clock
+// counter (s_memtime) and the 100 MHz real-time counter (s_memrealtime) when it starts and when
+// it leaves the tile loop: the clock that XCD ran the launch at.
+constexpr int kClockXcds = 8;
 
 constexpr int kCombineBlocks = 256;  // k_combine grid (one block per group, grid-stride)
 constexpr int kCombineWaves = 8;     // k_combine block: waves sharing a group's slots
@@ -187,7 +195,7 @@ constexpr uint64_t kRefGlobalWorkSize = 65536ull * 256ull;  // Bellard_192.hsp:5
 
 // The no-allocation entry point (clb_noalloc.hip): a fixed device array per GPU and a pinned
 // staging buffer, each kNoAllocBytes, set up once per device (with that device current).  A call
-// of at most kNoAllocMaxJobs jobs always fits: its block is at most 512 B of header, 64 B per job,
+// of at most kNoAllocMaxJobs jobs always fits: its block is under 1 KiB of header, 64 B per job,
 // 32 B per tail, 8 + 1 + 24 B per result and 64 B per accumulator copy, and the copies number at
 // most max(jobs, waves / 32).
 constexpr size_t kNoAllocBytes = 1u << 20;

@@ -259,6 +259,16 @@ __device__ __forceinline__ void tail_job(const TailDev tj, int lane, uint32_t re
   if (lane == 0) atomic_add192(acc + ((size_t)tj.slot * replicas + rep) * kAccWords, q);
 }
 
+// Clock samples of the launch (LaunchDev::clk_out): words at + 0, 1 at the start, at + 2, 3 when
+// the wave leaves the tile loop.  Counter reads only, outside the term loop.
+__device__ __forceinline__ void clock_sample(const LaunchDev* hdr, int at) {
+  if (block_x() < (uint32_t)kClockXcds && thread_x() == 0 && hdr->clk_out) {
+    uint64_t* clk = reinterpret_cast<uint64_t*>(hdr->clk_out) + 4 * block_x();
+    clk[at] = __builtin_amdgcn_s_memtime();
+    clk[at + 1] = __builtin_amdgcn_s_memrealtime();
+  }
+}
+
 // Persistent head-term kernel; it also takes the tail jobs (wave w the tails w, w + nwaves, ...,
 // before its tiles).  Flat tile f belongs to the last job with flat_begin <= f.  Wave w first
 // takes the static range [w T, (w + 1) T) of T = static_tiles consecutive tiles (no atomics;
@@ -292,6 +302,7 @@ k_head(LaunchDev* __restrict__ hdr) {
   // of many short flushes over more addresses (atomics on one address serialise).
   const uint32_t replicas = hdr->replicas;
   const uint32_t rep = wave_id & (replicas - 1);
+  clock_sample(hdr, 0);
   {
     const TailDev* __restrict__ tails = hdr_tails(hdr);
     for (uint32_t t = wave_id; t < hdr->ntails; t += nwaves)
@@ -343,6 +354,7 @@ k_head(LaunchDev* __restrict__ hdr) {
     tile(f);
     ++f;
   }
+  clock_sample(hdr, 2);
   if (cur >= 0)
     flush_job(sum, nterms, jb.numesign, lane, acc + ((size_t)jb.slot * replicas + rep) * kAccWords);
 }

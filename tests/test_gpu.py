@@ -457,6 +457,33 @@ def test_issue_rate_microbenchmark():
             assert r["per_s"] / (r["cus"] * r["sclk_ghz"] * 1e9) <= 64 * 1.2
 
 
+def test_head_kernel_clock_samples():
+    """clb_last_clock: after a head-kernel launch every XCD's sample gives a plausible shader
+    clock; before any call, and after calls that run no head-term kernel with samples (the
+    106-bit-mode replay, the no-allocation entry point), it refuses."""
+    _, d = py_normalize(10**10)
+    with clb.Context(0, clb.PREC_U192) as c:
+        with pytest.raises(clb.ClbError):
+            c.last_clock()
+        c.normal(10**10)  # ~0.16 s: long enough for the counters to mean something
+        mean, per = c.last_clock()
+        note(f"head kernel clock {mean:.3f} GHz, per XCD {[round(g, 3) for g in per]}")
+        assert len(per) == clb.CLOCK_XCDS and all(0.5 < g < 3.5 for g in per)
+        assert abs(mean - sum(per) / len(per)) < 1e-9
+        c.set_graph(True)  # the captured graph writes them too
+        c.normal(10**9)
+        assert 0.5 < c.last_clock()[0] < 3.5
+        c.set_graph(False)
+        c.run_jobs_noalloc([(d, 0, 1000, 0)])
+        with pytest.raises(clb.ClbError):
+            c.last_clock()
+    with clb.Context(0, clb.PREC_DD106) as c:
+        c.set_replay(clb.REF_GLOBAL_WORK_SIZE)
+        c.normal(10**6)
+        with pytest.raises(clb.ClbError):
+            c.last_clock()
+
+
 def test_schedule_knobs_give_identical_bits(oracle):
     """Every workgroup size, tile-queue schedule and slot-copy limit (clb_set_schedule) gives the
     oracle's bits, on small launches (static tiles) and large ones (the guided queue); bad values
