@@ -336,7 +336,7 @@ def bench_worklist(clb, args):
     [0, d) at N = 2.5e12) x 7 formulas in one launch, written as work files, summed like
     workmodeSum0_6, and compared with the single normal-mode run."""
     import tempfile
-    n_pos = args.position if args.position != POSITION else 2500000000000
+    n_pos = args.position or 2500000000000
     n_pos, d, _ = clb.normalize(n_pos)
     rng = splitmix64(1)
     lens = [10**8 + next(rng) % (10**11 - 10**8 + 1) for _ in range(64)]
@@ -410,10 +410,12 @@ def bench_schedule(clb, args):
     (1, 2, 4, 8 waves, each at full occupancy, i.e. 8 waves per SIMD), the tile queue (fixed
     chunks of 256 or 64 tiles, or the guided schedule with 1, 2 or 4 chunks per wave per phase),
     the slot copies (1, 4, 64 at most) and resident workgroups per CU."""
-    n = args.position if args.position != POSITION else 10**11
+    n = args.position or 10**11
     n, d, _ = clb.normalize(n)
     ctx = clb.Context(0, clb.PREC_U192)
     ref = ctx.normal(n)[0]
+
+    clocks = {}
 
     def timed(label, setup, pos=n, reps=2):
         setup()
@@ -423,28 +425,35 @@ def bench_schedule(clb, args):
             h, t = ctx.last_timing()
             if pos == n and frac != ref:
                 raise SystemExit(f"{label}: digits differ")
-            best = h if best is None else min(best, h)
+            if best is None or h < best:
+                best, ghz = h, ctx.last_clock()[0]
         ctx.set_schedule()
         ctx.set_launch(0)
         ctx.set_tiling(0)
-        log(f"{label:40s} head kernel {best:10.3f} ms")
+        clocks[label] = {"sclk_ghz": round(ghz, 3), "ms_x_ghz": round(best * ghz, 1)}
+        log(f"{label:40s} head kernel {best:10.3f} ms at {ghz:.3f} GHz ({best * ghz:.1f} ms*GHz)")
         return round(best, 3)
 
     rows = {}
-    for w in (1, 2, 4, 8):
-        rows[f"workgroup {64 * w}"] = timed(f"workgroup {64 * w}",
-                                            lambda w=w: ctx.set_schedule(waves_per_group=w))
-    for label, mc, cpw in (("queue fixed 256 (default)", 256, -1), ("queue fixed 64", 64, -1),
-                           ("queue guided 1/wave, min 64", 64, 1),
-                           ("queue guided 2/wave, min 64", 64, 2),
-                           ("queue guided 4/wave, min 64", 64, 4),
-                           ("queue guided 2/wave, min 16", 16, 2)):
+    if args.knobs == "all":
+        for w in (1, 2, 4, 8):
+            rows[f"workgroup {64 * w}"] = timed(f"workgroup {64 * w}",
+                                                lambda w=w: ctx.set_schedule(waves_per_group=w))
+    queues = (("queue fixed 256 (default)", 256, -1), ("queue fixed 64", 64, -1),
+              ("queue guided 1/wave, min 64", 64, 1), ("queue guided 2/wave, min 64", 64, 2),
+              ("queue guided 4/wave, min 64", 64, 4), ("queue guided 2/wave, min 16", 16, 2))
+    if args.knobs == "queue":  # the larger chunks, and guided queues ending in 256-tile chunks
+        queues = (("queue fixed 256 (default)", 256, -1), ("queue fixed 1024", 1024, -1),
+                  ("queue fixed 4096", 4096, -1), ("queue guided 4/wave, min 256", 256, 4),
+                  ("queue guided 8/wave, min 256", 256, 8))
+    for label, mc, cpw in queues:
         rows[label] = timed(label, lambda mc=mc, cpw=cpw: ctx.set_schedule(0, mc, cpw, 0))
-    for bpc in (4, 6):
-        rows[f"{bpc} workgroups per CU"] = timed(f"{bpc} workgroups per CU",
-                                                 lambda bpc=bpc: ctx.set_launch(bpc))
+    if args.knobs == "all":
+        for bpc in (4, 6):
+            rows[f"{bpc} workgroups per CU"] = timed(f"{bpc} workgroups per CU",
+                                                     lambda bpc=bpc: ctx.set_launch(bpc))
     small = {}
-    for pos in (100000, 1000000, 10000000):
+    for pos in ((100000, 1000000, 10000000) if args.knobs == "all" else ()):
         ref_small = ctx.normal(pos)[0]
         for r in (1, 4, 64):
             ctx.set_schedule(max_replicas=r)
@@ -474,8 +483,9 @@ def bench_schedule(clb, args):
         "unit": "ms", "n_gpus": 1, "steps": 2, "warmup": 1, "ms_per_step": base,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"launch-knob sweep: normal mode at N={n} (best of 2) and call "
-                               "latency at N=1e5, 1e6, 1e7 (us)", "head_kernel_ms": rows,
+        "config": {"workload": f"launch-knob sweep ({args.knobs}): normal mode at N={n} (best of "
+                               "2) and call latency at N=1e5, 1e6, 1e7 (us)",
+                   "head_kernel_ms": rows, "head_kernel_clock": clocks,
                    "small_call_us": small}}), flush=True)
     return 0
 
@@ -485,7 +495,7 @@ def bench_replay(clb, args):
     (clb_set_replay with the reference's 2^24 work-items per launch): the reference's own output
     bits, its double-double rounding included.  The leading 24 hex digits (16 in the 106-bit mode)
     are compared with the exact mode's."""
-    n_pos, d, _ = clb.normalize(args.position)
+    n_pos, d, _ = clb.normalize(args.position or POSITION)
     exact = None
     rows = {}
     for prec in (clb.PREC_U192_BETA, clb.PREC_U192, clb.PREC_DD106):
@@ -521,7 +531,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--position", type=int, default=POSITION)
+    ap.add_argument("--position", type=int, default=None,
+                    help="digit position (default per workload: normal and replay 10^12, "
+                         "worklist 2.5x10^12, schedule 10^11)")
     ap.add_argument("--windows", type=int, default=64)
     ap.add_argument("--window-log2", type=int, default=26)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -546,6 +558,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="batched: launch the kernels one by one instead of a captured graph")
     ap.add_argument("--calls", type=int, default=200, help="small: calls per mode")
+    ap.add_argument("--knobs", choices=["all", "queue"], default="all",
+                    help="schedule: every knob, or only the tile queue (fixed 256, 1024, 4096 "
+                         "tiles per chunk and guided queues ending in 256-tile chunks)")
     ap.add_argument("--positions", type=int, default=4096, help="batched: number of positions")
     ap.add_argument("--check", type=int, default=8,
                     help="batched: positions re-run alone and compared bit for bit")
@@ -584,7 +599,7 @@ def main():
     if args.workload == "schedule":
         return bench_schedule(clb, args)
 
-    n_pos, d, _ = clb.normalize(args.position)
+    n_pos, d, _ = clb.normalize(args.position or POSITION)
     ctx = clb.Context(0, clb.PREC_U192, kernel=clb.KERNEL_SPLIT)
     log(f"device {clb.device_name(0)}; N={n_pos} d={d}")
 
