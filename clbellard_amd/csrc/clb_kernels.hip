@@ -37,10 +37,10 @@ __device__ __forceinline__ void flush_job(U192 v, uint64_t nterms, uint32_t nume
 // 640 / m, so one step lands within an ulp of 1/m once m >= 2^37 and two steps once m >= 2^23;
 // smaller moduli take the hardware estimate and two steps (recip).  nrecip (0, 1 or 2) is the
 // tile's uniform choice for terms 1..TPL-1, from the modulus of its second term.
-// kFast: every term of the tile lies in the job and has E >= 1664, and m >= 2^23 from the second
-// term on (all tiles but the first ~10^6 k of a job and the last ~170 k before d), so the term
-// loop carries no range check, no exponent-form test, no per-term count and no choice of
-// reciprocal update: kRecip is nrecip, fixed.
+// kFast: every term of the tile lies in the job and has E >= 1664, and m >= 2^37 from the second
+// term on, or m >= 2^23 in 16-term tiles (all tiles but the first ~10^6 k of a job and the last
+// ~170 k before d), so the term loop carries no range check, no exponent-form test, no per-term
+// count and no choice of reciprocal update: kRecip is nrecip, fixed.
 template <int W, int TPL, bool kFast, int kRecip>
 __device__ __forceinline__ void split_terms(const JobDev& jb, uint64_t kbase, int lane, int nt,
                                             int nrecip, uint64_t E0, int tmax, double (&f)[4],
@@ -104,10 +104,14 @@ __device__ __forceinline__ void tile_split(const JobDev& jb, uint64_t kbase, int
   const int tmax = 49 + 63 - __builtin_clzll(jb.den0 * kbase + jb.den1);  // smallest m of the tile
   double f[4] = {0.0, 0.0, 0.0, 0.0};
   uint32_t corr = 0;
-  if (full && big && nrecip != 0) {
+  // A second fast loop costs the short-tile kernels (4 and 1 terms per lane, fully unrolled, for
+  // launches too small to fill the GPU with 16-term tiles, where the moduli are small anyway)
+  // registers and spills: there the two-step tiles take the general loop.
+  constexpr bool kTwoStepFast = TPL == kMaxTermsPerLane;
+  if (full && big && (nrecip == 1 || (kTwoStepFast && nrecip == 2))) {
     if (nrecip == 1)
       split_terms<W, TPL, true, 1>(jb, kbase, lane, nt, nrecip, E0, tmax, f, corr, nterms);
-    else
+    else if constexpr (kTwoStepFast)
       split_terms<W, TPL, true, 2>(jb, kbase, lane, nt, nrecip, E0, tmax, f, corr, nterms);
     nterms += TPL;  // every term counted; k = 0 with den1 = 1 is the only m = 1
     if (kbase == 0 && lane == 0 && jb.den1 == 1) nterms -= 1;
