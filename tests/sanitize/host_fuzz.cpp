@@ -1,4 +1,4 @@
-// Sanitizer driver for the host-only entry points of libclbellard.so (clb_format.cpp), built with
+// Sanitizer harness for the host-only entry points of libclbellard.so (clb_format.cpp), built with
 // g++ -fsanitize=address,undefined by tests/test_sanitizers.py and linked with that source file
 // directly (no HIP).  It feeds the entry points that parse or format external data -- worklists,
 // work files, launch-settings files, digit positions, 192-bit values, double-doubles -- with

@@ -1,4 +1,4 @@
-/* Sanitizer driver for the oracle (oracle/clb_oracle.c), built with gcc -fsanitize=address,undefined
+/* Sanitizer harness for the oracle (oracle/clb_oracle.c), built with gcc -fsanitize=address,undefined
  * by tests/test_sanitizers.py: random and boundary inputs for every function family, with the
  * consistency checks that need no reference (the 256-bit term's top bits are the 192-bit term,
  * range sums are partition invariant, normal() equals its 256-bit counterpart's top bits within
