@@ -305,13 +305,15 @@ int clb_issue_rate(int device, int op, double target_ms, double out[4]);
  * graph) and the whole call, in ms. */
 int clb_last_timing(clb_ctx *ctx, double *head_kernel_ms, double *total_ms);
 
-/* Shader clock of the last call's head-term kernel (no counterpart in the reference: a
- * measurement for the roofline report, whose peak is quoted at a nominal clock while boxes run at
- * 2.1-2.4 GHz).  The first wave of workgroup x < CLB_CLOCK_XCDS, one per XCD, reads the
- * shader-clock and 100 MHz real-time counters at the kernel's start and when it leaves the tile
- * loop; ghz[x] (optional) receives each one's mean clock in GHz and *mean_ghz (optional) their
- * mean.  CLB_EINVAL when the last call ran no head-term kernel (the 106-bit-mode replay,
- * clb_run_jobs_noalloc, or no call yet).  Synchronises the context's stream. */
+/* Shader clock of the head-term kernel of the last clb_normal / clb_batch_positions /
+ * clb_run_jobs / clb_batch_calc / clb_sglobal / clb_sglobal_after call (the calls
+ * clb_last_timing times; no counterpart in the reference: a measurement for the roofline report,
+ * whose peak is quoted at a nominal clock while GPUs run at 2.1-2.4 GHz).  The first wave of
+ * workgroup x < CLB_CLOCK_XCDS, one per XCD, reads the shader-clock and 100 MHz real-time
+ * counters at the kernel's start and when it leaves the tile loop; ghz[x] (optional) receives
+ * each one's mean clock in GHz and *mean_ghz (optional) their mean.  CLB_EINVAL when that call
+ * ran no head-term kernel (the 106-bit-mode replay, clb_run_jobs_noalloc, or no call yet).
+ * Synchronises the context's stream. */
 #define CLB_CLOCK_XCDS 8
 int clb_last_clock(clb_ctx *ctx, double ghz[CLB_CLOCK_XCDS], double *mean_ghz);
 
