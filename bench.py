@@ -255,10 +255,19 @@ def bench_batched(clb, args):
         pos.append(5 * round(10 ** (6 + 5 * u) / 5))
     terms = sum(clb.normalize(p)[1] for p in pos)
     ctx = clb.Context(0, clb.PREC_U192)
-    median = sorted(pos)[len(pos) // 2]
+    # tuned where the work is: the position below which half of the terms lie (the count median
+    # of a log-uniform draw holds a negligible share of them)
+    ranked = sorted(pos)
+    half, acc, median = terms / 2, 0, ranked[-1]
+    for p in ranked:
+        acc += clb.normalize(p)[1]
+        if acc >= half:
+            median = p
+            break
     tpl, bpc = ctx.autotune(median, 3000.0)
     sched = ctx.get_schedule()
-    log(f"autotune at N={median}: {tpl} terms per lane, {bpc} workgroups per CU, {sched}")
+    log(f"autotune at N={median} (term-weighted median): {tpl} terms per lane, {bpc} workgroups "
+        f"per CU, {sched}")
     ctx.set_graph(not args.no_graph)
     log(f"batched: {len(pos)} positions, {terms:.4g} terms, "
         f"{'eager launches' if args.no_graph else 'captured HIP graph'}")
@@ -285,7 +294,8 @@ def bench_batched(clb, args):
                                f"{args.batches} persistent launch(es)", "positions": len(pos),
                    "terms": terms,
                    "graph": not args.no_graph, "terms_per_lane": tpl,
-                   "workgroups_per_cu": bpc, "schedule": sched, "head_kernel_ms": head_ms,
+                   "workgroups_per_cu": bpc, "schedule": sched, "tuned_at": median,
+                   "head_kernel_ms": head_ms,
                    "checked_single": len(sample)}}), flush=True)
     ctx.close()
     return 0

@@ -979,71 +979,75 @@ int clb_autotune(clb_ctx* ctx, uint64_t n, double budget_ms, int* terms_per_lane
   const int bpc_choices[3] = {resident, (3 * resident + 3) / 4, (resident + 1) / 2};
   // Stage 2 tries, one at a time on the stage-1 choice, the workgroup size (1, 2, 8 waves), the tile
   // queue (fixed chunks of 64 or 1024 tiles, a guided queue) and the slot copies (at most 4).
-  struct Sched {
-    int wg, chunk, cpw, rep;
+  struct Cfg {
+    int tpl, bpc, wg, chunk, cpw, rep;  // bpc: resident workgroups of 4 waves per CU
   };
-  const Sched stage2[] = {{1, 256, 0, 64}, {2, 256, 0, 64}, {8, 256, 0, 64}, {4, 64, 0, 64},
-                          {4, 1024, 0, 64}, {4, 64, 4, 64}, {4, 256, 0, 4}};
+  const Cfg stage2[] = {{0, 0, 1, 256, 0, 64}, {0, 0, 2, 256, 0, 64}, {0, 0, 8, 256, 0, 64},
+                        {0, 0, 4, 64, 0, 64},  {0, 0, 4, 1024, 0, 64}, {0, 0, 4, 64, 4, 64},
+                        {0, 0, 4, 256, 0, 4}};
   const int nstage2 = (int)(sizeof stage2 / sizeof stage2[0]);
+  // Each trial times the candidate and the best so far alternately, reps times each: the GPU's
+  // clock moves over a tuning run (it settles under load), and timing candidates one after the
+  // other let whichever came later look faster.
   const int ncand = kNumTpl * 3 + nstage2, reps = 3;
-  uint64_t s = (uint64_t)(budget_ms / (ncand * reps + 1) / per_term);
+  uint64_t s = (uint64_t)(budget_ms / (2 * ncand * reps + 1) / per_term);
   if (s < probe) s = probe;
   if (s > d) s = d;
   const uint64_t k0 = (d - s) / 2;
   const std::vector<HostJob> sample = jobs_for(k0, s);
-  double best = -1.0;
-  int best_tpl = 0, best_bpc = resident;
-  // Times the context's current settings; a candidate must beat the best so far by 2% to
-  // replace it (the first is the default: 16 terms per lane, full residency, 256 work-items per
-  // workgroup, fixed chunks of 256 tiles), so timing noise does not move the choice.
-  auto trial = [&](bool* better) -> int {
-    double ms = -1.0;
+  auto apply = [&](const Cfg& c) {
+    ctx->tpl_index = c.tpl;
+    ctx->wg_waves = c.wg;
+    // the same resident waves per CU: workgroups of 4 waves per CU rescaled to this size
+    ctx->blocks_per_cu = c.bpc == resident ? 0 : (c.bpc * 4 + c.wg - 1) / c.wg;
+    ctx->min_chunk = c.chunk;
+    ctx->chunks_per_wave = c.cpw;
+    ctx->max_replicas = c.rep;
+  };
+  // the default first: 16 terms per lane, full residency, 256 work-items per workgroup, fixed
+  // chunks of 256 tiles, up to 64 slot copies
+  Cfg best = {0, resident, 4, 256, 0, 64};
+  bool have_res = false;
+  // A candidate replaces the best only when it is faster by 2% in the interleaved timing, so
+  // timing noise does not move the choice; every launch must return the same bits.
+  auto trial = [&](const Cfg& cand, bool* better) -> int {
+    double t_best = -1.0, t_cand = -1.0;
     for (int r = 0; r < reps; ++r) {
-      if (int rc = run(ctx, sample, shift, groups, res, nullptr, nullptr)) return rc;
-      if (ms < 0.0 || ctx->head_ms < ms) ms = ctx->head_ms;
+      for (int which = 0; which < 2; ++which) {
+        apply(which ? cand : best);
+        if (int rc = run(ctx, sample, shift, groups, res, nullptr, nullptr)) return rc;
+        double& t = which ? t_cand : t_best;
+        if (t < 0.0 || ctx->head_ms < t) t = ctx->head_ms;
+        if (have_res && memcmp(res, best_res, sizeof res) != 0)
+          return fail(CLB_EDEVICE, "autotune: results differ between launch shapes");
+        memcpy(best_res, res, sizeof res);
+        have_res = true;
+      }
     }
-    if (best >= 0.0 && memcmp(res, best_res, sizeof res) != 0)
-      return fail(CLB_EDEVICE, "autotune: results differ between launch shapes");
-    memcpy(best_res, res, sizeof res);
-    *better = best < 0.0 || ms < best * 0.98;
-    if (*better) best = ms;
+    *better = t_cand < t_best * 0.98;
     return CLB_OK;
   };
   for (int t = 0; t < kNumTpl; ++t)
     for (int b = 0; b < 3; ++b) {
       if (b > 0 && bpc_choices[b] == bpc_choices[b - 1]) continue;
-      ctx->tpl_index = t;
-      ctx->blocks_per_cu = bpc_choices[b];
+      if (t == 0 && b == 0) continue;  // the default itself
+      const Cfg cand = {t, bpc_choices[b], 4, 256, 0, 64};
       bool better = false;
-      if (int rc = trial(&better)) return restore(rc);
-      if (better) {
-        best_tpl = t;
-        best_bpc = bpc_choices[b];
-      }
+      if (int rc = trial(cand, &better)) return restore(rc);
+      if (better) best = cand;
     }
-  Sched best_s = {4, 256, 0, 64};
-  ctx->tpl_index = best_tpl;
   for (int i = 0; i < nstage2; ++i) {
-    const Sched& c = stage2[i];
-    ctx->wg_waves = c.wg;
-    // the same resident waves per CU: the stage-1 workgroups per CU rescaled to this size
-    ctx->blocks_per_cu = best_bpc == resident ? 0 : (best_bpc * 4 + c.wg - 1) / c.wg;
-    ctx->min_chunk = c.chunk;
-    ctx->chunks_per_wave = c.cpw;
-    ctx->max_replicas = c.rep;
+    Cfg cand = stage2[i];
+    cand.tpl = best.tpl;
+    cand.bpc = best.bpc;
     bool better = false;
-    if (int rc = trial(&better)) return restore(rc);
-    if (better) best_s = c;
+    if (int rc = trial(cand, &better)) return restore(rc);
+    if (better) best = cand;
   }
   restore(CLB_OK);
-  ctx->tpl_index = best_tpl;
-  ctx->wg_waves = best_s.wg;
-  ctx->blocks_per_cu = best_bpc == resident ? 0 : (best_bpc * 4 + best_s.wg - 1) / best_s.wg;
-  ctx->min_chunk = best_s.chunk;
-  ctx->chunks_per_wave = best_s.cpw;
-  ctx->max_replicas = best_s.rep;
-  if (terms_per_lane) *terms_per_lane = kTplChoices[best_tpl];
-  if (blocks_per_cu_out) *blocks_per_cu_out = best_bpc;
+  apply(best);
+  if (terms_per_lane) *terms_per_lane = kTplChoices[best.tpl];
+  if (blocks_per_cu_out) *blocks_per_cu_out = best.bpc;
   return CLB_OK;
 }
 
