@@ -977,14 +977,20 @@ int clb_autotune(clb_ctx* ctx, uint64_t n, double budget_ms, int* terms_per_lane
   const double per_term = (ctx->head_ms > 0.f ? ctx->head_ms : 1e-3) / (double)probe;
   const int resident = blocks_per_cu(ctx, kSplitW3, 0);
   const int bpc_choices[3] = {resident, (3 * resident + 3) / 4, (resident + 1) / 2};
-  // Stage 2 tries, one at a time on the stage-1 choice, the workgroup size (1, 2, 8 waves), the tile
-  // queue (fixed chunks of 64 or 1024 tiles, a guided queue) and the slot copies (at most 4).
+  // Stage 2 changes one knob at a time on the best settings so far (a value that won stays while
+  // the next one is tried): the workgroup size (1, 2, 8 waves), the tile queue (fixed chunks of 64
+  // or 1024 tiles, a guided queue) and the slot copies (at most 4).
   struct Cfg {
     int tpl, bpc, wg, chunk, cpw, rep;  // bpc: resident workgroups of 4 waves per CU
   };
-  const Cfg stage2[] = {{0, 0, 1, 256, 0, 64}, {0, 0, 2, 256, 0, 64}, {0, 0, 8, 256, 0, 64},
-                        {0, 0, 4, 64, 0, 64},  {0, 0, 4, 1024, 0, 64}, {0, 0, 4, 64, 4, 64},
-                        {0, 0, 4, 256, 0, 4}};
+  enum Knob { kWorkgroup, kQueue, kCopies };
+  struct Step {
+    Knob knob;
+    int a, b;  // workgroup waves; chunk tiles and chunks per wave (guided); slot copies
+  };
+  const Step stage2[] = {{kWorkgroup, 1, 0}, {kWorkgroup, 2, 0}, {kWorkgroup, 8, 0},
+                         {kQueue, 64, 0},    {kQueue, 1024, 0},  {kQueue, 64, 4},
+                         {kCopies, 4, 0}};
   const int nstage2 = (int)(sizeof stage2 / sizeof stage2[0]);
   // Each trial times the candidate and the best so far alternately, reps times each: the GPU's
   // clock moves over a tuning run (it settles under load), and timing candidates one after the
@@ -1037,9 +1043,16 @@ int clb_autotune(clb_ctx* ctx, uint64_t n, double budget_ms, int* terms_per_lane
       if (better) best = cand;
     }
   for (int i = 0; i < nstage2; ++i) {
-    Cfg cand = stage2[i];
-    cand.tpl = best.tpl;
-    cand.bpc = best.bpc;
+    const Step& st = stage2[i];
+    Cfg cand = best;
+    if (st.knob == kWorkgroup) {
+      cand.wg = st.a;
+    } else if (st.knob == kQueue) {
+      cand.chunk = st.a;
+      cand.cpw = st.b;
+    } else {
+      cand.rep = st.a;
+    }
     bool better = false;
     if (int rc = trial(cand, &better)) return restore(rc);
     if (better) best = cand;
