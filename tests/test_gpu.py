@@ -119,6 +119,32 @@ def test_split_variants_vs_oracle(ctx_split, oracle):
             assert w2i(ctx_split.read_partial()) == oracle.batch_calc(d, k0, k0 + span, j)
 
 
+KIND_LIMITS = (1 << 42, 1 << 46, 1 << 48, 1 << 55, 1 << 63)
+
+
+@pytest.mark.parametrize("limit", KIND_LIMITS, ids=lambda t: f"2^{t.bit_length() - 1}")
+def test_kernel_kind_limits_vs_oracle(ctx, ctx_plain, oracle, limit):
+    """Head ranges whose largest modulus sits just below each limit at which the host changes the
+    engine (fp64 3-, 2- and 1-bit windows below 2^42, 2^46 and 2^48; lazy Montgomery residues below
+    2^55; reduced residues below 2^63), so that each engine runs at the top of its residue bounds,
+    and ranges that straddle the limit, so that the next engine takes moduli just above it.  4096
+    terms, so that full tiles take the fast term loop; d = 3 k, so that the exponents are long."""
+    for j in (0, 2, 6):
+        k_top = (limit - 1 - DEN1[j]) // DEN0[j]  # largest k whose modulus is below the limit
+        d = min(999999999999999999, 3 * k_top)
+        if d <= k_top:
+            continue  # 4k + 1 reaches 2^63 only beyond the largest position
+        for k0, k1 in ((k_top - 4095, k_top + 1), (k_top - 2047, k_top + 2049)):
+            if DEN0[j] * (k1 - 1) + DEN1[j] >= 1 << 63:
+                continue
+            want = oracle.batch_calc(d, k0, k1, j)
+            for name, c in (("auto", ctx), ("plain", ctx_plain)):
+                note(f"limit 2^{limit.bit_length() - 1}: d={d} k=[{k0},{k1}) j={j} {name}")
+                c.set0mem()
+                c.batch_calc(d, k0, k1, j)
+                assert w2i(c.read_partial()) == want, (limit, d, k0, k1, j)
+
+
 def test_split_rejects_large_moduli(ctx_split):
     d = 10**15 + 1  # moduli 10k + 1 near k = d exceed 2^49
     with pytest.raises(clb.ClbError) as e:
